@@ -1,0 +1,830 @@
+// hc_api.cpp -- the C ABI of libhcgpu (include/hemocell_gpu.h): device state,
+// the time step in the order of HemoCell::iterate() (core/hemoCell.cpp:299-376),
+// state access and timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hc_launch.h"
+
+using namespace hcg;
+
+namespace {
+
+struct TypeBufs {
+  TypeHost host;
+  std::vector<void*> allocs;
+  TypeDev dev{};
+  int32_t* d_cells = nullptr;  // cells of this type
+  int ncells = 0;
+};
+
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  double total_ms[HC_PROF_NSTAGES] = {0};
+  int64_t count[HC_PROF_NSTAGES] = {0};
+};
+
+}  // namespace
+
+struct hc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // lattice
+  bool lattice_ready = false;
+  LatticeDev L{};
+  int precision = HC_FP64;
+  double tau = 1.0;
+  void* f = nullptr;
+  void* Fibm[2] = {nullptr, nullptr};
+  uint8_t* type = nullptr;
+  uint32_t* vel_nodes = nullptr;  // nodes with a velocity boundary
+  int n_vel = 0;
+  bool general = false;
+  int cur = 0;
+  bool other_clean = true;
+  int phase = 0;  // AA layout: 0 natural, 1 after an even step
+  double g[3] = {0, 0, 0};
+  double wall_u[6][3] = {{0}};
+  // cells
+  std::vector<std::unique_ptr<TypeBufs>> types;
+  std::vector<int32_t> cell_type, cell_id;
+  std::vector<int64_t> cell_base;
+  int64_t nv = 0, vcap = 0;
+  double *x = nullptr, *v = nullptr, *fv = nullptr, *frep = nullptr, *bend = nullptr;
+  int64_t* d_cell_base = nullptr;
+  bool cells_dirty = true;
+  // extra (stretch) forces
+  int64_t* d_extra_ids = nullptr;
+  double* d_extra_f = nullptr;
+  int n_extra = 0;
+  // controls
+  int64_t iter = 0;
+  int pv_timescale = 1;
+  double f_limit = 1e300;
+  bool repulsion = false;
+  double rep_k = 0, rep_cutoff = 0;
+  int rep_timescale = 1;
+  unsigned long long* d_wall_hits = nullptr;
+  unsigned int* d_flag = nullptr;
+  Prof prof;
+};
+
+namespace {
+
+hc_status fail(hc_ctx* c, hc_status s, const std::string& m) {
+  if (c) c->err = m;
+  return s;
+}
+hc_status hipfail(hc_ctx* c, hipError_t e, const char* where) {
+  return fail(c, HC_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+#define HC_TRY(expr, where)                  \
+  do {                                       \
+    hipError_t e_ = (expr);                  \
+    if (e_ != hipSuccess) return hipfail(ctx, e_, where); \
+  } while (0)
+
+size_t rsize(const hc_ctx* c) { return c->precision == HC_FP32_SHIFTED ? sizeof(float) : sizeof(double); }
+
+hc_status dalloc(hc_ctx* ctx, void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, HC_OOM, std::string("device allocation of ") + what + " (" + std::to_string(bytes) + " bytes) failed");
+  }
+  return HC_OK;
+}
+
+VertexDev vdev(hc_ctx* c) { return VertexDev{c->x, c->v, c->fv, c->frep, c->nv, c->vcap}; }
+
+void prof_begin(hc_ctx* c, int stage, hipEvent_t* start) {
+  *start = nullptr;
+  if (!c->prof.on) return;
+  if (c->prof.used + 2 > c->prof.pool.size()) {
+    for (int k = 0; k < 256; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      c->prof.pool.push_back(e);
+    }
+  }
+  *start = c->prof.pool[c->prof.used++];
+  hipEventRecord(*start, c->stream);
+  (void)stage;
+}
+void prof_end(hc_ctx* c, int stage, hipEvent_t start) {
+  if (!c->prof.on || !start) return;
+  hipEvent_t e = c->prof.pool[c->prof.used++];
+  hipEventRecord(e, c->stream);
+  c->prof.recs.push_back({stage, start, e});
+}
+// fold recorded event pairs into the totals (synchronises the stream)
+void prof_flush(hc_ctx* c) {
+  if (c->prof.recs.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& r : c->prof.recs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      c->prof.total_ms[r.stage] += ms;
+      c->prof.count[r.stage] += 1;
+    }
+  }
+  c->prof.recs.clear();
+  c->prof.used = 0;
+}
+
+hc_status need_lattice(hc_ctx* ctx) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (!ctx->lattice_ready) return fail(ctx, HC_STATE, "lattice not initialised (call hc_lattice_init first)");
+  return HC_OK;
+}
+
+// grow the vertex arrays to hold `need` vertices (keeps the contents)
+hc_status grow_vertices(hc_ctx* ctx, int64_t need) {
+  if (need <= ctx->vcap) return HC_OK;
+  int64_t cap = std::max<int64_t>(need, ctx->vcap * 2);
+  cap = (cap + 63) / 64 * 64;
+  double* arrs[5] = {ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend};
+  double* nw[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int a = 0; a < 5; ++a) {
+    hc_status s = dalloc(ctx, (void**)&nw[a], sizeof(double) * 3 * cap, "vertex state");
+    if (s != HC_OK) {
+      for (int b = 0; b < a; ++b) hipFree(nw[b]);
+      return s;
+    }
+    HC_TRY(hipMemsetAsync(nw[a], 0, sizeof(double) * 3 * cap, ctx->stream), "hipMemsetAsync");
+    if (arrs[a] && ctx->nv > 0)
+      for (int c = 0; c < 3; ++c)
+        HC_TRY(hipMemcpyAsync(nw[a] + c * cap, arrs[a] + c * ctx->vcap, sizeof(double) * ctx->nv,
+                              hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
+  }
+  HC_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  for (int a = 0; a < 5; ++a)
+    if (arrs[a]) hipFree(arrs[a]);
+  ctx->x = nw[0];
+  ctx->v = nw[1];
+  ctx->fv = nw[2];
+  ctx->frep = nw[3];
+  ctx->bend = nw[4];
+  ctx->vcap = cap;
+  return HC_OK;
+}
+
+// upload per-type cell lists and cell bases after cells were added
+hc_status sync_cells(hc_ctx* ctx) {
+  if (!ctx->cells_dirty) return HC_OK;
+  if (ctx->d_cell_base) hipFree(ctx->d_cell_base);
+  ctx->d_cell_base = nullptr;
+  size_t nc = ctx->cell_base.size();
+  hc_status s = dalloc(ctx, (void**)&ctx->d_cell_base, sizeof(int64_t) * std::max<size_t>(nc, 1), "cell bases");
+  if (s != HC_OK) return s;
+  if (nc) HC_TRY(hipMemcpy(ctx->d_cell_base, ctx->cell_base.data(), sizeof(int64_t) * nc, hipMemcpyHostToDevice), "hipMemcpy");
+  for (size_t t = 0; t < ctx->types.size(); ++t) {
+    TypeBufs& tb = *ctx->types[t];
+    std::vector<int32_t> list;
+    for (size_t c = 0; c < nc; ++c)
+      if (ctx->cell_type[c] == (int)t) list.push_back((int32_t)c);
+    if (tb.d_cells) hipFree(tb.d_cells);
+    tb.d_cells = nullptr;
+    s = dalloc(ctx, (void**)&tb.d_cells, sizeof(int32_t) * std::max<size_t>(list.size(), 1), "cell list");
+    if (s != HC_OK) return s;
+    if (!list.empty())
+      HC_TRY(hipMemcpy(tb.d_cells, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    tb.ncells = (int)list.size();
+  }
+  ctx->cells_dirty = false;
+  return HC_OK;
+}
+
+template <typename T>
+hc_status upload(hc_ctx* ctx, TypeBufs& tb, const std::vector<T>& h, const T** out) {
+  void* p = nullptr;
+  hc_status s = dalloc(ctx, &p, sizeof(T) * std::max<size_t>(h.size(), 1), "cell type tables");
+  if (s != HC_OK) return s;
+  tb.allocs.push_back(p);
+  if (!h.empty()) HC_TRY(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  *out = (const T*)p;
+  return HC_OK;
+}
+
+template <typename R>
+hc_status do_spread(hc_ctx* ctx) {
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_SPREAD, &ev);
+  if (ctx->n_extra > 0) HC_TRY(launch_add_extra_force(ctx->stream, vdev(ctx), ctx->d_extra_ids, ctx->d_extra_f, ctx->n_extra), "extra force");
+  HC_TRY(launch_spread<R>(ctx->stream, vdev(ctx), (R*)ctx->Fibm[ctx->cur], ctx->type, ctx->L, ctx->f_limit,
+                          ctx->repulsion ? 1 : 0), "spread");
+  prof_end(ctx, HC_PROF_SPREAD, ev);
+  return HC_OK;
+}
+
+template <typename R>
+hc_status do_collide(hc_ctx* ctx) {
+  CollideArgs a;
+  a.omega = 1.0 / ctx->tau;
+  for (int c = 0; c < 3; ++c) a.g[c] = ctx->g[c];
+  std::memcpy(a.wall_u, ctx->wall_u, sizeof(a.wall_u));
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_COLLIDE, &ev);
+  HC_TRY(launch_collide_stream<R>(ctx->stream, (R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], (R*)ctx->Fibm[1 - ctx->cur],
+                                  ctx->type, ctx->L, a, ctx->phase, ctx->general, ctx->vel_nodes, ctx->n_vel),
+         "collide_stream");
+  prof_end(ctx, HC_PROF_COLLIDE, ev);
+  ctx->phase ^= 1;
+  ctx->other_clean = true;
+  return HC_OK;
+}
+
+template <typename R>
+hc_status do_interp_advance(hc_ctx* ctx, int do_interp, int do_advance) {
+  if (ctx->nv == 0) return HC_OK;
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_INTERP, &ev);
+  HC_TRY(launch_interp_advance<R>(ctx->stream, vdev(ctx), (const R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], ctx->type,
+                                  ctx->L, ctx->phase, ctx->g, do_interp, do_advance, ctx->d_wall_hits),
+         "interp_advance");
+  prof_end(ctx, HC_PROF_INTERP, ev);
+  return HC_OK;
+}
+
+hc_status do_mechanics(hc_ctx* ctx, bool forced) {
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_MECH, &ev);
+  for (auto& tp : ctx->types) {
+    TypeBufs& tb = *tp;
+    if (tb.ncells == 0) continue;
+    if (!forced && (ctx->iter % tb.host.timescale) != 0) continue;
+    HC_TRY(launch_mechanics(ctx->stream, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base, ctx->bend),
+           "mechanics");
+  }
+  prof_end(ctx, HC_PROF_MECH, ev);
+  return HC_OK;
+}
+
+template <typename R>
+hc_status do_end(hc_ctx* ctx) {
+  if (!ctx->other_clean) {
+    HC_TRY(launch_fill_zero<R>(ctx->stream, (R*)ctx->Fibm[1 - ctx->cur], 3 * ctx->L.stride), "zero force");
+  }
+  // the buffer just used is never read again before the collide that zeroes it
+  ctx->cur ^= 1;
+  ctx->other_clean = false;
+  ctx->iter++;
+  return HC_OK;
+}
+
+template <typename R>
+hc_status step_impl(hc_ctx* ctx, int nsteps) {
+  hc_status s;
+  for (int k = 0; k < nsteps; ++k) {
+    hipEvent_t ev;
+    prof_begin(ctx, HC_PROF_STEP, &ev);
+    if (ctx->repulsion && ctx->iter % ctx->rep_timescale == 0) {
+      s = hc_stage_repulsion(ctx);
+      if (s != HC_OK) return s;
+    }
+    if (ctx->nv > 0 && (s = do_spread<R>(ctx)) != HC_OK) return s;
+    if ((s = do_collide<R>(ctx)) != HC_OK) return s;
+    if ((s = do_interp_advance<R>(ctx, ctx->iter % ctx->pv_timescale == 0, 1)) != HC_OK) return s;
+    if ((s = do_mechanics(ctx, false)) != HC_OK) return s;
+    if ((s = do_end<R>(ctx)) != HC_OK) return s;
+    prof_end(ctx, HC_PROF_STEP, ev);
+    if (ctx->prof.on && ctx->prof.used > 4096) prof_flush(ctx);
+  }
+  return HC_OK;
+}
+
+bool fp32(const hc_ctx* c) { return c->precision == HC_FP32_SHIFTED; }
+
+hc_status ready_for_step(hc_ctx* ctx) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  return sync_cells(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+hc_status hc_create(int device, hc_ctx** out) {
+  if (!out) return HC_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return HC_DEVICE;
+  if (device < 0 || device >= n) return HC_INVALID_ARG;
+  if (hipSetDevice(device) != hipSuccess) return HC_DEVICE;
+  hc_ctx* ctx = new hc_ctx();
+  ctx->device = device;
+  if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    return HC_DEVICE;
+  }
+  if (hipMalloc((void**)&ctx->d_wall_hits, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc((void**)&ctx->d_flag, sizeof(unsigned int)) != hipSuccess) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HC_OOM;
+  }
+  hipMemset(ctx->d_wall_hits, 0, sizeof(unsigned long long));
+  *out = ctx;
+  return HC_OK;
+}
+
+void hc_destroy(hc_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
+                ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag};
+  for (void* p : ps)
+    if (p) hipFree(p);
+  for (auto& tp : ctx->types) {
+    for (void* p : tp->allocs) hipFree(p);
+    if (tp->d_cells) hipFree(tp->d_cells);
+  }
+  for (hipEvent_t e : ctx->prof.pool) hipEventDestroy(e);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* hc_last_error(const hc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* node_type, double tau,
+                          const int periodic[3], int precision) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (ctx->lattice_ready) return fail(ctx, HC_STATE, "lattice already initialised");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(ctx, HC_INVALID_ARG, "lattice sizes must be >= 1");
+  if (!(tau > 0.5)) return fail(ctx, HC_INVALID_ARG, "tau must be > 0.5");
+  if (precision != HC_FP64 && precision != HC_FP32_SHIFTED) return fail(ctx, HC_INVALID_ARG, "unknown precision");
+  const int64_t N = (int64_t)nx * ny * nz;
+  if (N >= ((int64_t)1 << 31)) return fail(ctx, HC_INVALID_ARG, "lattice too large (at most 2^31-1 nodes)");
+  hipSetDevice(ctx->device);
+  ctx->precision = precision;
+  ctx->tau = tau;
+  ctx->L.nx = nx;
+  ctx->L.ny = ny;
+  ctx->L.nz = nz;
+  bool all_periodic = true;
+  for (int a = 0; a < 3; ++a) {
+    ctx->L.periodic[a] = periodic ? (periodic[a] != 0) : 1;
+    all_periodic &= ctx->L.periodic[a] != 0;
+  }
+  ctx->L.N = N;
+  ctx->L.stride = (N + 255) / 256 * 256;
+  bool has_walls = false;
+  std::vector<uint32_t> vel;
+  if (node_type) {
+    for (int64_t n = 0; n < N; ++n) {
+      uint8_t t = node_type[n];
+      if (t > HC_NODE_VELOCITY + 5) return fail(ctx, HC_INVALID_ARG, "unknown node type in mask");
+      if (t != HC_NODE_FLUID) has_walls = true;
+      if (t >= HC_NODE_VELOCITY) {
+        // a velocity node must sit on the domain face it names
+        const int face = t - HC_NODE_VELOCITY, axis = face >> 1;
+        const int64_t z = n % nz, y = (n / nz) % ny, x = n / ((int64_t)ny * nz);
+        const int64_t c = axis == 0 ? x : (axis == 1 ? y : z);
+        const int64_t last = (axis == 0 ? nx : (axis == 1 ? ny : nz)) - 1;
+        if (c != ((face & 1) ? last : 0))
+          return fail(ctx, HC_INVALID_ARG, "velocity-boundary node not on its domain face");
+        vel.push_back((uint32_t)n);
+      }
+    }
+  }
+  ctx->general = has_walls || !all_periodic;
+  const size_t rs = rsize(ctx);
+  hc_status s;
+  if ((s = dalloc(ctx, &ctx->f, rs * 19 * ctx->L.stride, "populations")) != HC_OK) return s;
+  for (int b = 0; b < 2; ++b)
+    if ((s = dalloc(ctx, &ctx->Fibm[b], rs * 3 * ctx->L.stride, "force field")) != HC_OK) return s;
+  HC_TRY(hipMemsetAsync(ctx->f, 0, rs * 19 * ctx->L.stride, ctx->stream), "hipMemsetAsync");
+  for (int b = 0; b < 2; ++b) HC_TRY(hipMemsetAsync(ctx->Fibm[b], 0, rs * 3 * ctx->L.stride, ctx->stream), "hipMemsetAsync");
+  if (has_walls) {
+    if ((s = dalloc(ctx, (void**)&ctx->type, N, "node types")) != HC_OK) return s;
+    HC_TRY(hipMemcpy(ctx->type, node_type, N, hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  if (!vel.empty()) {
+    if ((s = dalloc(ctx, (void**)&ctx->vel_nodes, sizeof(uint32_t) * vel.size(), "velocity nodes")) != HC_OK) return s;
+    HC_TRY(hipMemcpy(ctx->vel_nodes, vel.data(), sizeof(uint32_t) * vel.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    ctx->n_vel = (int)vel.size();
+  }
+  HC_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  ctx->lattice_ready = true;
+  ctx->phase = 0;
+  ctx->cur = 0;
+  ctx->other_clean = true;
+  return HC_OK;
+}
+
+hc_status hc_set_wall_velocity(hc_ctx* ctx, int face, const double u[3]) {
+  if (!ctx || !u) return HC_INVALID_ARG;
+  if (face < 0 || face > 5) return fail(ctx, HC_INVALID_ARG, "face must be 0..5");
+  for (int c = 0; c < 3; ++c) ctx->wall_u[face][c] = u[c];
+  return HC_OK;
+}
+
+hc_status hc_set_body_force(hc_ctx* ctx, const double g[3]) {
+  if (!ctx || !g) return HC_INVALID_ARG;
+  for (int c = 0; c < 3; ++c) ctx->g[c] = g[c];
+  return HC_OK;
+}
+
+hc_status hc_init_equilibrium(hc_ctx* ctx, double rho, const double u[3]) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!(rho > 0.0)) return fail(ctx, HC_INVALID_ARG, "rho must be > 0");
+  const double zero[3] = {0, 0, 0};
+  const double* uu = u ? u : zero;
+  if (fp32(ctx)) HC_TRY(launch_init_equilibrium<float>(ctx->stream, (float*)ctx->f, ctx->type, ctx->L, rho, uu), "init");
+  else HC_TRY(launch_init_equilibrium<double>(ctx->stream, (double*)ctx->f, ctx->type, ctx->L, rho, uu), "init");
+  ctx->phase = 0;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  return HC_OK;
+}
+
+hc_status hc_build_mesh(int shape, double radius_lu, int min_triangles, double aspect_ratio, int* nv, int* nt,
+                        double* vertices, int32_t* triangles) {
+  std::vector<double> v;
+  std::vector<int32_t> t;
+  std::string err;
+  if (!build_mesh(shape, radius_lu, min_triangles, aspect_ratio, v, t, err)) return HC_INVALID_ARG;
+  if (nv) *nv = (int)(v.size() / 3);
+  if (nt) *nt = (int)(t.size() / 3);
+  if (vertices) std::copy(v.begin(), v.end(), vertices);
+  if (triangles) std::copy(t.begin(), t.end(), triangles);
+  return HC_OK;
+}
+
+hc_status hc_add_cell_type(hc_ctx* ctx, int model, int nv, int nt, const int32_t* triangles, const double* rest,
+                           int n_inner, const int32_t* inner, const hc_model_params* params, int timescale,
+                           int* type_id) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (model < HC_MODEL_RBC || model > HC_MODEL_WBC) return fail(ctx, HC_INVALID_ARG, "unknown model");
+  if (nv < 4 || nt < 4 || !triangles || !rest || !params) return fail(ctx, HC_INVALID_ARG, "bad mesh arguments");
+  if (n_inner < 0 || (n_inner > 0 && !inner)) return fail(ctx, HC_INVALID_ARG, "bad inner edges");
+  auto tb = std::make_unique<TypeBufs>();
+  TypeHost& T = tb->host;
+  T.model = model;
+  T.nv = nv;
+  T.nt = nt;
+  T.tri.assign(triangles, triangles + 3 * nt);
+  T.rest.assign(rest, rest + 3 * nv);
+  if (n_inner > 0) T.inner.assign(inner, inner + 2 * n_inner);
+  T.p = *params;
+  T.timescale = timescale > 0 ? timescale : 1;
+  std::string err;
+  if (!build_type_tables(T, err)) return fail(ctx, HC_INVALID_ARG, "cell type: " + err);
+  TypeDev& D = tb->dev;
+  hc_status s;
+#define UP(field, dst) \
+  if ((s = upload(ctx, *tb, T.field, &D.dst)) != HC_OK) return s;
+  UP(tri, tri) UP(edge, edge) UP(edge_tris, edge_tris) UP(edge_outer, edge_outer) UP(ring, ring) UP(nring, nring)
+  UP(inner, inner) UP(edge_len_eq, edge_len_eq) UP(edge_angle_eq, edge_angle_eq) UP(tri_area_eq, tri_area_eq)
+  UP(patch_dist_eq, patch_dist_eq) UP(inner_len_eq, inner_len_eq) UP(vtri_off, vtri_off) UP(vtri, vtri)
+  UP(vbend_off, vbend_off) UP(vbend, vbend) UP(vedge_off, vedge_off) UP(vedge, vedge) UP(vinner_off, vinner_off)
+  UP(vinner, vinner)
+#undef UP
+  D.model = model;
+  D.nv = T.nv;
+  D.nt = T.nt;
+  D.ne = T.ne;
+  D.ni = T.ni;
+  D.volume_eq = T.volume_eq;
+  D.area_mean_eq = T.area_mean_eq;
+  D.edge_mean_eq = T.edge_mean_eq;
+  D.k_link = params->k_link;
+  D.k_bend = params->k_bend;
+  D.k_volume = params->k_volume;
+  D.k_area = params->k_area;
+  D.eta_m = params->eta_m;
+  D.k_inner_rigid = params->k_inner_rigid;
+  D.k_cyto = params->k_cytoskeleton;
+  D.core_radius = params->core_radius;
+  D.radius = params->radius;
+  ctx->types.push_back(std::move(tb));
+  if (type_id) *type_id = (int)ctx->types.size() - 1;
+  return HC_OK;
+}
+
+hc_status hc_add_cells(hc_ctx* ctx, int type_id, int ncells, const double* xyz, const int32_t* cell_ids) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (type_id < 0 || type_id >= (int)ctx->types.size()) return fail(ctx, HC_INVALID_ARG, "unknown cell type");
+  if (ncells < 0 || (ncells > 0 && !xyz)) return fail(ctx, HC_INVALID_ARG, "bad cell arguments");
+  if (ncells == 0) return HC_OK;
+  const int nvt = ctx->types[type_id]->host.nv;
+  const int64_t add = (int64_t)ncells * nvt;
+  hc_status s = grow_vertices(ctx, ctx->nv + add);
+  if (s != HC_OK) return s;
+  std::vector<double> plane(add);
+  for (int c = 0; c < 3; ++c) {
+    for (int64_t k = 0; k < add; ++k) plane[k] = xyz[3 * k + c];
+    HC_TRY(hipMemcpy(ctx->x + c * ctx->vcap + ctx->nv, plane.data(), sizeof(double) * add, hipMemcpyHostToDevice),
+           "hipMemcpy");
+  }
+  for (int c = 0; c < ncells; ++c) {
+    ctx->cell_base.push_back(ctx->nv + (int64_t)c * nvt);
+    ctx->cell_type.push_back(type_id);
+    ctx->cell_id.push_back(cell_ids ? cell_ids[c] : (int32_t)ctx->cell_id.size());
+  }
+  ctx->nv += add;
+  ctx->cells_dirty = true;
+  return HC_OK;
+}
+
+hc_status hc_set_velocity_timescale(hc_ctx* ctx, int ts) {
+  if (!ctx || ts < 1) return fail(ctx, HC_INVALID_ARG, "timescale must be >= 1");
+  ctx->pv_timescale = ts;
+  return HC_OK;
+}
+hc_status hc_set_material_timescale(hc_ctx* ctx, int type_id, int ts) {
+  if (!ctx || ts < 1) return fail(ctx, HC_INVALID_ARG, "timescale must be >= 1");
+  if (type_id < 0 || type_id >= (int)ctx->types.size()) return fail(ctx, HC_INVALID_ARG, "unknown cell type");
+  ctx->types[type_id]->host.timescale = ts;
+  return HC_OK;
+}
+hc_status hc_set_force_limit(hc_ctx* ctx, double f_limit) {
+  if (!ctx || !(f_limit > 0)) return fail(ctx, HC_INVALID_ARG, "force limit must be > 0");
+  ctx->f_limit = f_limit;
+  return HC_OK;
+}
+hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale) {
+  if (!ctx) return HC_INVALID_ARG;
+  (void)k_rep;
+  (void)cutoff_lu;
+  (void)timescale;
+  return fail(ctx, HC_STATE, "cell-cell repulsion is not available in this build");
+}
+
+hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* ids, const double* forces) {
+  if (!ctx || n < 0 || (n > 0 && (!ids || !forces))) return fail(ctx, HC_INVALID_ARG, "bad extra-force arguments");
+  if (ctx->d_extra_ids) hipFree(ctx->d_extra_ids);
+  if (ctx->d_extra_f) hipFree(ctx->d_extra_f);
+  ctx->d_extra_ids = nullptr;
+  ctx->d_extra_f = nullptr;
+  ctx->n_extra = 0;
+  if (n == 0) return HC_OK;
+  for (int k = 0; k < n; ++k)
+    if (ids[k] < 0 || ids[k] >= ctx->nv) return fail(ctx, HC_INVALID_ARG, "extra-force vertex id out of range");
+  hc_status s;
+  if ((s = dalloc(ctx, (void**)&ctx->d_extra_ids, sizeof(int64_t) * n, "extra ids")) != HC_OK) return s;
+  if ((s = dalloc(ctx, (void**)&ctx->d_extra_f, sizeof(double) * 3 * n, "extra forces")) != HC_OK) return s;
+  HC_TRY(hipMemcpy(ctx->d_extra_ids, ids, sizeof(int64_t) * n, hipMemcpyHostToDevice), "hipMemcpy");
+  HC_TRY(hipMemcpy(ctx->d_extra_f, forces, sizeof(double) * 3 * n, hipMemcpyHostToDevice), "hipMemcpy");
+  ctx->n_extra = n;
+  return HC_OK;
+}
+
+hc_status hc_step(hc_ctx* ctx, int nsteps) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  if (nsteps < 0) return fail(ctx, HC_INVALID_ARG, "nsteps must be >= 0");
+  return fp32(ctx) ? step_impl<float>(ctx, nsteps) : step_impl<double>(ctx, nsteps);
+}
+
+hc_status hc_synchronize(hc_ctx* ctx) {
+  if (!ctx) return HC_INVALID_ARG;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return HC_OK;
+}
+
+int64_t hc_iteration(const hc_ctx* ctx) { return ctx ? ctx->iter : -1; }
+
+hc_status hc_stage_spread(hc_ctx* ctx) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  if (ctx->nv == 0) return HC_OK;
+  return fp32(ctx) ? do_spread<float>(ctx) : do_spread<double>(ctx);
+}
+hc_status hc_stage_collide_stream(hc_ctx* ctx) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  return fp32(ctx) ? do_collide<float>(ctx) : do_collide<double>(ctx);
+}
+hc_status hc_stage_interpolate(hc_ctx* ctx) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  return fp32(ctx) ? do_interp_advance<float>(ctx, 1, 0) : do_interp_advance<double>(ctx, 1, 0);
+}
+hc_status hc_stage_advance(hc_ctx* ctx) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  return fp32(ctx) ? do_interp_advance<float>(ctx, 0, 1) : do_interp_advance<double>(ctx, 0, 1);
+}
+hc_status hc_stage_mechanics(hc_ctx* ctx, int forced) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  return do_mechanics(ctx, forced != 0);
+}
+hc_status hc_stage_repulsion(hc_ctx* ctx) {
+  if (!ctx) return HC_INVALID_ARG;
+  return fail(ctx, HC_STATE, "cell-cell repulsion is not available in this build");
+}
+hc_status hc_stage_end(hc_ctx* ctx) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  return fp32(ctx) ? do_end<float>(ctx) : do_end<double>(ctx);
+}
+
+int64_t hc_num_nodes(const hc_ctx* ctx) { return ctx && ctx->lattice_ready ? ctx->L.N : 0; }
+int64_t hc_num_vertices(const hc_ctx* ctx) { return ctx ? ctx->nv : 0; }
+int hc_num_cells(const hc_ctx* ctx) { return ctx ? (int)ctx->cell_base.size() : 0; }
+
+hc_status hc_get_populations(hc_ctx* ctx, double* f) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!f) return fail(ctx, HC_INVALID_ARG, "null output");
+  double* d = nullptr;
+  if ((s = dalloc(ctx, (void**)&d, sizeof(double) * 19 * ctx->L.N, "population readback")) != HC_OK) return s;
+  hipError_t e = fp32(ctx) ? launch_natural_populations<float>(ctx->stream, (float*)ctx->f, d, ctx->L, ctx->phase)
+                           : launch_natural_populations<double>(ctx->stream, (double*)ctx->f, d, ctx->L, ctx->phase);
+  if (e == hipSuccess) e = hipMemcpyAsync(f, d, sizeof(double) * 19 * ctx->L.N, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return hipfail(ctx, e, "get_populations");
+  return HC_OK;
+}
+
+hc_status hc_set_populations(hc_ctx* ctx, const double* f) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!f) return fail(ctx, HC_INVALID_ARG, "null input");
+  double* d = nullptr;
+  if ((s = dalloc(ctx, (void**)&d, sizeof(double) * 19 * ctx->L.N, "population upload")) != HC_OK) return s;
+  hipError_t e = hipMemcpyAsync(d, f, sizeof(double) * 19 * ctx->L.N, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = fp32(ctx) ? launch_store_populations<float>(ctx->stream, (float*)ctx->f, d, ctx->L)
+                  : launch_store_populations<double>(ctx->stream, (double*)ctx->f, d, ctx->L);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return hipfail(ctx, e, "set_populations");
+  ctx->phase = 0;
+  return HC_OK;
+}
+
+hc_status hc_get_force_field(hc_ctx* ctx, double* F) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!F) return fail(ctx, HC_INVALID_ARG, "null output");
+  const int64_t S = ctx->L.stride, N = ctx->L.N;
+  double* d = nullptr;
+  if ((s = dalloc(ctx, (void**)&d, sizeof(double) * 3 * S, "force readback")) != HC_OK) return s;
+  hipError_t e = fp32(ctx) ? launch_to_double<float>(ctx->stream, (const float*)ctx->Fibm[ctx->cur], d, 3 * S)
+                           : launch_to_double<double>(ctx->stream, (const double*)ctx->Fibm[ctx->cur], d, 3 * S);
+  for (int c = 0; c < 3 && e == hipSuccess; ++c)
+    e = hipMemcpyAsync(F + c * N, d + c * S, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return hipfail(ctx, e, "get_force_field");
+  return HC_OK;
+}
+
+hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  const int64_t N = ctx->L.N;
+  double *dr = nullptr, *du = nullptr;
+  if (rho && (s = dalloc(ctx, (void**)&dr, sizeof(double) * N, "rho readback")) != HC_OK) return s;
+  if (u && (s = dalloc(ctx, (void**)&du, sizeof(double) * 3 * N, "u readback")) != HC_OK) {
+    if (dr) hipFree(dr);
+    return s;
+  }
+  hipError_t e = fp32(ctx) ? launch_fluid_moments<float>(ctx->stream, (const float*)ctx->f,
+                                                         (const float*)ctx->Fibm[ctx->cur], ctx->L, ctx->phase, ctx->g, dr, du)
+                           : launch_fluid_moments<double>(ctx->stream, (const double*)ctx->f,
+                                                          (const double*)ctx->Fibm[ctx->cur], ctx->L, ctx->phase, ctx->g, dr, du);
+  if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, dr, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && u) e = hipMemcpyAsync(u, du, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (dr) hipFree(dr);
+  if (du) hipFree(du);
+  if (e != hipSuccess) return hipfail(ctx, e, "get_fluid");
+  return HC_OK;
+}
+
+static hc_status get_planes(hc_ctx* ctx, const double* d, double* out) {
+  const int64_t nv = ctx->nv;
+  std::vector<double> plane(nv);
+  for (int c = 0; c < 3; ++c) {
+    HC_TRY(hipMemcpy(plane.data(), d + c * ctx->vcap, sizeof(double) * nv, hipMemcpyDeviceToHost), "hipMemcpy");
+    for (int64_t k = 0; k < nv; ++k) out[3 * k + c] = plane[k];
+  }
+  return HC_OK;
+}
+static hc_status set_planes(hc_ctx* ctx, double* d, const double* in) {
+  const int64_t nv = ctx->nv;
+  std::vector<double> plane(nv);
+  for (int c = 0; c < 3; ++c) {
+    for (int64_t k = 0; k < nv; ++k) plane[k] = in[3 * k + c];
+    HC_TRY(hipMemcpy(d + c * ctx->vcap, plane.data(), sizeof(double) * nv, hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  return HC_OK;
+}
+
+hc_status hc_get_vertices(hc_ctx* ctx, double* x, double* v, double* f) {
+  if (!ctx) return HC_INVALID_ARG;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  hc_status s;
+  if (x && (s = get_planes(ctx, ctx->x, x)) != HC_OK) return s;
+  if (v && (s = get_planes(ctx, ctx->v, v)) != HC_OK) return s;
+  if (f && (s = get_planes(ctx, ctx->fv, f)) != HC_OK) return s;
+  return HC_OK;
+}
+hc_status hc_set_vertices(hc_ctx* ctx, const double* x, const double* v, const double* f) {
+  if (!ctx) return HC_INVALID_ARG;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  hc_status s;
+  if (x && (s = set_planes(ctx, ctx->x, x)) != HC_OK) return s;
+  if (v && (s = set_planes(ctx, ctx->v, v)) != HC_OK) return s;
+  if (f && (s = set_planes(ctx, ctx->fv, f)) != HC_OK) return s;
+  return HC_OK;
+}
+hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep) {
+  if (!ctx || !frep) return HC_INVALID_ARG;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  return get_planes(ctx, ctx->frep, frep);
+}
+
+hc_status hc_get_cell_volume_area(hc_ctx* ctx, double* volume, double* area) {
+  if (!ctx) return HC_INVALID_ARG;
+  hc_status s = sync_cells(ctx);
+  if (s != HC_OK) return s;
+  const int nc = (int)ctx->cell_base.size();
+  if (nc == 0) return HC_OK;
+  double *dv = nullptr, *da = nullptr;
+  if ((s = dalloc(ctx, (void**)&dv, sizeof(double) * nc, "volume")) != HC_OK) return s;
+  if ((s = dalloc(ctx, (void**)&da, sizeof(double) * nc, "area")) != HC_OK) {
+    hipFree(dv);
+    return s;
+  }
+  hipError_t e = hipSuccess;
+  for (auto& tp : ctx->types)
+    if (e == hipSuccess && tp->ncells)
+      e = launch_cell_volume_area(ctx->stream, vdev(ctx), tp->dev, tp->d_cells, tp->ncells, ctx->d_cell_base, dv, da);
+  std::vector<double> hv(nc), ha(nc);
+  if (e == hipSuccess) e = hipMemcpyAsync(hv.data(), dv, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ha.data(), da, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(dv);
+  hipFree(da);
+  if (e != hipSuccess) return hipfail(ctx, e, "cell volume/area");
+  if (volume) std::copy(hv.begin(), hv.end(), volume);
+  if (area) std::copy(ha.begin(), ha.end(), area);
+  return HC_OK;
+}
+
+int64_t hc_wall_hits(hc_ctx* ctx) {
+  if (!ctx) return -1;
+  unsigned long long h = 0;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
+  if (hipMemcpy(&h, ctx->d_wall_hits, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return (int64_t)h;
+}
+
+hc_status hc_check_finite(hc_ctx* ctx) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  HC_TRY(hipMemsetAsync(ctx->d_flag, 0, sizeof(unsigned int), ctx->stream), "memset");
+  if (fp32(ctx)) HC_TRY(launch_check_finite_f(ctx->stream, (const float*)ctx->f, 19 * ctx->L.stride, ctx->d_flag), "check");
+  else HC_TRY(launch_check_finite(ctx->stream, (const double*)ctx->f, 19 * ctx->L.stride, ctx->d_flag), "check");
+  if (ctx->nv) HC_TRY(launch_check_finite(ctx->stream, ctx->x, 3 * ctx->vcap, ctx->d_flag), "check");
+  unsigned int bad = 0;
+  HC_TRY(hipMemcpyAsync(&bad, ctx->d_flag, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream), "copy");
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  return bad ? fail(ctx, HC_NUMERIC, "non-finite population or vertex coordinate") : HC_OK;
+}
+
+hc_status hc_profile_enable(hc_ctx* ctx, int enable) {
+  if (!ctx) return HC_INVALID_ARG;
+  prof_flush(ctx);
+  ctx->prof.on = enable != 0;
+  for (int k = 0; k < HC_PROF_NSTAGES; ++k) {
+    ctx->prof.total_ms[k] = 0;
+    ctx->prof.count[k] = 0;
+  }
+  return HC_OK;
+}
+
+hc_status hc_profile_read(hc_ctx* ctx, int stage, double* total_ms, int64_t* count) {
+  if (!ctx || stage < 0 || stage >= HC_PROF_NSTAGES) return HC_INVALID_ARG;
+  prof_flush(ctx);
+  if (total_ms) *total_ms = ctx->prof.total_ms[stage];
+  if (count) *count = ctx->prof.count[stage];
+  return HC_OK;
+}
+
+}  // extern "C"

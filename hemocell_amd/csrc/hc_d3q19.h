@@ -1,0 +1,64 @@
+// hc_d3q19.h -- D3Q19 lattice constants and per-node device helpers.
+//
+// Velocity set and ordering are Palabos's ForcedD3Q19Descriptor
+// (config/constant_defaults.h:58-61): c_0 = 0, c_1..9 as below, c_{i+9} = -c_i
+// (the opposite-index rule i <-> i+9 of patch/palabos.patch:493-520).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "hc_internal.h"
+
+namespace hcg {
+
+__host__ __device__ constexpr int cx(int i) {
+  return i == 0 ? 0 : (i <= 9 ? (i == 1 || (i >= 4 && i <= 7) ? -1 : 0) : -cx(i - 9));
+}
+__host__ __device__ constexpr int cy(int i) {
+  return i == 0 ? 0
+                : (i <= 9 ? (i == 2 || i == 4 || i == 8 || i == 9 ? -1 : (i == 5 ? 1 : 0)) : -cy(i - 9));
+}
+__host__ __device__ constexpr int cz(int i) {
+  return i == 0 ? 0
+                : (i <= 9 ? (i == 3 || i == 6 || i == 8 ? -1 : (i == 7 || i == 9 ? 1 : 0)) : -cz(i - 9));
+}
+__host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9); }
+template <typename R>
+__host__ __device__ constexpr R wgt(int i) {
+  return i == 0 ? R(1) / R(3) : (((i - 1) % 9) < 3 ? R(1) / R(18) : R(1) / R(36));
+}
+
+static_assert(cx(1) == -1 && cy(1) == 0 && cz(1) == 0, "c1");
+static_assert(cx(5) == -1 && cy(5) == 1 && cz(5) == 0, "c5");
+static_assert(cx(7) == -1 && cy(7) == 0 && cz(7) == 1, "c7");
+static_assert(cx(9) == 0 && cy(9) == -1 && cz(9) == 1, "c9");
+static_assert(cx(14) == 1 && cy(14) == -1 && cz(14) == 0, "c14");
+static_assert(cx(18) == 0 && cy(18) == 1 && cz(18) == -1, "c18");
+
+// Node coordinate d + delta along an axis of size n: wrapped when periodic,
+// -1 when it leaves a non-periodic domain.
+__host__ __device__ inline int shift_coord(int d, int delta, int n, int periodic) {
+  int s = d + delta;
+  if (s >= 0 && s < n) return s;
+  if (!periodic) return -1;
+  return s < 0 ? s + n : s - n;
+}
+
+// Density deviation and momentum from the x = -1/0/+1 partial sums (Palabos
+// momentTemplates for D3Q19), in the oracle's summation order.
+template <typename R>
+__device__ __forceinline__ void rhobar_j(const R (&f)[19], R& rhoBar, R& j0, R& j1, R& j2) {
+  R xm = f[1] + f[4] + f[5] + f[6] + f[7];
+  R x0 = f[0] + f[2] + f[3] + f[8] + f[9] + f[11] + f[12] + f[17] + f[18];
+  R xp = f[10] + f[13] + f[14] + f[15] + f[16];
+  R ym = f[2] + f[4] + f[8] + f[9] + f[14];
+  R yp = f[5] + f[11] + f[13] + f[17] + f[18];
+  R zm = f[3] + f[6] + f[8] + f[16] + f[18];
+  R zp = f[7] + f[9] + f[12] + f[15] + f[17];
+  rhoBar = xm + x0 + xp;
+  j0 = xp - xm;
+  j1 = yp - ym;
+  j2 = zp - zm;
+}
+
+}  // namespace hcg

@@ -1,0 +1,59 @@
+// hc_internal.h -- shared declarations of libhcgpu (host side and kernels).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hemocell_gpu.h"
+
+namespace hcg {
+
+// Rest-geometry tables of one cell type, in the layout the mechanics kernels read.
+// The first block mirrors CommonCellConstants (mechanics/commonCellConstants.cpp:70-409);
+// the *_off/_list pairs are per-vertex gather lists that visit every force
+// contribution in the order the reference's scatter loops add it to sv.force.
+struct TypeHost {
+  int model = HC_MODEL_RBC;
+  int nv = 0, nt = 0, ne = 0, ni = 0;
+  std::vector<int32_t> tri;         // nt*3
+  std::vector<double> rest;         // nv*3
+  std::vector<int32_t> edge;        // ne*2 (a<b, triangle order)
+  std::vector<int32_t> edge_tris;   // ne*2, adjacent triangles ascending
+  std::vector<int32_t> edge_outer;  // ne*2, vertex opposite the edge in each
+  std::vector<int32_t> ring;        // nv*6, ordered 1-ring
+  std::vector<int32_t> nring;       // nv
+  std::vector<int32_t> inner;       // ni*2
+  std::vector<double> edge_len_eq, edge_angle_eq, tri_area_eq, patch_dist_eq, inner_len_eq;
+  double volume_eq = 0, area_mean_eq = 0, edge_mean_eq = 0, angle_mean_eq = 0;
+  // gather lists (CSR): entries per vertex in accumulation order
+  std::vector<int32_t> vtri_off, vtri;      // t*4 + corner slot
+  std::vector<int32_t> vbend_off, vbend;    // contributing vertex i (own term when i == k)
+  std::vector<int32_t> vedge_off, vedge;    // e*4 + kind (0 first end, 1 second end, 2 opposite vertex)
+  std::vector<int32_t> vinner_off, vinner;  // ie*2 + end (0 first, 1 second)
+  hc_model_params p{};
+  int timescale = 1;
+};
+
+bool build_type_tables(TypeHost& t, std::string& err);
+bool build_mesh(int shape, double radius_lu, int min_triangles, double aspect_ratio,
+                std::vector<double>& verts, std::vector<int32_t>& tris, std::string& err);
+
+// Device view of one cell type (passed by value to the mechanics kernels).
+struct TypeDev {
+  int model, nv, nt, ne, ni;
+  const int32_t *tri, *edge, *edge_tris, *edge_outer, *ring, *nring, *inner;
+  const double *edge_len_eq, *edge_angle_eq, *tri_area_eq, *patch_dist_eq, *inner_len_eq;
+  const int32_t *vtri_off, *vtri, *vbend_off, *vbend, *vedge_off, *vedge, *vinner_off, *vinner;
+  double volume_eq, area_mean_eq, edge_mean_eq;
+  double k_link, k_bend, k_volume, k_area, eta_m, k_inner_rigid, k_cyto, core_radius, radius;
+};
+
+// Lattice geometry passed to every lattice / IBM kernel.
+struct LatticeDev {
+  int nx, ny, nz;
+  int periodic[3];
+  int64_t N;       // nodes
+  int64_t stride;  // population / force component stride (>= N, padded)
+};
+
+}  // namespace hcg

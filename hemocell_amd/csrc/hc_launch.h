@@ -1,0 +1,60 @@
+// hc_launch.h -- host-callable launchers of the libhcgpu kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hc_internal.h"
+
+namespace hcg {
+
+// lattice (hc_lbm.hip); R = float or double
+struct CollideArgs {
+  double omega;
+  double g[3];
+  double wall_u[6][3];
+};
+template <typename R>
+hipError_t launch_collide_stream(hipStream_t s, R* f, const R* Fcur, R* Fclear, const uint8_t* type,
+                                 const LatticeDev& L, const CollideArgs& a, int odd, bool general,
+                                 const uint32_t* vel_nodes, int n_vel);
+template <typename R>
+hipError_t launch_init_equilibrium(hipStream_t s, R* f, const uint8_t* type, const LatticeDev& L,
+                                   double rho, const double u[3]);
+template <typename R>
+hipError_t launch_natural_populations(hipStream_t s, const R* f, double* out, const LatticeDev& L, int phase);
+template <typename R>
+hipError_t launch_store_populations(hipStream_t s, R* f, const double* in, const LatticeDev& L);
+template <typename R>
+hipError_t launch_fluid_moments(hipStream_t s, const R* f, const R* Fcur, const LatticeDev& L, int phase,
+                                const double g[3], double* rho, double* u);
+template <typename R>
+hipError_t launch_fill_zero(hipStream_t s, R* p, int64_t n);
+template <typename R>
+hipError_t launch_to_double(hipStream_t s, const R* in, double* out, int64_t n);
+
+// immersed boundary (hc_ibm.hip)
+struct VertexDev {
+  double* x;      // 3 components, stride vs
+  double* v;
+  double* f;
+  double* frep;
+  int64_t n;      // vertices
+  int64_t vs;     // component stride
+};
+template <typename R>
+hipError_t launch_spread(hipStream_t s, const VertexDev& V, R* Fibm, const uint8_t* type, const LatticeDev& L,
+                         double f_limit, int use_repulsion);
+template <typename R>
+hipError_t launch_interp_advance(hipStream_t s, const VertexDev& V, const R* f, const R* Fcur, const uint8_t* type,
+                                 const LatticeDev& L, int phase, const double g[3], int do_interp, int do_advance,
+                                 unsigned long long* wall_hits);
+hipError_t launch_add_extra_force(hipStream_t s, const VertexDev& V, const int64_t* ids, const double* forces, int n);
+
+// mechanics (hc_mech.hip)
+hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
+                            const int64_t* cell_base, double* bend_scratch);
+hipError_t launch_cell_volume_area(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells,
+                                   int ncells, const int64_t* cell_base, double* vol, double* area);
+hipError_t launch_check_finite(hipStream_t s, const double* p, int64_t n, unsigned int* bad);
+hipError_t launch_check_finite_f(hipStream_t s, const float* p, int64_t n, unsigned int* bad);
+
+}  // namespace hcg

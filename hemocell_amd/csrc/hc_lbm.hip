@@ -1,0 +1,465 @@
+// hc_lbm.hip -- D3Q19 Guo-forced BGK collide-and-stream on MI355X.
+//
+// Replaces Palabos's MultiBlockLattice3D::collideAndStream() with
+// GuoExternalForceBGKdynamics (core/hemoCell.cpp:317, :455-469), BounceBack
+// (helper/genericFunctions.cpp:144,154) and the local regularized velocity
+// boundary (helper/hemocellInit.hh:72-77), plus the per-step force reset
+// setExternalVector(..., 0) (core/hemoCell.cpp:369-371).
+//
+// One in-place kernel per step, AA access pattern over structure-of-arrays
+// populations f[i*stride + n] (n = z + nz*(y + ny*x), z fastest so a wavefront
+// reads 64 consecutive nodes of one population array: fully coalesced):
+//   even step: read f[i][n], collide, write f*_i into f[opp(i)][n];
+//   odd step : read f_i(n) = f[opp(i)][n - c_i], collide, write f*_i into f[i][n + c_i].
+// Each thread reads and writes the same 19 slots, so no second lattice and no
+// race.  After an odd step the array holds natural post-stream populations; after
+// an even step the post-stream value of (n, i) is f[opp(i)][n - c_i].  Values
+// that would arrive from outside a non-periodic domain are 0 (shifted storage).
+//
+// The IBM force is double-buffered: the kernel reads the buffer the spread just
+// filled and zeroes the other one, which the next step's spread fills -- the
+// reference's full-lattice force reset folded into this pass.
+//
+// The Guo-BGK update is evaluated in an FMA-friendly form that is algebraically
+// the Palabos one: with u = j/rho + F/2, mu = rho (1 - omega/2),
+//   f_i' = (1-omega) f_i + t_i [A + B cu + C cu^2 + D cF + E cu cF],
+//   cu = c_i.u, cF = c_i.F, A = omega (rhoBar - 3/2 rho u^2) - 3 mu u.F,
+//   B = 3 omega rho, C = 9/2 omega rho, D = 3 mu, E = 9 mu,
+// split into even and odd parts over each opposite pair (i, i+9).
+#include <utility>
+
+#include "hc_d3q19.h"
+#include "hc_launch.h"
+
+namespace hcg {
+namespace {
+
+constexpr int BLOCK = 256;
+
+// c_I . (a0, a1, a2) with the zero components of c_I dropped at compile time.
+template <int I, typename R>
+__device__ __forceinline__ R cdot(R a0, R a1, R a2) {
+  constexpr int X = cx(I), Y = cy(I), Z = cz(I);
+  R s = R(0);
+  bool first = true;
+  if constexpr (X != 0) { s = X > 0 ? a0 : -a0; first = false; }
+  if constexpr (Y != 0) { s = first ? (Y > 0 ? a1 : -a1) : (Y > 0 ? s + a1 : s - a1); first = false; }
+  if constexpr (Z != 0) { s = first ? (Z > 0 ? a2 : -a2) : (Z > 0 ? s + a2 : s - a2); }
+  return s;
+}
+
+template <typename R>
+struct Coef {
+  R A, B, C, D, E, om1, u0, u1, u2, F0, F1, F2;
+};
+
+template <int I, typename R>
+__device__ __forceinline__ void relax_pair(R (&f)[19], const Coef<R>& k) {
+  const R cu = cdot<I>(k.u0, k.u1, k.u2);
+  const R cF = cdot<I>(k.F0, k.F1, k.F2);
+  const R sym = k.A + cu * (k.C * cu + k.E * cF);
+  const R asym = k.B * cu + k.D * cF;
+  constexpr R w = wgt<R>(I);
+  f[I] = k.om1 * f[I] + w * (sym + asym);
+  f[I + 9] = k.om1 * f[I + 9] + w * (sym - asym);
+}
+
+template <typename R, int... Is>
+__device__ __forceinline__ void relax_all(R (&f)[19], const Coef<R>& k, std::integer_sequence<int, Is...>) {
+  (relax_pair<Is + 1>(f, k), ...);
+}
+
+template <typename R>
+__device__ __forceinline__ void collide_guo(R (&f)[19], R F0, R F1, R F2, R omega) {
+  R rhoBar, j0, j1, j2;
+  rhobar_j(f, rhoBar, j0, j1, j2);
+  const R rho = rhoBar + R(1);
+  const R invRho = R(1) / rho;
+  Coef<R> k;
+  k.u0 = j0 * invRho + F0 * R(0.5);
+  k.u1 = j1 * invRho + F1 * R(0.5);
+  k.u2 = j2 * invRho + F2 * R(0.5);
+  k.F0 = F0;
+  k.F1 = F1;
+  k.F2 = F2;
+  const R mu = rho * (R(1) - omega * R(0.5));
+  const R usq = k.u0 * k.u0 + k.u1 * k.u1 + k.u2 * k.u2;
+  const R uF = k.u0 * F0 + k.u1 * F1 + k.u2 * F2;
+  k.A = omega * (rhoBar - R(1.5) * rho * usq) - R(3) * mu * uF;
+  k.B = R(3) * omega * rho;
+  k.C = R(4.5) * omega * rho;
+  k.D = R(3) * mu;
+  k.E = R(9) * mu;
+  k.om1 = R(1) - omega;
+  f[0] = k.om1 * f[0] + wgt<R>(0) * k.A;
+  relax_all(f, k, std::make_integer_sequence<int, 9>{});
+}
+
+// Second-order equilibrium in shifted form (used by boundary reconstruction and init).
+template <typename R>
+__device__ __forceinline__ R feq(int i, R rhoBar, R invRho, R j0, R j1, R j2, R jSqr) {
+  R c_j = R(cx(i)) * j0;
+  c_j += R(cy(i)) * j1;
+  c_j += R(cz(i)) * j2;
+  return wgt<R>(i) * (rhoBar + R(3) * c_j + R(3) / R(2) * invRho * (R(3) * c_j * c_j - jSqr));
+}
+
+// Local regularized velocity boundary on a domain face: rebuild all populations
+// as f_eq(rhoBar, rho u_w) + f_neq(Pi_neq), Pi_neq from the known populations
+// (on-wall ones and those arriving from the interior, the latter counted twice:
+// bounce-back of the non-equilibrium part).
+template <typename R>
+__device__ __forceinline__ void velocity_boundary(R (&f)[19], int axis, int orient, const double uw[3]) {
+  auto cc = [](int i, int a) { return a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i)); };
+  R rhoOnWall = R(0), rhoNormal = R(0);
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    int c = cc(i, axis);
+    if (c == 0) rhoOnWall += f[i];
+    else if (c == orient) rhoNormal += f[i];
+  }
+  const R w0 = R(uw[0]), w1 = R(uw[1]), w2 = R(uw[2]);
+  const R velNormal = R(orient) * (axis == 0 ? w0 : (axis == 1 ? w1 : w2));
+  const R rhoBar = (R(2) * rhoNormal + rhoOnWall - velNormal) / (R(1) + velNormal);
+  const R rho = rhoBar + R(1);
+  const R j0 = rho * w0, j1 = rho * w1, j2 = rho * w2;
+  const R jSqr = j0 * j0 + j1 * j1 + j2 * j2;
+  const R invRho = R(1) / rho;
+  R fNeq[19];
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    int c = cc(i, axis);
+    fNeq[i] = (c == 0 || c == orient) ? f[i] - feq<R>(i, rhoBar, invRho, j0, j1, j2, jSqr) : R(0);
+  }
+  R Pi[6];
+  int p = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b) {
+      R s = R(0);
+#pragma unroll
+      for (int i = 0; i < 19; ++i)
+        if (cc(i, axis) == 0) s += R(cc(i, a) * cc(i, b)) * fNeq[i];
+#pragma unroll
+      for (int i = 0; i < 19; ++i)
+        if (cc(i, axis) == orient) s += R(2 * cc(i, a) * cc(i, b)) * fNeq[i];
+      Pi[p++] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    R fn = R(0);
+    int q = 0;
+  #pragma unroll
+  for (int a = 0; a < 3; ++a)
+      for (int b = a; b < 3; ++b) {
+        R t = R(cc(i, a) * cc(i, b));
+        if (a == b) t -= R(1) / R(3);
+        else t *= R(2);
+        fn += t * Pi[q++];
+      }
+    fn *= wgt<R>(i) * R(3) * R(3) / R(2);
+    f[i] = feq<R>(i, rhoBar, invRho, j0, j1, j2, jSqr) + fn;
+  }
+}
+
+__device__ __forceinline__ void coords(int64_t n, const LatticeDev& L, int& x, int& y, int& z) {
+  z = (int)(n % L.nz);
+  int64_t t = n / L.nz;
+  y = (int)(t % L.ny);
+  x = (int)(t / L.ny);
+}
+
+// Index offsets of the six axis neighbours of node (x, y, z); a neighbour
+// outside a non-periodic domain gets INVALID.
+constexpr int INVALID = 0x40000000;
+struct NbrOff {
+  int xm, xp, ym, yp, zm, zp;
+};
+__device__ __forceinline__ NbrOff neighbour_offsets(int x, int y, int z, const LatticeDev& L) {
+  const int nyz = L.ny * L.nz;
+  NbrOff o;
+  o.xm = x > 0 ? -nyz : (L.periodic[0] ? (L.nx - 1) * nyz : INVALID);
+  o.xp = x < L.nx - 1 ? nyz : (L.periodic[0] ? -(L.nx - 1) * nyz : INVALID);
+  o.ym = y > 0 ? -L.nz : (L.periodic[1] ? (L.ny - 1) * L.nz : INVALID);
+  o.yp = y < L.ny - 1 ? L.nz : (L.periodic[1] ? -(L.ny - 1) * L.nz : INVALID);
+  o.zm = z > 0 ? -1 : (L.periodic[2] ? L.nz - 1 : INVALID);
+  o.zp = z < L.nz - 1 ? 1 : (L.periodic[2] ? -(L.nz - 1) : INVALID);
+  return o;
+}
+// offset of node n + (DX, DY, DZ); sets ok = false when it leaves the domain
+template <int DX, int DY, int DZ, bool CHECK>
+__device__ __forceinline__ int nbr_offset(const NbrOff& o, bool& ok) {
+  int d = 0;
+  ok = true;
+  if constexpr (DX < 0) { d += o.xm; if (CHECK) ok &= o.xm != INVALID; }
+  if constexpr (DX > 0) { d += o.xp; if (CHECK) ok &= o.xp != INVALID; }
+  if constexpr (DY < 0) { d += o.ym; if (CHECK) ok &= o.ym != INVALID; }
+  if constexpr (DY > 0) { d += o.yp; if (CHECK) ok &= o.yp != INVALID; }
+  if constexpr (DZ < 0) { d += o.zm; if (CHECK) ok &= o.zm != INVALID; }
+  if constexpr (DZ > 0) { d += o.zp; if (CHECK) ok &= o.zp != INVALID; }
+  return d;
+}
+
+// AA-pattern read of the 19 post-stream populations of node n
+template <typename R, bool ODD, bool CHECK, int... Is>
+__device__ __forceinline__ void aa_load(const R* __restrict__ f, int64_t S, uint32_t n, const NbrOff& o, R (&fl)[19],
+                                        std::integer_sequence<int, Is...>) {
+  auto one = [&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (!ODD) {
+      fl[i] = f[i * S + n];
+    } else {
+      bool ok;
+      const int d = nbr_offset<-cx(i), -cy(i), -cz(i), CHECK>(o, ok);
+      fl[i] = ok ? f[opp(i) * S + (uint32_t)(n + d)] : R(0);
+    }
+  };
+  (one(std::integral_constant<int, Is>{}), ...);
+}
+
+// AA-pattern write of the 19 post-collision populations of node n
+template <typename R, bool ODD, bool CHECK, int... Is>
+__device__ __forceinline__ void aa_store(R* __restrict__ f, int64_t S, uint32_t n, const NbrOff& o, const R (&fl)[19],
+                                         std::integer_sequence<int, Is...>) {
+  auto one = [&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (!ODD) {
+      f[opp(i) * S + n] = fl[i];
+    } else {
+      bool ok;
+      const int d = nbr_offset<cx(i), cy(i), cz(i), CHECK>(o, ok);
+      if (ok) f[i * S + (uint32_t)(n + d)] = fl[i];
+      if constexpr (CHECK) {
+        bool in;
+        (void)nbr_offset<-cx(i), -cy(i), -cz(i), CHECK>(o, in);
+        if (!in) f[i * S + n] = R(0);  // nothing arrives from outside the domain
+      }
+    }
+  };
+  (one(std::integral_constant<int, Is>{}), ...);
+}
+
+// GENERAL: node-type mask present and/or some axis non-periodic.  Velocity
+// boundary nodes are skipped here and updated by k_velocity_nodes (each node
+// only touches its own 19 slots, so the two kernels may run in either order).
+template <typename R, bool ODD, bool GENERAL>
+__global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, const R* __restrict__ Fcur,
+                                                          R* __restrict__ Fclr, const uint8_t* __restrict__ type,
+                                                          LatticeDev L, CollideArgs a) {
+  const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= (uint32_t)L.N) return;
+  const int64_t S = L.stride;
+  const uint8_t ty = GENERAL && type ? type[n] : (uint8_t)HC_NODE_FLUID;
+  if (GENERAL && ty >= HC_NODE_VELOCITY) return;
+  const int z = (int)(n % (uint32_t)L.nz);
+  const uint32_t t = n / (uint32_t)L.nz;
+  const int y = (int)(t % (uint32_t)L.ny);
+  const int x = (int)(t / (uint32_t)L.ny);
+  const NbrOff o = neighbour_offsets(x, y, z, L);
+  R fl[19];
+  aa_load<R, ODD, GENERAL>(f, S, n, o, fl, std::make_integer_sequence<int, 19>{});
+  if (GENERAL && ty == HC_NODE_BOUNCEBACK) {
+#pragma unroll
+    for (int i = 1; i <= 9; ++i) {
+      R tmp = fl[i];
+      fl[i] = fl[i + 9];
+      fl[i + 9] = tmp;
+    }
+  } else {
+    const R F0 = R(a.g[0]) + Fcur[n], F1 = R(a.g[1]) + Fcur[S + n], F2 = R(a.g[2]) + Fcur[2 * S + n];
+    collide_guo<R>(fl, F0, F1, F2, R(a.omega));
+  }
+  Fclr[n] = R(0);
+  Fclr[S + n] = R(0);
+  Fclr[2 * S + n] = R(0);
+  aa_store<R, ODD, GENERAL>(f, S, n, o, fl, std::make_integer_sequence<int, 19>{});
+}
+
+// Velocity-boundary nodes (listed in `nodes`): regularized reconstruction, then
+// the Guo-BGK collision, with the same AA read/write as the bulk kernel.
+template <typename R, bool ODD>
+__global__ __launch_bounds__(64) void k_velocity_nodes(R* __restrict__ f, const R* __restrict__ Fcur,
+                                                       R* __restrict__ Fclr, const uint8_t* __restrict__ type,
+                                                       const uint32_t* __restrict__ nodes, int count, LatticeDev L,
+                                                       CollideArgs a) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t n = nodes[k];
+  const int64_t S = L.stride;
+  const int z = (int)(n % (uint32_t)L.nz);
+  const uint32_t t = n / (uint32_t)L.nz;
+  const int y = (int)(t % (uint32_t)L.ny);
+  const int x = (int)(t / (uint32_t)L.ny);
+  const NbrOff o = neighbour_offsets(x, y, z, L);
+  R fl[19];
+  aa_load<R, ODD, true>(f, S, n, o, fl, std::make_integer_sequence<int, 19>{});
+  const int face = type[n] - HC_NODE_VELOCITY;
+  velocity_boundary<R>(fl, face >> 1, (face & 1) ? 1 : -1, a.wall_u[face]);
+  const R F0 = R(a.g[0]) + Fcur[n], F1 = R(a.g[1]) + Fcur[S + n], F2 = R(a.g[2]) + Fcur[2 * S + n];
+  collide_guo<R>(fl, F0, F1, F2, R(a.omega));
+  Fclr[n] = R(0);
+  Fclr[S + n] = R(0);
+  Fclr[2 * S + n] = R(0);
+  aa_store<R, ODD, true>(f, S, n, o, fl, std::make_integer_sequence<int, 19>{});
+}
+
+template <typename R>
+__global__ void k_init_eq(R* f, const uint8_t* type, LatticeDev L, double rho, double u0, double u1, double u2) {
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= L.N) return;
+  const bool bb = type && type[n] == HC_NODE_BOUNCEBACK;
+  const R rhoBar = R(rho - 1.0), invRho = R(1.0 / rho);
+  const R j0 = R(rho * u0), j1 = R(rho * u1), j2 = R(rho * u2);
+  const R jSqr = j0 * j0 + j1 * j1 + j2 * j2;
+  for (int i = 0; i < 19; ++i) f[i * L.stride + n] = bb ? R(0) : feq<R>(i, rhoBar, invRho, j0, j1, j2, jSqr);
+}
+
+// post-stream population (n, i) in the given phase of the AA pattern
+template <typename R>
+__device__ __forceinline__ R post_stream(const R* f, const LatticeDev& L, int phase, int x, int y, int z, int i,
+                                         int64_t n) {
+  if (phase == 0) return f[i * L.stride + n];
+  int X = shift_coord(x, -cx(i), L.nx, L.periodic[0]);
+  int Y = shift_coord(y, -cy(i), L.ny, L.periodic[1]);
+  int Z = shift_coord(z, -cz(i), L.nz, L.periodic[2]);
+  if (X < 0 || Y < 0 || Z < 0) return R(0);
+  return f[opp(i) * L.stride + (int64_t)Z + (int64_t)L.nz * ((int64_t)Y + (int64_t)L.ny * X)];
+}
+
+template <typename R>
+__global__ void k_natural(const R* f, double* out, LatticeDev L, int phase) {
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= L.N) return;
+  int x, y, z;
+  coords(n, L, x, y, z);
+  for (int i = 0; i < 19; ++i) out[i * L.N + n] = (double)post_stream(f, L, phase, x, y, z, i, n);
+}
+
+template <typename R>
+__global__ void k_store(R* f, const double* in, LatticeDev L) {
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= L.N) return;
+  for (int i = 0; i < 19; ++i) f[i * L.stride + n] = (R)in[i * L.N + n];
+}
+
+template <typename R>
+__global__ void k_moments(const R* f, const R* Fcur, LatticeDev L, int phase, double g0, double g1, double g2,
+                          double* rho, double* u) {
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= L.N) return;
+  int x, y, z;
+  coords(n, L, x, y, z);
+  R fl[19];
+  for (int i = 0; i < 19; ++i) fl[i] = post_stream(f, L, phase, x, y, z, i, n);
+  R rhoBar, j0, j1, j2;
+  rhobar_j(fl, rhoBar, j0, j1, j2);
+  const R invRho = R(1) / (rhoBar + R(1));
+  const int64_t S = L.stride;
+  if (rho) rho[n] = (double)(rhoBar + R(1));
+  if (u) {
+    u[3 * n + 0] = (double)(j0 * invRho + (R(g0) + Fcur[n]) / R(2));
+    u[3 * n + 1] = (double)(j1 * invRho + (R(g1) + Fcur[S + n]) / R(2));
+    u[3 * n + 2] = (double)(j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2));
+  }
+}
+
+template <typename R>
+__global__ void k_zero(R* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t step = (int64_t)gridDim.x * BLOCK;
+  for (; i < n; i += step) p[i] = R(0);
+}
+
+template <typename R>
+__global__ void k_to_double(const R* in, double* out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t step = (int64_t)gridDim.x * BLOCK;
+  for (; i < n; i += step) out[i] = (double)in[i];
+}
+
+inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+inline dim3 grid_capped(int64_t n) {
+  int64_t b = (n + BLOCK - 1) / BLOCK;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+}  // namespace
+
+template <typename R>
+hipError_t launch_collide_stream(hipStream_t s, R* f, const R* Fcur, R* Fclr, const uint8_t* type,
+                                 const LatticeDev& L, const CollideArgs& a, int odd, bool general,
+                                 const uint32_t* vel_nodes, int n_vel) {
+  dim3 g = grid_for(L.N);
+  if (general) {
+    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+    else hipLaunchKernelGGL((k_collide_stream<R, false, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+    if (n_vel > 0) {
+      dim3 gv((unsigned)((n_vel + 63) / 64));
+      if (odd)
+        hipLaunchKernelGGL((k_velocity_nodes<R, true>), gv, dim3(64), 0, s, f, Fcur, Fclr, type, vel_nodes, n_vel, L, a);
+      else
+        hipLaunchKernelGGL((k_velocity_nodes<R, false>), gv, dim3(64), 0, s, f, Fcur, Fclr, type, vel_nodes, n_vel, L, a);
+    }
+  } else {
+    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, false>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+    else hipLaunchKernelGGL((k_collide_stream<R, false, false>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+  }
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_init_equilibrium(hipStream_t s, R* f, const uint8_t* type, const LatticeDev& L, double rho,
+                                   const double u[3]) {
+  hipLaunchKernelGGL((k_init_eq<R>), grid_for(L.N), dim3(BLOCK), 0, s, f, type, L, rho, u[0], u[1], u[2]);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_natural_populations(hipStream_t s, const R* f, double* out, const LatticeDev& L, int phase) {
+  hipLaunchKernelGGL((k_natural<R>), grid_for(L.N), dim3(BLOCK), 0, s, f, out, L, phase);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_store_populations(hipStream_t s, R* f, const double* in, const LatticeDev& L) {
+  hipLaunchKernelGGL((k_store<R>), grid_for(L.N), dim3(BLOCK), 0, s, f, in, L);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_fluid_moments(hipStream_t s, const R* f, const R* Fcur, const LatticeDev& L, int phase,
+                                const double g[3], double* rho, double* u) {
+  hipLaunchKernelGGL((k_moments<R>), grid_for(L.N), dim3(BLOCK), 0, s, f, Fcur, L, phase, g[0], g[1], g[2], rho, u);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_fill_zero(hipStream_t s, R* p, int64_t n) {
+  hipLaunchKernelGGL((k_zero<R>), grid_capped(n), dim3(BLOCK), 0, s, p, n);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_to_double(hipStream_t s, const R* in, double* out, int64_t n) {
+  hipLaunchKernelGGL((k_to_double<R>), grid_capped(n), dim3(BLOCK), 0, s, in, out, n);
+  return hipGetLastError();
+}
+
+#define HC_INSTANTIATE(R)                                                                                      \
+  template hipError_t launch_collide_stream<R>(hipStream_t, R*, const R*, R*, const uint8_t*, const LatticeDev&, \
+                                               const CollideArgs&, int, bool, const uint32_t*, int);          \
+  template hipError_t launch_init_equilibrium<R>(hipStream_t, R*, const uint8_t*, const LatticeDev&, double,    \
+                                                 const double*);                                              \
+  template hipError_t launch_natural_populations<R>(hipStream_t, const R*, double*, const LatticeDev&, int);    \
+  template hipError_t launch_store_populations<R>(hipStream_t, R*, const double*, const LatticeDev&);           \
+  template hipError_t launch_fluid_moments<R>(hipStream_t, const R*, const R*, const LatticeDev&, int,          \
+                                              const double*, double*, double*);                               \
+  template hipError_t launch_fill_zero<R>(hipStream_t, R*, int64_t);                                            \
+  template hipError_t launch_to_double<R>(hipStream_t, const R*, double*, int64_t);
+HC_INSTANTIATE(float)
+HC_INSTANTIATE(double)
+#undef HC_INSTANTIATE
+
+}  // namespace hcg

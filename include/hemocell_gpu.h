@@ -1,0 +1,195 @@
+/* hemocell_gpu.h -- C ABI of libhcgpu.so, the MI355X (gfx950) implementation of
+ * HemoCell's IB-LBM time step.
+ *
+ * The reference drives the step from C++ (hemo::HemoCell::iterate(),
+ * core/hemoCell.cpp:299-376) over Palabos lattices and per-block particle
+ * vectors.  This header is the drop-in boundary that a host (the hemo::HemoCell
+ * C++ mirror in hemocell_amd/csrc/hemocell.h, a ctypes/cgo/JNI binding, ...)
+ * binds to.  Plain pointers and sizes only; all arrays are host memory owned by
+ * the caller and copied at the call.  Every call returns hc_status; the text of
+ * the last error is available from hc_last_error().
+ *
+ * Units: lattice units everywhere (the SI -> lattice conversion of
+ * mechanics/constantConversion.cpp:36-90 is host-side: hc_lbm_base_parameters(),
+ * hc_model_constants()).
+ * Lattice node index n = z + nz*(y + ny*x) (patch/palabos.patch:244-250).
+ * D3Q19 population order is Palabos's: c_0 = 0, c_1..9 =
+ * (-1,0,0),(0,-1,0),(0,0,-1),(-1,-1,0),(-1,1,0),(-1,0,-1),(-1,0,1),(0,-1,-1),(0,-1,1),
+ * c_{i+9} = -c_i.  Populations are exchanged shifted by the lattice weights
+ * (f_i - t_i).
+ */
+#ifndef HEMOCELL_GPU_H
+#define HEMOCELL_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hc_ctx hc_ctx;
+
+typedef enum {
+  HC_OK = 0,
+  HC_INVALID_ARG = 1, /* bad size, index, order of calls */
+  HC_OOM = 2,         /* device allocation failed */
+  HC_DEVICE = 3,      /* HIP runtime / kernel error */
+  HC_NUMERIC = 4,     /* NaN or Inf detected by hc_check_finite() */
+  HC_STATE = 5        /* call not valid in the current state */
+} hc_status;
+
+typedef enum {
+  HC_FP64 = 0,          /* FP64 populations and collision: the correctness mode */
+  HC_FP32_SHIFTED = 1   /* FP32 populations (f_i - t_i) and FP32 collision; vertex
+                           state, IBM weights and mechanics stay FP64 */
+} hc_precision;
+
+/* Node types of the lattice mask (one uint8 per node). */
+enum {
+  HC_NODE_FLUID = 0,      /* GuoExternalForceBGKdynamics */
+  HC_NODE_BOUNCEBACK = 1, /* Palabos BounceBack (full-way, on-node); isBoundary() */
+  HC_NODE_VELOCITY = 2    /* 2 + face: local regularized velocity boundary on domain
+                             face 0..5 = x-,x+,y-,y+,z-,z+ (helper/hemocellInit.hh:58-92) */
+};
+
+/* Cell mechanics models (the mechanics/ plugins of the reference). */
+enum {
+  HC_MODEL_RBC = 0, /* RbcHighOrderModel  mechanics/rbcHighOrderModel.cpp:38-207 */
+  HC_MODEL_PLT = 1, /* PltSimpleModel     mechanics/pltSimpleModel.cpp:44-208 */
+  HC_MODEL_WBC = 2  /* WbcHighOrderModel  mechanics/wbcHighOrderModel.cpp:42-225 */
+};
+
+/* Mesh shapes (config/constant_defaults.h:80-84). */
+enum {
+  HC_SHAPE_WBC_SPHERE = 0,
+  HC_SHAPE_RBC_FROM_SPHERE = 1,
+  HC_SHAPE_ELLIPSOID_FROM_SPHERE = 6
+};
+
+/* Stiffness and geometry constants of one cell type, lattice units (what the
+ * reference's CellMechanics::calculate_k* return, mechanics/cellMechanics.h:50-78,
+ * wbcHighOrderModel.cpp:243-262). */
+typedef struct {
+  double k_link, k_bend, k_volume, k_area, eta_m;
+  double k_inner_rigid, k_cytoskeleton, core_radius, radius; /* WBC only */
+} hc_model_params;
+
+/* SI inputs (dx [m], dt [s] (< 0: choose dt for tau = 1), nu_p [m^2/s], rho_p [kg/m^3],
+ * kBT_p [J]) and the derived lattice quantities. */
+typedef struct {
+  double dx, dt, nu_p, rho_p, kBT_p;          /* in (dt is replaced when < 0) */
+  double tau, nu_lbm, dm, df, f_limit, kBT_lbm; /* out */
+} hc_units;
+
+/* Material constants of a cell type as in its <MaterialModel> XML (SI where dimensional). */
+typedef struct {
+  double kLink, kBend, kVolume, kArea, eta_m;
+  double kInnerRigid, kCytoskeleton, coreRadius, radius; /* WBC only */
+} hc_material;
+
+/* Replaces: Parameters::lbm_base_parameters (mechanics/constantConversion.cpp:36-59). Host only. */
+hc_status hc_lbm_base_parameters(hc_units* u);
+/* Replaces: CellMechanics::calculate_kLink/kBend/kVolume/kArea/etaM (mechanics/cellMechanics.h:50-78)
+ * and WbcHighOrderModel::calculate_* (mechanics/wbcHighOrderModel.cpp:243-262). Host only. */
+hc_status hc_model_constants(int model, const hc_material* m, int n_triangles, const hc_units* u,
+                             hc_model_params* out);
+
+/* ---- context -------------------------------------------------------------
+ * Replaces: hemo::HemoCell::HemoCell (core/hemoCell.cpp:69-95) ownership of the
+ * lattice and cell fields.  One context = one HIP stream on `device`. */
+hc_status hc_create(int device, hc_ctx** out);
+void hc_destroy(hc_ctx* ctx);
+const char* hc_last_error(const hc_ctx* ctx);
+
+/* ---- lattice ---------------------------------------------------------------
+ * Replaces: new MultiBlockLattice3D<T,ForcedD3Q19>(..., new GuoExternalForceBGKdynamics(1/tau))
+ * (core/hemoCell.cpp:455-469), defineDynamics(BounceBack) (helper/genericFunctions.cpp:144,154),
+ * periodicity().toggle (core/hemoCell.cpp:162-176).  node_type may be NULL (all fluid). */
+hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* node_type,
+                          double tau, const int periodic[3], int precision);
+/* Replaces: setBoundaryVelocity on a face (helper/hemocellInit.hh:75-77). */
+hc_status hc_set_wall_velocity(hc_ctx* ctx, int face, const double u[3]);
+/* Replaces: the driver's setExternalVector(lattice, bbox, force, g) after every
+ * iterate (examples/pipeflow/pipeflow.cpp:144-146) -- a uniform body force. */
+hc_status hc_set_body_force(hc_ctx* ctx, const double g[3]);
+/* Replaces: HemoCell::latticeEquilibrium / initializeAtEquilibrium (core/hemoCell.cpp:129-133). */
+hc_status hc_init_equilibrium(hc_ctx* ctx, double rho, const double u[3]);
+
+/* ---- cell types and cells ------------------------------------------------
+ * Replaces: HemoCell::addCellType<Model>(name, shape) (hemocell.h:122-128) with the
+ * mesh built by hc_build_mesh and the constants of CommonCellConstants
+ * (mechanics/commonCellConstants.cpp:70-409) derived here from the rest mesh. */
+hc_status hc_build_mesh(int shape, double radius_lu, int min_triangles, double aspect_ratio,
+                        int* nv, int* nt, double* vertices /* nullable, nv*3 */,
+                        int32_t* triangles /* nullable, nt*3 */);
+hc_status hc_add_cell_type(hc_ctx* ctx, int model, int nv, int nt, const int32_t* triangles,
+                           const double* rest_vertices, int n_inner, const int32_t* inner_edges,
+                           const hc_model_params* params, int timescale, int* type_id);
+/* Replaces: readPositionsBloodCellField3D placement (io/readPositionsBloodCells.cpp:120-170):
+ * vertex positions (lattice units, unwrapped) of ncells cells of one type, nv*3 per cell.
+ * cell_ids may be NULL (sequential ids). */
+hc_status hc_add_cells(hc_ctx* ctx, int type_id, int ncells, const double* xyz,
+                       const int32_t* cell_ids);
+
+/* ---- step controls -------------------------------------------------------
+ * Replaces: setParticleVelocityUpdateTimeScaleSeparation, setMaterialTimeScaleSeparation,
+ * setRepulsion (core/hemoCell.cpp:380-426) and param::f_limit (constantConversion.cpp:55-57). */
+hc_status hc_set_velocity_timescale(hc_ctx* ctx, int timescale);
+hc_status hc_set_material_timescale(hc_ctx* ctx, int type_id, int timescale);
+hc_status hc_set_force_limit(hc_ctx* ctx, double f_limit);
+hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale);
+/* Replaces: HemoCellStretch::applyForce (helper/hemoCellStretch.cpp:63-110): an extra
+ * force added to the listed vertices (global vertex ids) before every step's spread. */
+hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* vertex_ids,
+                                    const double* forces);
+
+/* ---- the step --------------------------------------------------------------
+ * Replaces: HemoCell::iterate() (core/hemoCell.cpp:299-376), nsteps times. Asynchronous
+ * on the context stream; errors of the launches are reported. */
+hc_status hc_step(hc_ctx* ctx, int nsteps);
+hc_status hc_synchronize(hc_ctx* ctx);
+int64_t hc_iteration(const hc_ctx* ctx);
+
+/* Single stages of iterate(), in its order (parity tests and partial drivers). */
+hc_status hc_stage_spread(hc_ctx* ctx);          /* spreadParticleForce  hemoCellParticleField.cpp:841-863 */
+hc_status hc_stage_collide_stream(hc_ctx* ctx);  /* lattice->collideAndStream()  hemoCell.cpp:317 */
+hc_status hc_stage_interpolate(hc_ctx* ctx);     /* interpolateFluidVelocity  hemoCellParticleField.cpp:819-839 */
+hc_status hc_stage_advance(hc_ctx* ctx);         /* advanceParticles  hemoCellParticleField.cpp:566-588 */
+hc_status hc_stage_mechanics(hc_ctx* ctx, int forced); /* applyConstitutiveModel  :633-675 */
+hc_status hc_stage_repulsion(hc_ctx* ctx);       /* applyRepulsionForce  :677-743 */
+hc_status hc_stage_end(hc_ctx* ctx);             /* setExternalVector reset + iter++  hemoCell.cpp:369-374 */
+
+/* ---- state access (synchronous) ------------------------------------------ */
+int64_t hc_num_nodes(const hc_ctx* ctx);
+int64_t hc_num_vertices(const hc_ctx* ctx);
+int hc_num_cells(const hc_ctx* ctx);
+/* populations, natural (post-stream) layout, f[i*N + n], shifted, as double */
+hc_status hc_get_populations(hc_ctx* ctx, double* f);
+hc_status hc_set_populations(hc_ctx* ctx, const double* f);
+/* IBM part of the force field (3*N, comp*N + n); the body force is not included */
+hc_status hc_get_force_field(hc_ctx* ctx, double* F);
+/* density and velocity u = j/rho + F/2 (F = body force + IBM force) per node; either may be NULL */
+hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u /* 3*N, n*3+c */);
+/* vertex state, n*3+c; any pointer may be NULL */
+hc_status hc_get_vertices(hc_ctx* ctx, double* x, double* v, double* f);
+hc_status hc_set_vertices(hc_ctx* ctx, const double* x, const double* v, const double* f);
+hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep);
+/* per-cell FP64 volume and area of the current shape (helper/cellInfo.cpp:39-120) */
+hc_status hc_get_cell_volume_area(hc_ctx* ctx, double* volume, double* area);
+/* vertices whose nearest node became a wall node during advance (the reference deletes
+ * their cells, hemoCellParticleField.cpp:570-584) */
+int64_t hc_wall_hits(hc_ctx* ctx);
+/* HC_NUMERIC if any population or vertex coordinate is NaN/Inf */
+hc_status hc_check_finite(hc_ctx* ctx);
+
+/* ---- timing (HIP events on the context stream) --------------------------- */
+enum { HC_PROF_SPREAD = 0, HC_PROF_COLLIDE = 1, HC_PROF_INTERP = 2, HC_PROF_MECH = 3,
+       HC_PROF_REPULSION = 4, HC_PROF_STEP = 5, HC_PROF_NSTAGES = 6 };
+hc_status hc_profile_enable(hc_ctx* ctx, int enable);
+/* total milliseconds and number of timed launches of a stage since enabling */
+hc_status hc_profile_read(hc_ctx* ctx, int stage, double* total_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HEMOCELL_GPU_H */

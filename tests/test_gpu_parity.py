@@ -1,0 +1,279 @@
+"""GPU parity: libhcgpu.so (through the C ABI) against the FP64 CPU oracle.
+
+Tolerances (relative L2 unless stated):
+  FP64 lattice, stage or step          <= 1e-12 (summation-order round-off only)
+  FP64 full step, 100 steps            <= 1e-10 (velocity field and vertex positions)
+  FP32-shifted, fluid velocity         <= 1e-5 of the velocity scale
+  FP32-shifted, vertex positions       <= 1e-4 of the displacement
+Integer data (node masks, support node sets, cell/vertex counts) is bit-exact.
+"""
+import numpy as np
+import pytest
+
+import cases
+import hemocell_amd as hc
+from oracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(nx, ny, nz, tau, periodic=(1, 1, 1), mask=None, precision=hc.HC_FP64):
+    g = hc.Simulation(0)
+    g.lattice_init(nx, ny, nz, tau, periodic=periodic, node_type=mask, precision=precision)
+    o = OracleSim(nx, ny, nz, tau, periodic=periodic, node_type=mask)
+    return g, o
+
+
+def both(g, o, name, *args):
+    getattr(g, name)(*args)
+    getattr(o, name)(*args)
+
+
+# ------------------------------------------------------------------ lattice --
+@pytest.mark.parametrize("shape", [(8, 8, 8), (5, 7, 9)])
+@pytest.mark.parametrize("nsteps", [1, 2, 5])
+def test_collide_stream_periodic_fp64(shape, nsteps):
+    rng = np.random.default_rng(1)
+    g, o = pair(*shape, tau=0.8)
+    f0 = cases.random_populations(np.prod(shape), rng)
+    both(g, o, "set_populations", f0)
+    both(g, o, "set_body_force", [1e-5, -2e-5, 3e-5])
+    for _ in range(nsteps):
+        both(g, o, "step", 1)
+    assert cases.rel_l2(g.populations(), o.populations()) < 1e-12
+    rg, ug = g.fluid()
+    ro, uo = o.fluid()
+    assert cases.rel_l2(rg, ro) < 1e-14
+    assert cases.rel_l2(ug, uo) < 1e-12
+
+
+@pytest.mark.parametrize("nsteps", [1, 4])
+def test_collide_stream_bounceback_channel_fp64(nsteps):
+    rng = np.random.default_rng(2)
+    nx, ny, nz = 6, 9, 7
+    mask = cases.channel_mask(nx, ny, nz, axis=1)
+    g, o = pair(nx, ny, nz, tau=1.1, periodic=(1, 0, 1), mask=mask)
+    f0 = cases.random_populations(nx * ny * nz, rng)
+    both(g, o, "set_populations", f0)
+    both(g, o, "set_body_force", [2e-5, 0, 1e-5])
+    both(g, o, "step", nsteps)
+    fg, fo = g.populations(), o.populations()
+    fluid = mask == 0
+    assert cases.rel_l2(fg[:, fluid], fo[:, fluid]) < 1e-12
+    assert cases.rel_l2(fg, fo) < 1e-12
+
+
+@pytest.mark.parametrize("nsteps", [1, 2, 7])
+def test_collide_stream_velocity_walls_fp64(nsteps):
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 6, 5, 8
+    mask = cases.couette_mask(nx, ny, nz)
+    g, o = pair(nx, ny, nz, tau=1.0, periodic=(1, 1, 0), mask=mask)
+    both(g, o, "set_wall_velocity", 4, [1e-3, 0, 0])
+    both(g, o, "set_wall_velocity", 5, [-1e-3, 2e-4, 0])
+    f0 = cases.random_populations(nx * ny * nz, rng)
+    both(g, o, "set_populations", f0)
+    both(g, o, "step", nsteps)
+    assert cases.rel_l2(g.populations(), o.populations()) < 1e-12
+
+
+def test_collide_stream_tube_fp32():
+    rng = np.random.default_rng(4)
+    nx, ny, nz = 12, 10, 10
+    mask = cases.tube_mask(nx, ny, nz, axis=0)
+    g, o = pair(nx, ny, nz, tau=0.9, periodic=(1, 0, 0), mask=mask, precision=hc.HC_FP32_SHIFTED)
+    f0 = cases.random_populations(nx * ny * nz, rng, amp=1e-4)
+    both(g, o, "set_populations", f0)
+    both(g, o, "set_body_force", [1e-6, 0, 0])
+    both(g, o, "step", 6)
+    _, ug = g.fluid()
+    _, uo = o.fluid()
+    assert cases.rel_l2(ug, uo) < 1e-5
+
+
+def test_init_equilibrium_matches():
+    nx, ny, nz = 4, 5, 6
+    mask = cases.channel_mask(nx, ny, nz, axis=2)
+    g, o = pair(nx, ny, nz, tau=1.0, periodic=(1, 1, 0), mask=mask)
+    both(g, o, "init_equilibrium", 1.01, [1e-3, -2e-3, 5e-4])
+    assert cases.rel_l2(g.populations(), o.populations()) < 1e-15
+
+
+# ---------------------------------------------------------------- IBM stages --
+def _one_cell(g, o, u, kind, center, rng, amp=0.05):
+    model, v, t, p, inner = cases.mesh(kind, u, hc.build_mesh)
+    tg = g.add_cell_type(model, v, t, p, inner)
+    to = o.add_cell_type(model, v, t, p, inner)
+    x = cases.perturb(cases.place(v, center, cases.rotation(rng)), rng, amp)
+    g.add_cells(tg, x[None])
+    o.add_cells(to, x[None])
+    return x
+
+
+def test_spread_and_interpolate_stage_fp64():
+    rng = np.random.default_rng(5)
+    u = cases.units()
+    nx, ny, nz = 24, 26, 28
+    mask = cases.channel_mask(nx, ny, nz, axis=2)
+    g, o = pair(nx, ny, nz, tau=u.tau, periodic=(1, 1, 0), mask=mask)
+    _one_cell(g, o, u, "RBC", (12.3, 13.1, 13.7), rng)
+    both(g, o, "set_populations", cases.random_populations(nx * ny * nz, rng, 1e-4))
+    nv = g.num_vertices
+    forces = 1e-3 * rng.normal(size=(nv, 3))
+    forces[::7] *= 50.0  # some exceed f_limit and get capped in place
+    both(g, o, "set_vertices", None, None, forces)
+    both(g, o, "set_force_limit", u.f_limit)
+    g.stage("spread")
+    o.stage("spread")
+    assert cases.rel_l2(g.force_field(), o.force_field()) < 1e-13
+    _, _, fg = g.vertices()
+    _, _, fo = o.vertices()
+    assert np.array_equal(fg, fo)  # the capped forces are written back identically
+    g.stage("collide_stream")
+    o.stage("collide_stream")
+    g.stage("interpolate")
+    o.stage("interpolate")
+    _, vg, _ = g.vertices()
+    _, vo, _ = o.vertices()
+    assert cases.rel_l2(vg, vo) < 1e-12
+
+
+# ----------------------------------------------------------- mechanics stage --
+@pytest.mark.parametrize("kind", ["RBC", "PLT", "WBC", "WBC2562"])
+def test_mechanics_stage_fp64(kind):
+    rng = np.random.default_rng(6)
+    u = cases.units()
+    g, o = pair(40, 40, 40, tau=u.tau)
+    _one_cell(g, o, u, kind, (20.2, 19.7, 20.4), rng, amp=0.08)
+    nv = g.num_vertices
+    vel = 1e-4 * rng.normal(size=(nv, 3))
+    both(g, o, "set_vertices", None, vel, None)
+    g.stage("mechanics", 1)
+    o.stage("mechanics", 1)
+    _, _, fg = g.vertices()
+    _, _, fo = o.vertices()
+    assert np.abs(fo).max() > 0
+    assert cases.rel_l2(fg, fo) < 1e-11
+
+
+def test_mechanics_rest_shape_is_force_free():
+    u = cases.units()
+    g = hc.Simulation(0)
+    g.lattice_init(40, 40, 40, u.tau)
+    model, v, t, p, inner = cases.mesh("RBC", u, hc.build_mesh)
+    tid = g.add_cell_type(model, v, t, p, inner)
+    g.add_cells(tid, (v + 20.0)[None])
+    g.stage("mechanics", 1)
+    _, _, f = g.vertices()
+    assert np.abs(f).max() < 1e-12 * max(p.k_link, p.k_bend, p.k_area, p.k_volume)
+    vol, area = g.cell_volume_area()
+    assert abs(vol[0] * (u.dx * 1e6) ** 3 - 81.116) < 5e-4
+
+
+# ---------------------------------------------------------------- full step --
+def _shear_case(precision, n=32, steps=100, seed=7):
+    rng = np.random.default_rng(seed)
+    u = cases.units()
+    mask = cases.couette_mask(n, n, n)
+    g, o = pair(n, n, n, tau=u.tau, periodic=(1, 1, 0), mask=mask, precision=precision)
+    shear = 1000.0 * u.dt  # 1000 /s in lattice units
+    vhalf = (n - 1) * shear * 0.5
+    both(g, o, "set_wall_velocity", 5, [-vhalf, 0, 0])
+    both(g, o, "set_wall_velocity", 4, [vhalf, 0, 0])
+    both(g, o, "init_equilibrium", 1.0, [0, 0, 0])
+    both(g, o, "set_force_limit", u.f_limit)
+    _one_cell(g, o, u, "RBC", (n / 2 + 0.3, n / 2 - 0.2, n / 2 + 0.1), rng, amp=0.0)
+    x0 = g.vertices()[0]
+    for _ in range(steps // 10):
+        both(g, o, "step", 10)
+    return g, o, x0
+
+
+def test_full_step_shear_fp64_100_steps():
+    g, o, x0 = _shear_case(hc.HC_FP64)
+    xg, vg, fg = g.vertices()
+    xo, vo, fo = o.vertices()
+    assert cases.rel_l2(xg, xo) < 1e-10
+    assert cases.rel_l2(xg - x0, xo - x0) < 1e-8
+    assert cases.rel_l2(vg, vo) < 1e-10
+    _, ug = g.fluid()
+    _, uo = o.fluid()
+    assert cases.rel_l2(ug, uo) < 1e-10
+    assert g.wall_hits() == o.wall_hits() == 0
+    g.check_finite()
+
+
+def test_full_step_shear_fp32_shifted():
+    g, o, x0 = _shear_case(hc.HC_FP32_SHIFTED)
+    xg, _, _ = g.vertices()
+    xo, _, _ = o.vertices()
+    _, ug = g.fluid()
+    _, uo = o.fluid()
+    assert cases.rel_l2(ug, uo) < 1e-5
+    assert cases.rel_l2(xg - x0, xo - x0) < 1e-4
+
+
+def _mixed(sims, rng_seed=8, nudge=0.0):
+    rng = np.random.default_rng(rng_seed)
+    u = cases.units()
+    for s in sims:
+        s.set_body_force([0, 0, 2e-6])
+        s.set_force_limit(u.f_limit)
+    for kind, centers in (("RBC", [(10, 10, 10), (28, 18, 22)]), ("PLT", [(20, 25, 5), (33, 5, 30)]),
+                          ("WBC", [(25, 12, 28)])):
+        model, v, t, p, inner = cases.mesh(kind, u, hc.build_mesh)
+        xs = np.stack([cases.perturb(cases.place(v, c, cases.rotation(rng)), rng, 0.03) for c in centers])
+        if nudge:
+            xs = xs * (1 + nudge * np.random.default_rng(1).uniform(-1, 1, xs.shape))
+        for s in sims:
+            tid = s.add_cell_type(model, v, t, p, inner)
+            s.add_cells(tid, xs)
+
+
+def test_full_step_mixed_suspension_body_force_fp64():
+    u = cases.units()
+    nx, ny, nz = 40, 30, 36
+    g, o = pair(nx, ny, nz, tau=u.tau)
+    _mixed([g, o])
+    # the oracle's own response to a 1e-15 relative nudge of the positions: the
+    # conditioning of each observable in this strongly coupled case
+    o2 = OracleSim(nx, ny, nz, u.tau)
+    _mixed([o2], nudge=1e-15)
+    for _ in range(5):
+        both(g, o, "step", 10)
+        o2.step(10)
+    xg, vg, _ = g.vertices()
+    xo, vo, _ = o.vertices()
+    _, v2, _ = o2.vertices()
+    assert cases.rel_l2(xg, xo) < 1e-10
+    cond_v = cases.rel_l2(v2, vo)
+    assert cases.rel_l2(vg, vo) < max(1e-10, 10 * cond_v)
+    _, ug = g.fluid()
+    _, uo = o.fluid()
+    assert cases.rel_l2(ug, uo) < 1e-10
+
+
+def test_stretch_extra_force_fp64():
+    rng = np.random.default_rng(9)
+    u = cases.units()
+    g, o = pair(36, 28, 28, tau=u.tau)
+    both(g, o, "set_force_limit", u.f_limit)
+    x = _one_cell(g, o, u, "RBC", (18, 14, 14), rng, amp=0.0)
+    order = np.argsort(x[:, 0])
+    lo, hi = order[:7], order[-7:]
+    F = 100e-12 / u.df
+    ids = np.concatenate([lo, hi]).astype(np.int64)
+    forces = np.zeros((14, 3))
+    forces[:7, 0] = -F / 7
+    forces[7:, 0] = F / 7
+    g.set_extra_vertex_force(ids, forces)
+    for _ in range(30):
+        # the oracle applies the stretch force the way the reference driver does: before every step
+        xo, vo, fo = o.vertices()
+        fo[ids] += forces
+        o.set_vertices(None, None, fo)
+        o.step(1)
+    g.step(30)
+    xg, _, _ = g.vertices()
+    xo, _, _ = o.vertices()
+    assert cases.rel_l2(xg, xo) < 1e-10
