@@ -1,0 +1,179 @@
+"""Benchmark of the HemoCell IB-LBM time step on one MI355X.
+
+Workload (the headline "Ht 30%" configuration): a 256x256x512 periodic D3Q19 box,
+body force along z, 30% hematocrit of 642-vertex RBCs, 66-vertex platelets
+(1 per 10 RBC) and 2562-vertex WBCs (1 per 500 RBC), seeded non-overlapping
+placement; dx = 0.5 um, dt = 1e-7 s (tau = 1.82), reference default stiffness.
+
+One step = HemoCell::iterate(): spread, collide-stream, interpolate + advance,
+mechanics, force reset.  The timed region runs K steps back to back with inputs
+resident in HBM; the device stream is synchronised on both sides.  `value` is
+MLUPS (fluid nodes x steps / s) of the FP32-shifted production form; the FP64
+form is measured in the same run.  The collide-stream kernel's mean duration is
+taken from HIP events recorded on the library's stream around every launch in
+the timed region.
+
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision both|fp32|fp64] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS & % HBM roofline, FP32-shifted vs FP64 at Ht 30%; p50/p99 case latency"
+# algorithmic bytes per fluid-node update of the collide-stream kernel:
+# 19 population reads + 19 writes + 3 IBM-force reads + 3 writes zeroing the other force buffer
+BYTES_PER_LUP = {"fp32": 44 * 4, "fp64": 44 * 8}
+
+
+def build_case(hc, susp, shape, precision, seed=20261202):
+    units = hc.lbm_base_parameters(5e-7, 1e-7)
+    types = susp.cell_types(units)
+    cells = susp.pack(shape, types, seed=seed)
+    sim = hc.Simulation(0)
+    sim.lattice_init(*shape, units.tau, periodic=(1, 1, 1), precision=precision)
+    sim.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    sim.set_body_force((0.0, 0.0, 1e-6))
+    sim.set_force_limit(units.f_limit)
+    for name in ("RBC", "PLT", "WBC"):
+        if name not in cells:
+            continue
+        t = types[name]
+        tid = sim.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner)
+        sim.add_cells(tid, cells[name])
+    return sim, units, types, cells
+
+
+def run_gpu(hc, susp, shape, precision, steps, warmup):
+    label = "fp32" if precision == hc.HC_FP32_SHIFTED else "fp64"
+    print("[bench] building %s case %s" % (label, shape), file=sys.stderr, flush=True)
+    t0 = time.time()
+    sim, units, types, cells = build_case(hc, susp, shape, precision)
+    nv = sim.num_vertices
+    print("[bench] %d cells, %d vertices, Ht %.3f, setup %.1fs" % (
+        sum(len(c) for c in cells.values()), nv, susp.hematocrit(types, cells, shape), time.time() - t0),
+        file=sys.stderr, flush=True)
+    sim.step(warmup)
+    sim.synchronize()
+    sim.profile_enable(True)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    dt = time.perf_counter() - t1
+    stages = {}
+    for name, k in (("spread", hc.PROF_SPREAD), ("collide_stream", hc.PROF_COLLIDE),
+                    ("interp_advance", hc.PROF_INTERP), ("mechanics", hc.PROF_MECH)):
+        ms, cnt = sim.profile_read(k)
+        stages[name] = ms / max(cnt, 1)
+    sim.profile_enable(False)
+    sim.check_finite()
+    N = int(np.prod(shape))
+    res = dict(label=label, ms_per_step=1e3 * dt / steps, mlups=N * steps / dt / 1e6,
+               vertex_updates_per_s=nv * steps / dt, stage_ms=stages, vertices=nv, nodes=N,
+               cells={k: int(len(v)) for k, v in cells.items()})
+    res["collide_ms"] = stages["collide_stream"]
+    res["collide_gbs"] = BYTES_PER_LUP[label] * N / (stages["collide_stream"] * 1e-3) / 1e9
+    print("[bench] %s: %.3f ms/step, %.1f MLUPS, collide %.3f ms = %.0f GB/s" % (
+        label, res["ms_per_step"], res["mlups"], res["collide_ms"], res["collide_gbs"]), file=sys.stderr, flush=True)
+    sim.close()
+    return res
+
+
+def cpu_baseline(hc, susp, seconds=12.0):
+    """FP64 CPU oracle (a restatement of the reference step, OpenMP over nodes and
+    cells) on a 64x64x128 crop of the workload at the same hematocrit."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import OracleSim, library
+    shape = (64, 64, 128)
+    units = hc.lbm_base_parameters(5e-7, 1e-7)
+    types = susp.cell_types(units)
+    cells = susp.pack(shape, types, seed=7)
+    o = OracleSim(*shape, units.tau)
+    o.init_equilibrium(1.0, (0, 0, 0))
+    o.set_body_force((0, 0, 1e-6))
+    o.set_force_limit(units.f_limit)
+    for name in ("RBC", "PLT", "WBC"):
+        if name in cells:
+            t = types[name]
+            tid = o.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner)
+            o.add_cells(tid, cells[name])
+    o.step(1)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds and n < 200:
+        o.step(1)
+        n += 1
+    el = time.perf_counter() - t0
+    threads = library().ho_num_threads()
+    N = int(np.prod(shape))
+    return dict(value=N * n / el / 1e6, unit="MLUPS", cores=int(threads), kind="port",
+                sample="64x64x128 periodic crop at Ht %.2f (%d RBC, %d PLT, %d WBC), %d steps FP64 after 1 warm-up" % (
+                    susp.hematocrit(types, cells, shape), len(cells["RBC"]), len(cells.get("PLT", [])),
+                    len(cells.get("WBC", [])), n))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", default="both", choices=["both", "fp32", "fp64"])
+    ap.add_argument("--shape", default="256,256,512")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import hemocell_amd as hc
+    from hemocell_amd import suspension as susp
+    shape = tuple(int(s) for s in args.shape.split(","))
+    res = {}
+    if args.precision in ("both", "fp32"):
+        res["fp32"] = run_gpu(hc, susp, shape, hc.HC_FP32_SHIFTED, args.steps, args.warmup)
+    if args.precision in ("both", "fp64"):
+        res["fp64"] = run_gpu(hc, susp, shape, hc.HC_FP64, args.steps, args.warmup)
+    head = res.get("fp32") or res["fp64"]
+    line = {
+        "metric": METRIC,
+        "value": round(head["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head["label"],
+        "data": "synthetic",
+        "config": {"workload": "periodic %dx%dx%d body-force suspension, Ht 30%%: RBC 642v, PLT 66v 1:10, "
+                               "WBC 2562v 1:500, seeded non-overlapping" % shape,
+                   "cells": head["cells"], "vertices": head["vertices"], "precision": head["label"]},
+        "roofline": {"bound": "hbm", "achieved": round(head["collide_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["collide_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_collide_stream", "mean_ms": round(head["collide_ms"], 4),
+                     "bytes_per_lup": BYTES_PER_LUP[head["label"]]},
+        "vertex_updates_per_s": round(head["vertex_updates_per_s"], 1),
+        "stage_ms": {k: round(v, 4) for k, v in head["stage_ms"].items()},
+    }
+    if "fp64" in res and "fp32" in res:
+        f64 = res["fp64"]
+        line["fp64"] = {"value": round(f64["mlups"], 2), "ms_per_step": round(f64["ms_per_step"], 4),
+                        "collide_gbs": round(f64["collide_gbs"], 1),
+                        "collide_frac": round(f64["collide_gbs"] / HBM_PEAK_GBS, 4),
+                        "stage_ms": {k: round(v, 4) for k, v in f64["stage_ms"].items()}}
+        line["fp32_over_fp64"] = round(res["fp32"]["mlups"] / f64["mlups"], 3)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(hc, susp)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

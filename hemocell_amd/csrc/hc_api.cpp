@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -51,6 +52,7 @@ struct hc_ctx {
   double tau = 1.0;
   void* f = nullptr;
   void* Fibm[2] = {nullptr, nullptr};
+  void* ufield = nullptr;  // fluid velocity per node, filled before the interpolation
   uint8_t* type = nullptr;
   uint32_t* vel_nodes = nullptr;  // nodes with a velocity boundary
   int n_vel = 0;
@@ -67,7 +69,10 @@ struct hc_ctx {
   int64_t nv = 0, vcap = 0;
   double *x = nullptr, *v = nullptr, *fv = nullptr, *frep = nullptr, *bend = nullptr;
   int64_t* d_cell_base = nullptr;
+  int32_t* d_cell_nv = nullptr;
   bool cells_dirty = true;
+  int lds_limit = 65536;        // dynamic LDS per workgroup available on the device
+  bool spread_per_cell = true;  // LDS-staged spread (false: one thread per vertex, global atomics)
   // extra (stretch) forces
   int64_t* d_extra_ids = nullptr;
   double* d_extra_f = nullptr;
@@ -195,6 +200,15 @@ hc_status sync_cells(hc_ctx* ctx) {
   hc_status s = dalloc(ctx, (void**)&ctx->d_cell_base, sizeof(int64_t) * std::max<size_t>(nc, 1), "cell bases");
   if (s != HC_OK) return s;
   if (nc) HC_TRY(hipMemcpy(ctx->d_cell_base, ctx->cell_base.data(), sizeof(int64_t) * nc, hipMemcpyHostToDevice), "hipMemcpy");
+  if (ctx->d_cell_nv) hipFree(ctx->d_cell_nv);
+  ctx->d_cell_nv = nullptr;
+  s = dalloc(ctx, (void**)&ctx->d_cell_nv, sizeof(int32_t) * std::max<size_t>(nc, 1), "cell sizes");
+  if (s != HC_OK) return s;
+  {
+    std::vector<int32_t> cnv(nc);
+    for (size_t c = 0; c < nc; ++c) cnv[c] = ctx->types[ctx->cell_type[c]]->host.nv;
+    if (nc) HC_TRY(hipMemcpy(ctx->d_cell_nv, cnv.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice), "hipMemcpy");
+  }
   for (size_t t = 0; t < ctx->types.size(); ++t) {
     TypeBufs& tb = *ctx->types[t];
     std::vector<int32_t> list;
@@ -228,8 +242,17 @@ hc_status do_spread(hc_ctx* ctx) {
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_SPREAD, &ev);
   if (ctx->n_extra > 0) HC_TRY(launch_add_extra_force(ctx->stream, vdev(ctx), ctx->d_extra_ids, ctx->d_extra_f, ctx->n_extra), "extra force");
-  HC_TRY(launch_spread<R>(ctx->stream, vdev(ctx), (R*)ctx->Fibm[ctx->cur], ctx->type, ctx->L, ctx->f_limit,
-                          ctx->repulsion ? 1 : 0), "spread");
+  if (ctx->spread_per_cell) {
+    // LDS budget: room for the support box of a tumbling RBC in FP32, an aligned one in FP64
+    const int want = sizeof(R) == 4 ? 65536 : 98304;
+    HC_TRY(launch_spread_cells<R>(ctx->stream, vdev(ctx), (R*)ctx->Fibm[ctx->cur], ctx->type, ctx->L, ctx->d_cell_base,
+                                  ctx->d_cell_nv, (int)ctx->cell_base.size(), ctx->f_limit, ctx->repulsion ? 1 : 0,
+                                  std::min(want, ctx->lds_limit)),
+           "spread");
+  } else {
+    HC_TRY(launch_spread<R>(ctx->stream, vdev(ctx), (R*)ctx->Fibm[ctx->cur], ctx->type, ctx->L, ctx->f_limit,
+                            ctx->repulsion ? 1 : 0), "spread");
+  }
   prof_end(ctx, HC_PROF_SPREAD, ev);
   return HC_OK;
 }
@@ -256,8 +279,12 @@ hc_status do_interp_advance(hc_ctx* ctx, int do_interp, int do_advance) {
   if (ctx->nv == 0) return HC_OK;
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_INTERP, &ev);
-  HC_TRY(launch_interp_advance<R>(ctx->stream, vdev(ctx), (const R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], ctx->type,
-                                  ctx->L, ctx->phase, ctx->g, do_interp, do_advance, ctx->d_wall_hits),
+  if (do_interp)
+    HC_TRY(launch_velocity_field<R>(ctx->stream, (const R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], (R*)ctx->ufield,
+                                    ctx->L, ctx->phase, ctx->general, ctx->g),
+           "velocity_field");
+  HC_TRY(launch_interp_advance<R>(ctx->stream, vdev(ctx), (const R*)ctx->ufield, ctx->type, ctx->L, do_interp,
+                                  do_advance, ctx->d_wall_hits),
          "interp_advance");
   prof_end(ctx, HC_PROF_INTERP, ev);
   return HC_OK;
@@ -342,6 +369,11 @@ hc_status hc_create(int device, hc_ctx** out) {
     return HC_OOM;
   }
   hipMemset(ctx->d_wall_hits, 0, sizeof(unsigned long long));
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    ctx->lds_limit = lds - 256;  // the kernel's own static LDS
+  const char* sp = std::getenv("HC_SPREAD");
+  if (sp && std::string(sp) == "vertex") ctx->spread_per_cell = false;
   *out = ctx;
   return HC_OK;
 }
@@ -350,7 +382,7 @@ void hc_destroy(hc_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
+  void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
                 ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag};
   for (void* p : ps)
     if (p) hipFree(p);
@@ -412,6 +444,7 @@ hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* no
   if ((s = dalloc(ctx, &ctx->f, rs * 19 * ctx->L.stride, "populations")) != HC_OK) return s;
   for (int b = 0; b < 2; ++b)
     if ((s = dalloc(ctx, &ctx->Fibm[b], rs * 3 * ctx->L.stride, "force field")) != HC_OK) return s;
+  if ((s = dalloc(ctx, &ctx->ufield, rs * 3 * ctx->L.stride, "velocity field")) != HC_OK) return s;
   HC_TRY(hipMemsetAsync(ctx->f, 0, rs * 19 * ctx->L.stride, ctx->stream), "hipMemsetAsync");
   for (int b = 0; b < 2; ++b) HC_TRY(hipMemsetAsync(ctx->Fibm[b], 0, rs * 3 * ctx->L.stride, ctx->stream), "hipMemsetAsync");
   if (has_walls) {

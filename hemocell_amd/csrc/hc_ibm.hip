@@ -33,9 +33,10 @@ __device__ __forceinline__ int64_t wrap_mod(int64_t c, int n) {
   return m < 0 ? m + n : m;
 }
 
-// phi2 support of position p; returns the number of nodes (<= 8).
+// phi2 support of position p; returns the number of nodes (<= 8).  When `loc`
+// is given it receives each node's unwrapped coordinates.
 __device__ int ibm_support(double p0, double p1, double p2, const LatticeDev& L, const uint8_t* type,
-                           int64_t (&idx)[8], double (&w)[8]) {
+                           int64_t (&idx)[8], double (&w)[8], int (*loc)[3] = nullptr) {
   if (!(isfinite(p0) && isfinite(p1) && isfinite(p2))) return 0;
   const double p[3] = {p0, p1, p2};
   const int dims[3] = {L.nx, L.ny, L.nz};
@@ -65,6 +66,11 @@ __device__ int ibm_support(double p0, double p1, double p2, const LatticeDev& L,
         if (cnt < 8) {
           idx[cnt] = n;
           w[cnt] = ww;
+          if (loc) {
+            loc[cnt][0] = (int)nd[0];
+            loc[cnt][1] = (int)nd[1];
+            loc[cnt][2] = (int)nd[2];
+          }
           total += ww;
           ++cnt;
         }
@@ -111,44 +117,40 @@ __global__ __launch_bounds__(BLOCK) void k_spread(VertexDev V, R* __restrict__ F
   }
 }
 
-// post-stream populations of node n (phase of the AA pattern as in hc_lbm.hip)
-template <typename R>
-__device__ __forceinline__ void node_velocity(const R* __restrict__ f, const R* __restrict__ Fcur, const LatticeDev& L,
-                                              int phase, int64_t n, R g0, R g1, R g2, R (&u)[3]) {
+// Fluid velocity u = j/rho + F/2 of every node from its post-stream populations
+// (phase of the AA pattern as in hc_lbm.hip) and the force field the spread
+// filled this step: ExternalForceDynamics::computeVelocity restated.  One
+// coalesced pass; the interpolation then reads 3 values per support node.
+template <typename R, bool PHASE1, bool CHECK>
+__global__ __launch_bounds__(BLOCK) void k_velocity_field(const R* __restrict__ f, const R* __restrict__ Fcur,
+                                                          R* __restrict__ uf, LatticeDev L, double g0, double g1,
+                                                          double g2) {
+  const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= (uint32_t)L.N) return;
   const int64_t S = L.stride;
+  const int z = (int)(n % (uint32_t)L.nz);
+  const uint32_t t = n / (uint32_t)L.nz;
+  const int y = (int)(t % (uint32_t)L.ny);
+  const int x = (int)(t / (uint32_t)L.ny);
+  const NbrOff o = neighbour_offsets(x, y, z, L);
   R fl[19];
-  if (phase == 0) {
-#pragma unroll
-    for (int i = 0; i < 19; ++i) fl[i] = f[i * S + n];
-  } else {
-    const int z = (int)(n % L.nz);
-    const int64_t t = n / L.nz;
-    const int y = (int)(t % L.ny), x = (int)(t / L.ny);
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-      const int X = shift_coord(x, -cx(i), L.nx, L.periodic[0]);
-      const int Y = shift_coord(y, -cy(i), L.ny, L.periodic[1]);
-      const int Z = shift_coord(z, -cz(i), L.nz, L.periodic[2]);
-      fl[i] = (X < 0 || Y < 0 || Z < 0) ? R(0)
-                                        : f[opp(i) * S + (int64_t)Z + (int64_t)L.nz * ((int64_t)Y + (int64_t)L.ny * X)];
-    }
-  }
+  aa_load<R, PHASE1, CHECK>(f, S, n, o, fl, std::make_integer_sequence<int, 19>{});
   R rhoBar, j0, j1, j2;
   rhobar_j(fl, rhoBar, j0, j1, j2);
   const R invRho = R(1) / (rhoBar + R(1));
-  u[0] = j0 * invRho + (g0 + Fcur[n]) / R(2);
-  u[1] = j1 * invRho + (g1 + Fcur[S + n]) / R(2);
-  u[2] = j2 * invRho + (g2 + Fcur[2 * S + n]) / R(2);
+  uf[n] = j0 * invRho + (R(g0) + Fcur[n]) / R(2);
+  uf[S + n] = j1 * invRho + (R(g1) + Fcur[S + n]) / R(2);
+  uf[2 * S + n] = j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2);
 }
 
-template <typename R, int PHASE>
-__global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* __restrict__ f, const R* __restrict__ Fcur,
-                                                          const uint8_t* __restrict__ type, LatticeDev L, double g0,
-                                                          double g1, double g2, int do_interp, int do_advance,
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* __restrict__ uf,
+                                                          const uint8_t* __restrict__ type, LatticeDev L,
+                                                          int do_interp, int do_advance,
                                                           unsigned long long* wall_hits) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= V.n) return;
-  const int64_t vs = V.vs;
+  const int64_t vs = V.vs, S = L.stride;
   double x0 = V.x[i], x1 = V.x[vs + i], x2 = V.x[2 * vs + i];
   double v0, v1, v2;
   if (do_interp) {
@@ -159,11 +161,10 @@ __global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* 
     v1 = 0.0;
     v2 = 0.0;
     for (int k = 0; k < cnt; ++k) {
-      R u[3];
-      node_velocity<R>(f, Fcur, L, PHASE, idx[k], R(g0), R(g1), R(g2), u);
-      v0 += (double)u[0] * w[k];
-      v1 += (double)u[1] * w[k];
-      v2 += (double)u[2] * w[k];
+      const int64_t n = idx[k];
+      v0 += (double)uf[n] * w[k];
+      v1 += (double)uf[S + n] * w[k];
+      v2 += (double)uf[2 * S + n] * w[k];
     }
     V.v[i] = v0;
     V.v[vs + i] = v1;
@@ -196,6 +197,128 @@ __global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* 
   }
 }
 
+// Cap the vertex force to f_limit in place (hemoCellParticleField.cpp:847-852) and
+// return the force to spread (plus the repulsion force when enabled).
+__device__ __forceinline__ void spread_force(const VertexDev& V, int64_t i, double f_limit, int use_rep, double& t0,
+                                             double& t1, double& t2) {
+  const int64_t vs = V.vs;
+  double f0 = V.f[i], f1 = V.f[vs + i], f2 = V.f[2 * vs + i];
+  const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+  if (fm > f_limit) {
+    const double s = f_limit / fm;
+    f0 *= s;
+    f1 *= s;
+    f2 *= s;
+    V.f[i] = f0;
+    V.f[vs + i] = f1;
+    V.f[2 * vs + i] = f2;
+  }
+  t0 = f0;
+  t1 = f1;
+  t2 = f2;
+  if (use_rep) {
+    t0 = V.frep[i] + f0;
+    t1 = V.frep[vs + i] + f1;
+    t2 = V.frep[2 * vs + i] + f2;
+  }
+}
+
+__device__ __forceinline__ void lds_add(float* p, float v) { atomicAdd(p, v); }
+__device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
+
+// Force spreading, one workgroup per cell: the cell's contributions are summed
+// in an LDS box covering its support (LDS atomics, many vertices share nodes),
+// then every non-zero node of the box is added to the force field with one
+// global atomic.  A cell whose box does not fit the LDS budget spreads with
+// global atomics directly.
+template <typename R, int BS>
+__global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict__ Fibm, const uint8_t* __restrict__ type,
+                                                     LatticeDev L, const int64_t* __restrict__ cell_base,
+                                                     const int32_t* __restrict__ cell_nv, double f_limit, int use_rep,
+                                                     int lds_nodes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  R* box = (R*)smem;
+  __shared__ int bmin[3], bmax[3], bad;
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const int64_t base = cell_base[cell];
+  const int nv = cell_nv[cell];
+  const int64_t vs = V.vs, S = L.stride;
+  if (tid == 0) {
+    for (int a = 0; a < 3; ++a) {
+      bmin[a] = 0x7fffffff;
+      bmax[a] = -0x7fffffff;
+    }
+    bad = 0;
+  }
+  __syncthreads();
+  int lmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, lmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
+  bool lbad = false;
+  for (int k = tid; k < nv; k += BS) {
+    const int64_t i = base + k;
+    for (int a = 0; a < 3; ++a) {
+      const double p = V.x[a * vs + i];
+      if (!(fabs(p) < 1.0e9)) {
+        lbad = true;
+        continue;
+      }
+      const int c = (int)floor(p + 0.5);
+      lmin[a] = min(lmin[a], c);
+      lmax[a] = max(lmax[a], c);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    atomicMin(&bmin[a], lmin[a]);
+    atomicMax(&bmax[a], lmax[a]);
+  }
+  if (lbad) atomicOr(&bad, 1);
+  __syncthreads();
+  const int o0 = bmin[0] - 1, o1 = bmin[1] - 1, o2 = bmin[2] - 1;
+  const int s0 = bmax[0] - bmin[0] + 3, s1 = bmax[1] - bmin[1] + 3, s2 = bmax[2] - bmin[2] + 3;
+  const int64_t total = (int64_t)s0 * s1 * s2;
+  const bool direct = bad || total > lds_nodes || s0 <= 0;
+  if (!direct)
+    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = R(0);
+  __syncthreads();
+  for (int k = tid; k < nv; k += BS) {
+    const int64_t i = base + k;
+    int64_t idx[8];
+    double w[8];
+    int loc[8][3];
+    const int cnt = ibm_support(V.x[i], V.x[vs + i], V.x[2 * vs + i], L, type, idx, w, loc);
+    double t0, t1, t2;
+    spread_force(V, i, f_limit, use_rep, t0, t1, t2);
+    for (int q = 0; q < cnt; ++q) {
+      if (direct) {
+        atomic_add(&Fibm[idx[q]], (R)(t0 * w[q]));
+        atomic_add(&Fibm[S + idx[q]], (R)(t1 * w[q]));
+        atomic_add(&Fibm[2 * S + idx[q]], (R)(t2 * w[q]));
+      } else {
+        const int64_t e = (int64_t)(loc[q][2] - o2) + (int64_t)s2 * ((loc[q][1] - o1) + (int64_t)s1 * (loc[q][0] - o0));
+        lds_add(&box[e], (R)(t0 * w[q]));
+        lds_add(&box[total + e], (R)(t1 * w[q]));
+        lds_add(&box[2 * total + e], (R)(t2 * w[q]));
+      }
+    }
+  }
+  if (direct) return;
+  __syncthreads();
+  const int dims[3] = {L.nx, L.ny, L.nz};
+  for (int64_t e = tid; e < total; e += BS) {
+    const R a = box[e], b = box[total + e], c = box[2 * total + e];
+    if (a == R(0) && b == R(0) && c == R(0)) continue;
+    const int lz = (int)(e % s2);
+    const int64_t r = e / s2;
+    const int ly = (int)(r % s1), lx = (int)(r / s1);
+    const int64_t g[3] = {(int64_t)o0 + lx, (int64_t)o1 + ly, (int64_t)o2 + lz};
+    int64_t wc[3];
+    for (int q = 0; q < 3; ++q) wc[q] = (g[q] >= 0 && g[q] < dims[q]) ? g[q] : wrap_mod(g[q], dims[q]);
+    const int64_t n = wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0]);
+    if (a != R(0)) atomic_add(&Fibm[n], a);
+    if (b != R(0)) atomic_add(&Fibm[S + n], b);
+    if (c != R(0)) atomic_add(&Fibm[2 * S + n], c);
+  }
+}
+
 __global__ void k_add_extra(VertexDev V, const int64_t* ids, const double* forces, int n) {
   const int k = blockIdx.x * BLOCK + threadIdx.x;
   if (k >= n) return;
@@ -219,16 +342,35 @@ hipError_t launch_spread(hipStream_t s, const VertexDev& V, R* Fibm, const uint8
 }
 
 template <typename R>
-hipError_t launch_interp_advance(hipStream_t s, const VertexDev& V, const R* f, const R* Fcur, const uint8_t* type,
-                                 const LatticeDev& L, int phase, const double g[3], int do_interp, int do_advance,
-                                 unsigned long long* wall_hits) {
+hipError_t launch_spread_cells(hipStream_t s, const VertexDev& V, R* Fibm, const uint8_t* type, const LatticeDev& L,
+                               const int64_t* cell_base, const int32_t* cell_nv, int ncells, double f_limit,
+                               int use_rep, int lds_bytes) {
+  if (ncells <= 0) return hipSuccess;
+  const int lds_nodes = lds_bytes / (3 * (int)sizeof(R));
+  hipLaunchKernelGGL((k_spread_cells<R, 256>), dim3((unsigned)ncells), dim3(256), (size_t)lds_bytes, s, V, Fibm, type,
+                     L, cell_base, cell_nv, f_limit, use_rep, lds_nodes);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_velocity_field(hipStream_t s, const R* f, const R* Fcur, R* uf, const LatticeDev& L, int phase,
+                                 bool general, const double g[3]) {
+  dim3 gr = grid_for(L.N);
+  if (phase) {
+    if (general) hipLaunchKernelGGL((k_velocity_field<R, true, true>), gr, dim3(BLOCK), 0, s, f, Fcur, uf, L, g[0], g[1], g[2]);
+    else hipLaunchKernelGGL((k_velocity_field<R, true, false>), gr, dim3(BLOCK), 0, s, f, Fcur, uf, L, g[0], g[1], g[2]);
+  } else {
+    hipLaunchKernelGGL((k_velocity_field<R, false, false>), gr, dim3(BLOCK), 0, s, f, Fcur, uf, L, g[0], g[1], g[2]);
+  }
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_interp_advance(hipStream_t s, const VertexDev& V, const R* uf, const uint8_t* type,
+                                 const LatticeDev& L, int do_interp, int do_advance, unsigned long long* wall_hits) {
   if (V.n == 0) return hipSuccess;
-  if (phase)
-    hipLaunchKernelGGL((k_interp_advance<R, 1>), grid_for(V.n), dim3(BLOCK), 0, s, V, f, Fcur, type, L, g[0], g[1], g[2],
-                       do_interp, do_advance, wall_hits);
-  else
-    hipLaunchKernelGGL((k_interp_advance<R, 0>), grid_for(V.n), dim3(BLOCK), 0, s, V, f, Fcur, type, L, g[0], g[1], g[2],
-                       do_interp, do_advance, wall_hits);
+  hipLaunchKernelGGL((k_interp_advance<R>), grid_for(V.n), dim3(BLOCK), 0, s, V, uf, type, L, do_interp, do_advance,
+                     wall_hits);
   return hipGetLastError();
 }
 
@@ -238,12 +380,17 @@ hipError_t launch_add_extra_force(hipStream_t s, const VertexDev& V, const int64
   return hipGetLastError();
 }
 
-template hipError_t launch_spread<float>(hipStream_t, const VertexDev&, float*, const uint8_t*, const LatticeDev&, double, int);
-template hipError_t launch_spread<double>(hipStream_t, const VertexDev&, double*, const uint8_t*, const LatticeDev&, double, int);
-template hipError_t launch_interp_advance<float>(hipStream_t, const VertexDev&, const float*, const float*, const uint8_t*,
-                                                 const LatticeDev&, int, const double*, int, int, unsigned long long*);
-template hipError_t launch_interp_advance<double>(hipStream_t, const VertexDev&, const double*, const double*,
-                                                  const uint8_t*, const LatticeDev&, int, const double*, int,
-                                                  int, unsigned long long*);
+#define HC_INSTANTIATE(R)                                                                                     \
+  template hipError_t launch_spread<R>(hipStream_t, const VertexDev&, R*, const uint8_t*, const LatticeDev&, double,  \
+                                       int);                                                                         \
+  template hipError_t launch_spread_cells<R>(hipStream_t, const VertexDev&, R*, const uint8_t*, const LatticeDev&,    \
+                                             const int64_t*, const int32_t*, int, double, int, int);                  \
+  template hipError_t launch_velocity_field<R>(hipStream_t, const R*, const R*, R*, const LatticeDev&, int, bool,     \
+                                               const double*);                                                       \
+  template hipError_t launch_interp_advance<R>(hipStream_t, const VertexDev&, const R*, const uint8_t*,               \
+                                               const LatticeDev&, int, int, unsigned long long*);
+HC_INSTANTIATE(float)
+HC_INSTANTIATE(double)
+#undef HC_INSTANTIATE
 
 }  // namespace hcg

@@ -44,9 +44,15 @@ template <typename R>
 hipError_t launch_spread(hipStream_t s, const VertexDev& V, R* Fibm, const uint8_t* type, const LatticeDev& L,
                          double f_limit, int use_repulsion);
 template <typename R>
-hipError_t launch_interp_advance(hipStream_t s, const VertexDev& V, const R* f, const R* Fcur, const uint8_t* type,
-                                 const LatticeDev& L, int phase, const double g[3], int do_interp, int do_advance,
-                                 unsigned long long* wall_hits);
+hipError_t launch_spread_cells(hipStream_t s, const VertexDev& V, R* Fibm, const uint8_t* type, const LatticeDev& L,
+                               const int64_t* cell_base, const int32_t* cell_nv, int ncells, double f_limit,
+                               int use_repulsion, int lds_bytes);
+template <typename R>
+hipError_t launch_velocity_field(hipStream_t s, const R* f, const R* Fcur, R* uf, const LatticeDev& L, int phase,
+                                 bool general, const double g[3]);
+template <typename R>
+hipError_t launch_interp_advance(hipStream_t s, const VertexDev& V, const R* uf, const uint8_t* type,
+                                 const LatticeDev& L, int do_interp, int do_advance, unsigned long long* wall_hits);
 hipError_t launch_add_extra_force(hipStream_t s, const VertexDev& V, const int64_t* ids, const double* forces, int n);
 
 // mechanics (hc_mech.hip)

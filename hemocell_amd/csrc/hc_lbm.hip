@@ -169,76 +169,6 @@ __device__ __forceinline__ void coords(int64_t n, const LatticeDev& L, int& x, i
   x = (int)(t / L.ny);
 }
 
-// Index offsets of the six axis neighbours of node (x, y, z); a neighbour
-// outside a non-periodic domain gets INVALID.
-constexpr int INVALID = 0x40000000;
-struct NbrOff {
-  int xm, xp, ym, yp, zm, zp;
-};
-__device__ __forceinline__ NbrOff neighbour_offsets(int x, int y, int z, const LatticeDev& L) {
-  const int nyz = L.ny * L.nz;
-  NbrOff o;
-  o.xm = x > 0 ? -nyz : (L.periodic[0] ? (L.nx - 1) * nyz : INVALID);
-  o.xp = x < L.nx - 1 ? nyz : (L.periodic[0] ? -(L.nx - 1) * nyz : INVALID);
-  o.ym = y > 0 ? -L.nz : (L.periodic[1] ? (L.ny - 1) * L.nz : INVALID);
-  o.yp = y < L.ny - 1 ? L.nz : (L.periodic[1] ? -(L.ny - 1) * L.nz : INVALID);
-  o.zm = z > 0 ? -1 : (L.periodic[2] ? L.nz - 1 : INVALID);
-  o.zp = z < L.nz - 1 ? 1 : (L.periodic[2] ? -(L.nz - 1) : INVALID);
-  return o;
-}
-// offset of node n + (DX, DY, DZ); sets ok = false when it leaves the domain
-template <int DX, int DY, int DZ, bool CHECK>
-__device__ __forceinline__ int nbr_offset(const NbrOff& o, bool& ok) {
-  int d = 0;
-  ok = true;
-  if constexpr (DX < 0) { d += o.xm; if (CHECK) ok &= o.xm != INVALID; }
-  if constexpr (DX > 0) { d += o.xp; if (CHECK) ok &= o.xp != INVALID; }
-  if constexpr (DY < 0) { d += o.ym; if (CHECK) ok &= o.ym != INVALID; }
-  if constexpr (DY > 0) { d += o.yp; if (CHECK) ok &= o.yp != INVALID; }
-  if constexpr (DZ < 0) { d += o.zm; if (CHECK) ok &= o.zm != INVALID; }
-  if constexpr (DZ > 0) { d += o.zp; if (CHECK) ok &= o.zp != INVALID; }
-  return d;
-}
-
-// AA-pattern read of the 19 post-stream populations of node n
-template <typename R, bool ODD, bool CHECK, int... Is>
-__device__ __forceinline__ void aa_load(const R* __restrict__ f, int64_t S, uint32_t n, const NbrOff& o, R (&fl)[19],
-                                        std::integer_sequence<int, Is...>) {
-  auto one = [&](auto I) {
-    constexpr int i = decltype(I)::value;
-    if constexpr (!ODD) {
-      fl[i] = f[i * S + n];
-    } else {
-      bool ok;
-      const int d = nbr_offset<-cx(i), -cy(i), -cz(i), CHECK>(o, ok);
-      fl[i] = ok ? f[opp(i) * S + (uint32_t)(n + d)] : R(0);
-    }
-  };
-  (one(std::integral_constant<int, Is>{}), ...);
-}
-
-// AA-pattern write of the 19 post-collision populations of node n
-template <typename R, bool ODD, bool CHECK, int... Is>
-__device__ __forceinline__ void aa_store(R* __restrict__ f, int64_t S, uint32_t n, const NbrOff& o, const R (&fl)[19],
-                                         std::integer_sequence<int, Is...>) {
-  auto one = [&](auto I) {
-    constexpr int i = decltype(I)::value;
-    if constexpr (!ODD) {
-      f[opp(i) * S + n] = fl[i];
-    } else {
-      bool ok;
-      const int d = nbr_offset<cx(i), cy(i), cz(i), CHECK>(o, ok);
-      if (ok) f[i * S + (uint32_t)(n + d)] = fl[i];
-      if constexpr (CHECK) {
-        bool in;
-        (void)nbr_offset<-cx(i), -cy(i), -cz(i), CHECK>(o, in);
-        if (!in) f[i * S + n] = R(0);  // nothing arrives from outside the domain
-      }
-    }
-  };
-  (one(std::integral_constant<int, Is>{}), ...);
-}
-
 // GENERAL: node-type mask present and/or some axis non-periodic.  Velocity
 // boundary nodes are skipped here and updated by k_velocity_nodes (each node
 // only touches its own 19 slots, so the two kernels may run in either order).

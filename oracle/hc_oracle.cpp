@@ -42,6 +42,9 @@
 #include <vector>
 #include <algorithm>
 #include <stdexcept>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace {
 
@@ -897,6 +900,13 @@ void ho_type_tables(void* h, int t, int* edge, int* edge_tris, int* edge_outer, 
   scalars4[0] = ct.volume_eq; scalars4[1] = ct.area_mean_eq; scalars4[2] = ct.edge_mean_eq; scalars4[3] = ct.angle_mean_eq;
 }
 
+int ho_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 long ho_num_vertices(void* h) { return ((Oracle*)h)->NV(); }
 long ho_wall_hits(void* h) { return ((Oracle*)h)->wall_hits; }
 

@@ -32,6 +32,7 @@ _SIG = {
     "ho_add_cells": (_INT, [_P, _INT, _INT, _D]),
     "ho_type_sizes": (None, [_P, _INT, _I32]),
     "ho_type_tables": (None, [_P, _INT, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _D, _D]),
+    "ho_num_threads": (_INT, []),
     "ho_num_vertices": (ctypes.c_long, [_P]),
     "ho_wall_hits": (ctypes.c_long, [_P]),
     "ho_get_populations": (None, [_P, _D]),
