@@ -736,10 +736,16 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u) {
     if (dr) hipFree(dr);
     return s;
   }
+  CollideArgs a;
+  a.omega = 1.0 / ctx->tau;
+  for (int c = 0; c < 3; ++c) a.g[c] = ctx->g[c];
+  std::memcpy(a.wall_u, ctx->wall_u, sizeof(a.wall_u));
   hipError_t e = fp32(ctx) ? launch_fluid_moments<float>(ctx->stream, (const float*)ctx->f,
-                                                         (const float*)ctx->Fibm[ctx->cur], ctx->L, ctx->phase, ctx->g, dr, du)
+                                                         (const float*)ctx->Fibm[ctx->cur], ctx->type, ctx->L,
+                                                         ctx->phase, a, dr, du)
                            : launch_fluid_moments<double>(ctx->stream, (const double*)ctx->f,
-                                                          (const double*)ctx->Fibm[ctx->cur], ctx->L, ctx->phase, ctx->g, dr, du);
+                                                          (const double*)ctx->Fibm[ctx->cur], ctx->type, ctx->L,
+                                                          ctx->phase, a, dr, du);
   if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, dr, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess && u) e = hipMemcpyAsync(u, du, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -24,8 +24,8 @@ hipError_t launch_natural_populations(hipStream_t s, const R* f, double* out, co
 template <typename R>
 hipError_t launch_store_populations(hipStream_t s, R* f, const double* in, const LatticeDev& L);
 template <typename R>
-hipError_t launch_fluid_moments(hipStream_t s, const R* f, const R* Fcur, const LatticeDev& L, int phase,
-                                const double g[3], double* rho, double* u);
+hipError_t launch_fluid_moments(hipStream_t s, const R* f, const R* Fcur, const uint8_t* type, const LatticeDev& L,
+                                int phase, const CollideArgs& a, double* rho, double* u);
 template <typename R>
 hipError_t launch_fill_zero(hipStream_t s, R* p, int64_t n);
 template <typename R>

@@ -272,9 +272,12 @@ __global__ void k_store(R* f, const double* in, LatticeDev L) {
   for (int i = 0; i < 19; ++i) f[i * L.stride + n] = (R)in[i * L.N + n];
 }
 
+// Density and velocity of every node for output; a velocity-boundary node
+// reports its imposed velocity and the density of the mass balance over its
+// known populations.
 template <typename R>
-__global__ void k_moments(const R* f, const R* Fcur, LatticeDev L, int phase, double g0, double g1, double g2,
-                          double* rho, double* u) {
+__global__ void k_moments(const R* f, const R* Fcur, const uint8_t* type, LatticeDev L, int phase, double g0,
+                          double g1, double g2, CollideArgs a, double* rho, double* u) {
   const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (n >= L.N) return;
   int x, y, z;
@@ -285,11 +288,29 @@ __global__ void k_moments(const R* f, const R* Fcur, LatticeDev L, int phase, do
   rhobar_j(fl, rhoBar, j0, j1, j2);
   const R invRho = R(1) / (rhoBar + R(1));
   const int64_t S = L.stride;
-  if (rho) rho[n] = (double)(rhoBar + R(1));
+  double u0 = (double)(j0 * invRho + (R(g0) + Fcur[n]) / R(2));
+  double u1 = (double)(j1 * invRho + (R(g1) + Fcur[S + n]) / R(2));
+  double u2 = (double)(j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2));
+  double rb = (double)rhoBar;
+  if (type && type[n] >= HC_NODE_VELOCITY) {
+    const int face = type[n] - HC_NODE_VELOCITY, axis = face >> 1, orient = (face & 1) ? 1 : -1;
+    double on = 0.0, nrm = 0.0;
+    for (int i = 0; i < 19; ++i) {
+      const int c = axis == 0 ? cx(i) : (axis == 1 ? cy(i) : cz(i));
+      if (c == 0) on += (double)fl[i];
+      else if (c == orient) nrm += (double)fl[i];
+    }
+    const double vn = orient * a.wall_u[face][axis];
+    rb = (2.0 * nrm + on - vn) / (1.0 + vn);
+    u0 = a.wall_u[face][0];
+    u1 = a.wall_u[face][1];
+    u2 = a.wall_u[face][2];
+  }
+  if (rho) rho[n] = rb + 1.0;
   if (u) {
-    u[3 * n + 0] = (double)(j0 * invRho + (R(g0) + Fcur[n]) / R(2));
-    u[3 * n + 1] = (double)(j1 * invRho + (R(g1) + Fcur[S + n]) / R(2));
-    u[3 * n + 2] = (double)(j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2));
+    u[3 * n + 0] = u0;
+    u[3 * n + 1] = u1;
+    u[3 * n + 2] = u2;
   }
 }
 
@@ -359,9 +380,10 @@ hipError_t launch_store_populations(hipStream_t s, R* f, const double* in, const
 }
 
 template <typename R>
-hipError_t launch_fluid_moments(hipStream_t s, const R* f, const R* Fcur, const LatticeDev& L, int phase,
-                                const double g[3], double* rho, double* u) {
-  hipLaunchKernelGGL((k_moments<R>), grid_for(L.N), dim3(BLOCK), 0, s, f, Fcur, L, phase, g[0], g[1], g[2], rho, u);
+hipError_t launch_fluid_moments(hipStream_t s, const R* f, const R* Fcur, const uint8_t* type, const LatticeDev& L,
+                                int phase, const CollideArgs& a, double* rho, double* u) {
+  hipLaunchKernelGGL((k_moments<R>), grid_for(L.N), dim3(BLOCK), 0, s, f, Fcur, type, L, phase, a.g[0], a.g[1], a.g[2],
+                     a, rho, u);
   return hipGetLastError();
 }
 
@@ -384,8 +406,8 @@ hipError_t launch_to_double(hipStream_t s, const R* in, double* out, int64_t n) 
                                                  const double*);                                              \
   template hipError_t launch_natural_populations<R>(hipStream_t, const R*, double*, const LatticeDev&, int);    \
   template hipError_t launch_store_populations<R>(hipStream_t, R*, const double*, const LatticeDev&);           \
-  template hipError_t launch_fluid_moments<R>(hipStream_t, const R*, const R*, const LatticeDev&, int,          \
-                                              const double*, double*, double*);                               \
+  template hipError_t launch_fluid_moments<R>(hipStream_t, const R*, const R*, const uint8_t*, const LatticeDev&,  \
+                                              int, const CollideArgs&, double*, double*);                     \
   template hipError_t launch_fill_zero<R>(hipStream_t, R*, int64_t);                                            \
   template hipError_t launch_to_double<R>(hipStream_t, const R*, double*, int64_t);
 HC_INSTANTIATE(float)

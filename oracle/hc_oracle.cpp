@@ -540,10 +540,13 @@ struct Oracle {
     }
   }
 
+  // The support is recomputed from the positions, which are unchanged since the
+  // spread of this step (the reference stores it on the particle instead).
   void interpolate() {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < NV(); ++i) {
-      const Kernel& k = kern[i];
+      Kernel k;
+      make_kernel(&x[3 * i], k);
       double vel[3] = {0, 0, 0};
       for (int j = 0; j < k.n; ++j) {
         double u[3];
@@ -915,17 +918,29 @@ void ho_set_populations(void* h, const double* f) { Oracle* o = (Oracle*)h; std:
 void ho_get_force_field(void* h, double* F) { Oracle* o = (Oracle*)h; std::copy(o->Fibm.begin(), o->Fibm.end(), F); }
 void ho_set_force_field(void* h, const double* F) { Oracle* o = (Oracle*)h; std::copy(F, F + 3 * o->N, o->Fibm.begin()); }
 
-// Density and velocity (u = j/rho + F/2 with the current force field) of every node.
+// Density and velocity (u = j/rho + F/2 with the current force field) of every
+// node; a velocity-boundary node reports its imposed velocity and the density of
+// the mass balance over its known populations (the boundary dynamics' moments).
 void ho_get_fluid(void* h, double* rho, double* u) {
   Oracle* o = (Oracle*)h;
   for (long n = 0; n < o->N; ++n) {
     double fl[19];
     for (int i = 0; i < 19; ++i) fl[i] = o->f[(long)i * o->N + n];
-    double rhoBar, j[3];
+    double rhoBar, j[3], uu[3];
     rhobar_j(fl, rhoBar, j);
-    if (rho) rho[n] = rhoBar + 1.0;
-    double uu[3];
     o->velocity_at(n, uu);
+    if (o->type[n] >= VELWALL0) {
+      const int face = o->type[n] - VELWALL0, axis = face / 2, orient = (face % 2) ? 1 : -1;
+      double on = 0.0, nrm = 0.0;
+      for (int i = 0; i < 19; ++i) {
+        if (C[i][axis] == 0) on += fl[i];
+        else if (C[i][axis] == orient) nrm += fl[i];
+      }
+      const double vn = orient * o->wall_u[face][axis];
+      rhoBar = (2.0 * nrm + on - vn) / (1.0 + vn);
+      for (int c = 0; c < 3; ++c) uu[c] = o->wall_u[face][c];
+    }
+    if (rho) rho[n] = rhoBar + 1.0;
     if (u) for (int c = 0; c < 3; ++c) u[3 * n + c] = uu[c];
   }
 }
