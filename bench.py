@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp32", "fp64"])
     ap.add_argument("--shape", default="256,256,512")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a device-to-device copy of MIB MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -134,6 +136,15 @@ def main():
     import hemocell_amd as hc
     from hemocell_amd import suspension as susp
     shape = tuple(int(s) for s in args.shape.split(","))
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(args.copy_calibration * 2 ** 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: %d MiB read + %d MiB written" % (args.copy_calibration, args.copy_calibration),
+              file=sys.stderr, flush=True)
+        del a, b
     res = {}
     if args.precision in ("both", "fp32"):
         res["fp32"] = run_gpu(hc, susp, shape, hc.HC_FP32_SHIFTED, args.steps, args.warmup)

@@ -33,51 +33,74 @@ __device__ __forceinline__ int64_t wrap_mod(int64_t c, int n) {
   return m < 0 ? m + n : m;
 }
 
-// phi2 support of position p; returns the number of nodes (<= 8).  When `loc`
-// is given it receives each node's unwrapped coordinates.
-__device__ int ibm_support(double p0, double p1, double p2, const LatticeDev& L, const uint8_t* type,
-                           int64_t (&idx)[8], double (&w)[8], int (*loc)[3] = nullptr) {
-  if (!(isfinite(p0) && isfinite(p1) && isfinite(p2))) return 0;
+// phi2 support of position p.  phi2(d) = max(1 - |d|, 0) is non-zero only on
+// the nodes floor(p) and floor(p) + 1 of each axis, so the reference's 3x3x3
+// candidates around the nearest node reduce to this 2x2x2 block, visited in the
+// same x -> y -> z order.  Each of the 8 slots keeps its weight (0 when the
+// node lies outside a non-periodic domain, has zero weight or is a wall node);
+// adding the zero weights to the total changes nothing, so the normalised
+// weights are those of immersedBoundaryMethod.h:62-138.  Fixed slots keep
+// everything in registers.
+struct Support {
+  uint32_t idx[8];
+  double w[8];
+  int lo[3];  // unwrapped coordinates of the slot-0 node
+};
+
+__device__ __forceinline__ int64_t wrap_far(int64_t c, int n) {
+  const double q = floor((double)c / (double)n);
+  return c - (int64_t)q * n;
+}
+
+__device__ __forceinline__ void ibm_support(double p0, double p1, double p2, const LatticeDev& L,
+                                            const uint8_t* __restrict__ type, Support& s) {
   const double p[3] = {p0, p1, p2};
   const int dims[3] = {L.nx, L.ny, L.nz};
-  int64_t c[3];
-  for (int a = 0; a < 3; ++a) c[a] = (int64_t)floor(p[a] + 0.5);
-  int cnt = 0;
+  double q[3][2];
+  int wi[3][2];
+  bool ok[3][2];
+  bool finite = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    finite &= fabs(p[a]) < 1.0e15;
+    const double fl = finite ? floor(p[a]) : 0.0;
+    const int64_t n0 = (int64_t)fl;
+    s.lo[a] = (int)n0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int64_t node = n0 + k;
+      q[a][k] = fmax(1.0 - fabs(p[a] - (double)node), 0.0);
+      if (node >= 0 && node < dims[a]) {
+        wi[a][k] = (int)node;
+        ok[a][k] = true;
+      } else if (L.periodic[a]) {
+        wi[a][k] = (int)(node == -1 ? dims[a] - 1 : (node == dims[a] ? 0 : wrap_far(node, dims[a])));
+        ok[a][k] = true;
+      } else {
+        wi[a][k] = 0;
+        ok[a][k] = false;
+      }
+    }
+  }
   double total = 0.0;
-  for (int dx = -1; dx < 2; ++dx)
-    for (int dy = -1; dy < 2; ++dy)
-      for (int dz = -1; dz < 2; ++dz) {
-        const int64_t nd[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
-        int64_t wc[3];
-        bool out = false;
-        for (int a = 0; a < 3; ++a) {
-          if (nd[a] >= 0 && nd[a] < dims[a]) wc[a] = nd[a];
-          else if (L.periodic[a]) wc[a] = wrap_mod(nd[a], dims[a]);
-          else out = true;
-        }
-        if (out) continue;
-        const double q0 = fmax(1.0 - fabs(p[0] - (double)nd[0]), 0.0);
-        const double q1 = fmax(1.0 - fabs(p[1] - (double)nd[1]), 0.0);
-        const double q2 = fmax(1.0 - fabs(p[2] - (double)nd[2]), 0.0);
-        const double ww = q0 * q1 * q2;
-        if (ww == 0.0) continue;
-        const int64_t n = wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0]);
-        if (type && type[n] != HC_NODE_FLUID) continue;
-        if (cnt < 8) {
-          idx[cnt] = n;
-          w[cnt] = ww;
-          if (loc) {
-            loc[cnt][0] = (int)nd[0];
-            loc[cnt][1] = (int)nd[1];
-            loc[cnt][2] = (int)nd[2];
-          }
-          total += ww;
-          ++cnt;
-        }
+#pragma unroll
+  for (int ix = 0; ix < 2; ++ix)
+#pragma unroll
+    for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+      for (int iz = 0; iz < 2; ++iz) {
+        const int slot = ix * 4 + iy * 2 + iz;
+        const uint32_t n = (uint32_t)wi[2][iz] + (uint32_t)L.nz * ((uint32_t)wi[1][iy] + (uint32_t)L.ny * (uint32_t)wi[0][ix]);
+        const double ww = q[0][ix] * q[1][iy] * q[2][iz];
+        bool keep = finite && ok[0][ix] && ok[1][iy] && ok[2][iz] && ww != 0.0;
+        if (keep && type) keep = type[n] == HC_NODE_FLUID;
+        s.idx[slot] = n;
+        s.w[slot] = keep ? ww : 0.0;
+        total += s.w[slot];
       }
   const double coeff = 1.0 / total;
-  for (int k = 0; k < cnt; ++k) w[k] *= coeff;
-  return cnt;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s.w[k] = s.w[k] != 0.0 ? s.w[k] * coeff : 0.0;
 }
 
 __device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
@@ -89,9 +112,8 @@ __global__ __launch_bounds__(BLOCK) void k_spread(VertexDev V, R* __restrict__ F
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= V.n) return;
   const int64_t vs = V.vs;
-  int64_t idx[8];
-  double w[8];
-  const int cnt = ibm_support(V.x[i], V.x[vs + i], V.x[2 * vs + i], L, type, idx, w);
+  Support sp;
+  ibm_support(V.x[i], V.x[vs + i], V.x[2 * vs + i], L, type, sp);
   double f0 = V.f[i], f1 = V.f[vs + i], f2 = V.f[2 * vs + i];
   const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
   if (fm > f_limit) {
@@ -110,17 +132,21 @@ __global__ __launch_bounds__(BLOCK) void k_spread(VertexDev V, R* __restrict__ F
     t2 = V.frep[2 * vs + i] + f2;
   }
   const int64_t S = L.stride;
-  for (int k = 0; k < cnt; ++k) {
-    atomic_add(&Fibm[idx[k]], (R)(t0 * w[k]));
-    atomic_add(&Fibm[S + idx[k]], (R)(t1 * w[k]));
-    atomic_add(&Fibm[2 * S + idx[k]], (R)(t2 * w[k]));
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (sp.w[k] == 0.0) continue;
+    const int64_t n = sp.idx[k];
+    atomic_add(&Fibm[n], (R)(t0 * sp.w[k]));
+    atomic_add(&Fibm[S + n], (R)(t1 * sp.w[k]));
+    atomic_add(&Fibm[2 * S + n], (R)(t2 * sp.w[k]));
   }
 }
 
 // Fluid velocity u = j/rho + F/2 of every node from its post-stream populations
 // (phase of the AA pattern as in hc_lbm.hip) and the force field the spread
 // filled this step: ExternalForceDynamics::computeVelocity restated.  One
-// coalesced pass; the interpolation then reads 3 values per support node.
+// coalesced pass writing (u0, u1, u2, 0) per node, so the interpolation reads
+// each support node with one vector load.
 template <typename R, bool PHASE1, bool CHECK>
 __global__ __launch_bounds__(BLOCK) void k_velocity_field(const R* __restrict__ f, const R* __restrict__ Fcur,
                                                           R* __restrict__ uf, LatticeDev L, double g0, double g1,
@@ -138,9 +164,11 @@ __global__ __launch_bounds__(BLOCK) void k_velocity_field(const R* __restrict__ 
   R rhoBar, j0, j1, j2;
   rhobar_j(fl, rhoBar, j0, j1, j2);
   const R invRho = R(1) / (rhoBar + R(1));
-  uf[n] = j0 * invRho + (R(g0) + Fcur[n]) / R(2);
-  uf[S + n] = j1 * invRho + (R(g1) + Fcur[S + n]) / R(2);
-  uf[2 * S + n] = j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2);
+  R* q = uf + 4 * (int64_t)n;  // interleaved: one vector load per support node
+  q[0] = j0 * invRho + (R(g0) + Fcur[n]) / R(2);
+  q[1] = j1 * invRho + (R(g1) + Fcur[S + n]) / R(2);
+  q[2] = j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2);
+  q[3] = R(0);
 }
 
 template <typename R>
@@ -150,21 +178,22 @@ __global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* 
                                                           unsigned long long* wall_hits) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= V.n) return;
-  const int64_t vs = V.vs, S = L.stride;
+  const int64_t vs = V.vs;
   double x0 = V.x[i], x1 = V.x[vs + i], x2 = V.x[2 * vs + i];
   double v0, v1, v2;
   if (do_interp) {
-    int64_t idx[8];
-    double w[8];
-    const int cnt = ibm_support(x0, x1, x2, L, type, idx, w);
+    Support sp;
+    ibm_support(x0, x1, x2, L, type, sp);
     v0 = 0.0;
     v1 = 0.0;
     v2 = 0.0;
-    for (int k = 0; k < cnt; ++k) {
-      const int64_t n = idx[k];
-      v0 += (double)uf[n] * w[k];
-      v1 += (double)uf[S + n] * w[k];
-      v2 += (double)uf[2 * S + n] * w[k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (sp.w[k] == 0.0) continue;
+      const R* q = uf + 4 * (int64_t)sp.idx[k];
+      v0 += (double)q[0] * sp.w[k];
+      v1 += (double)q[1] * sp.w[k];
+      v2 += (double)q[2] * sp.w[k];
     }
     V.v[i] = v0;
     V.v[vs + i] = v1;
@@ -261,7 +290,7 @@ __global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict_
         lbad = true;
         continue;
       }
-      const int c = (int)floor(p + 0.5);
+      const int c = (int)floor(p);
       lmin[a] = min(lmin[a], c);
       lmax[a] = max(lmax[a], c);
     }
@@ -272,31 +301,38 @@ __global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict_
   }
   if (lbad) atomicOr(&bad, 1);
   __syncthreads();
-  const int o0 = bmin[0] - 1, o1 = bmin[1] - 1, o2 = bmin[2] - 1;
-  const int s0 = bmax[0] - bmin[0] + 3, s1 = bmax[1] - bmin[1] + 3, s2 = bmax[2] - bmin[2] + 3;
+  // support nodes of a vertex are floor(p) and floor(p) + 1 on each axis
+  const int o0 = bmin[0], o1 = bmin[1], o2 = bmin[2];
+  const int s0 = bmax[0] - bmin[0] + 2, s1 = bmax[1] - bmin[1] + 2, s2 = bmax[2] - bmin[2] + 2;
   const int64_t total = (int64_t)s0 * s1 * s2;
   const bool direct = bad || total > lds_nodes || s0 <= 0;
   if (!direct)
     for (int64_t e = tid; e < 3 * total; e += BS) box[e] = R(0);
   __syncthreads();
-  for (int k = tid; k < nv; k += BS) {
-    const int64_t i = base + k;
-    int64_t idx[8];
-    double w[8];
-    int loc[8][3];
-    const int cnt = ibm_support(V.x[i], V.x[vs + i], V.x[2 * vs + i], L, type, idx, w, loc);
+  // lanes of one wavefront take vertices ~nv/64 apart on the mesh, so they rarely
+  // add to the same LDS node at once (consecutive mesh vertices share support nodes)
+  const int per = (nv + 63) / 64;
+  for (int k = tid; k < 64 * per; k += BS) {
+    const int vtx = (k & 63) * per + (k >> 6);
+    if (vtx >= nv) continue;
+    const int64_t i = base + vtx;
+    Support sp;
+    ibm_support(V.x[i], V.x[vs + i], V.x[2 * vs + i], L, type, sp);
     double t0, t1, t2;
     spread_force(V, i, f_limit, use_rep, t0, t1, t2);
-    for (int q = 0; q < cnt; ++q) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (sp.w[q] == 0.0) continue;
       if (direct) {
-        atomic_add(&Fibm[idx[q]], (R)(t0 * w[q]));
-        atomic_add(&Fibm[S + idx[q]], (R)(t1 * w[q]));
-        atomic_add(&Fibm[2 * S + idx[q]], (R)(t2 * w[q]));
+        const int64_t n = sp.idx[q];
+        atomic_add(&Fibm[n], (R)(t0 * sp.w[q]));
+        atomic_add(&Fibm[S + n], (R)(t1 * sp.w[q]));
+        atomic_add(&Fibm[2 * S + n], (R)(t2 * sp.w[q]));
       } else {
-        const int64_t e = (int64_t)(loc[q][2] - o2) + (int64_t)s2 * ((loc[q][1] - o1) + (int64_t)s1 * (loc[q][0] - o0));
-        lds_add(&box[e], (R)(t0 * w[q]));
-        lds_add(&box[total + e], (R)(t1 * w[q]));
-        lds_add(&box[2 * total + e], (R)(t2 * w[q]));
+        const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * (sp.lo[0] + (q >> 2) - o0));
+        lds_add(&box[e], (R)(t0 * sp.w[q]));
+        lds_add(&box[total + e], (R)(t1 * sp.w[q]));
+        lds_add(&box[2 * total + e], (R)(t2 * sp.w[q]));
       }
     }
   }

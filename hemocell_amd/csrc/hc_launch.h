@@ -57,7 +57,7 @@ hipError_t launch_add_extra_force(hipStream_t s, const VertexDev& V, const int64
 
 // mechanics (hc_mech.hip)
 hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
-                            const int64_t* cell_base, double* bend_scratch);
+                            const int64_t* cell_base, double* bend_scratch, int lds_limit);
 hipError_t launch_cell_volume_area(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells,
                                    int ncells, const int64_t* cell_base, double* vol, double* area);
 hipError_t launch_check_finite(hipStream_t s, const double* p, int64_t n, unsigned int* bad);

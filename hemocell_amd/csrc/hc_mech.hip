@@ -15,6 +15,7 @@
 // terms are recomputed by each incident vertex (cheap, no storage); the
 // per-vertex bending force, needed by the whole 1-ring, is computed once into a
 // scratch array and read back after a barrier.
+#include <algorithm>
 #include <cmath>
 
 #include "hc_launch.h"
@@ -51,10 +52,11 @@ __device__ __forceinline__ double normp(D3 a) {
   return sqrt(r);
 }
 
+// Positions of one cell's vertices: three coordinate arrays, in LDS when the
+// cell is staged there, otherwise in the global vertex planes.
 struct CellView {
-  const double* x;
-  int64_t base, vs;
-  __device__ __forceinline__ D3 pos(int k) const { return {x[base + k], x[vs + base + k], x[2 * vs + base + k]}; }
+  const double *px, *py, *pz;
+  __device__ __forceinline__ D3 pos(int k) const { return {px[k], py[k], pz[k]}; }
 };
 
 // area and unit normal of triangle t (array.h:271-285)
@@ -125,15 +127,51 @@ __device__ D3 bending_force(const CellView& c, const TypeDev& T, int i) {
   return {pn.x * s, pn.y * s, pn.z * s};
 }
 
-template <int MODEL, int BS>
+// STAGED: the cell's positions, bending forces and (with membrane viscosity)
+// velocities live in LDS for the whole kernel (9 x 8 bytes per vertex at most).
+template <int MODEL, int BS, bool STAGED>
 __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
                                                   const int64_t* __restrict__ cell_base, double* __restrict__ bend) {
+  extern __shared__ double sm[];
   __shared__ double red[BS];
   __shared__ double vforce_sh;
   const int cell = cells[blockIdx.x];
-  const CellView c{V.x, cell_base[cell], V.vs};
+  const int64_t base = cell_base[cell];
   const int tid = threadIdx.x;
   const int64_t vs = V.vs;
+  const int nv = T.nv;
+  const bool visc = (MODEL != HC_MODEL_RBC) || (T.eta_m != 0.0);
+  const double *vx, *vy, *vz;
+  double *bx, *by, *bz;
+  CellView c;
+  if (STAGED) {
+    for (int k = tid; k < nv; k += BS) {
+      sm[k] = V.x[base + k];
+      sm[nv + k] = V.x[vs + base + k];
+      sm[2 * nv + k] = V.x[2 * vs + base + k];
+      if (visc) {
+        sm[6 * nv + k] = V.v[base + k];
+        sm[7 * nv + k] = V.v[vs + base + k];
+        sm[8 * nv + k] = V.v[2 * vs + base + k];
+      }
+    }
+    c = CellView{sm, sm + nv, sm + 2 * nv};
+    bx = sm + 3 * nv;
+    by = sm + 4 * nv;
+    bz = sm + 5 * nv;
+    vx = sm + 6 * nv;
+    vy = sm + 7 * nv;
+    vz = sm + 8 * nv;
+    __syncthreads();
+  } else {
+    c = CellView{V.x + base, V.x + vs + base, V.x + 2 * vs + base};
+    bx = bend + base;
+    by = bend + vs + base;
+    bz = bend + 2 * vs + base;
+    vx = V.v + base;
+    vy = V.v + vs + base;
+    vz = V.v + 2 * vs + base;
+  }
 
   // ---- cell volume (sum of signed triple products, x 1/6) --------------------
   double part = 0.0;
@@ -162,15 +200,13 @@ __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const 
   if (MODEL != HC_MODEL_PLT) {
     for (int i = tid; i < T.nv; i += BS) {
       const D3 b = bending_force(c, T, i);
-      const int64_t k = c.base + i;
-      bend[k] = b.x;
-      bend[vs + k] = b.y;
-      bend[2 * vs + k] = b.z;
+      bx[i] = b.x;
+      by[i] = b.y;
+      bz[i] = b.z;
     }
   }
   __syncthreads();
   const double vforce = vforce_sh;
-  const bool visc = (MODEL != HC_MODEL_RBC) || (T.eta_m != 0.0);
 
   // ---- gather every term of vertex k in the reference's order ----------------
   for (int k = tid; k < T.nv; k += BS) {
@@ -204,8 +240,7 @@ __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const 
     if (MODEL != HC_MODEL_PLT) {
       for (int e = T.vbend_off[k]; e < T.vbend_off[k + 1]; ++e) {
         const int i = T.vbend[e];
-        const int64_t q = c.base + i;
-        const double b0 = bend[q], b1 = bend[vs + q], b2 = bend[2 * vs + q];
+        const double b0 = bx[i], b1 = by[i], b2 = bz[i];
         if (i == k) {
           F0 += b0;
           F1 += b1;
@@ -234,8 +269,7 @@ __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const 
         if (sgn > 0) { F0 += fl.x; F1 += fl.y; F2 += fl.z; }
         else { F0 -= fl.x; F1 -= fl.y; F2 -= fl.z; }
         if (visc) {
-          const int64_t ia = c.base + a, ib = c.base + b;
-          const D3 rv = {V.v[ib] - V.v[ia], V.v[vs + ib] - V.v[vs + ia], V.v[2 * vs + ib] - V.v[2 * vs + ia]};
+          const D3 rv = {vx[b] - vx[a], vy[b] - vy[a], vz[b] - vz[a]};
           const double pr = dotp(rv, uv);
           D3 fv = {(pr * uv.x) * T.eta_m, (pr * uv.y) * T.eta_m, (pr * uv.z) * T.eta_m};
           const double m = normp(fv);
@@ -290,7 +324,7 @@ __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const 
         }
       }
     }
-    const int64_t q = c.base + k;
+    const int64_t q = base + k;
     V.f[q] = F0;
     V.f[vs + q] = F1;
     V.f[2 * vs + q] = F2;
@@ -302,7 +336,8 @@ __global__ __launch_bounds__(BS) void k_volume_area(VertexDev V, TypeDev T, cons
                                                     const int64_t* __restrict__ cell_base, double* vol, double* area) {
   __shared__ double rv[BS], ra[BS];
   const int cell = cells[blockIdx.x];
-  const CellView c{V.x, cell_base[cell], V.vs};
+  const int64_t base = cell_base[cell];
+  const CellView c{V.x + base, V.x + V.vs + base, V.x + 2 * V.vs + base};
   const int tid = threadIdx.x;
   double pv = 0.0, pa = 0.0;
   for (int t = tid; t < T.nt; t += BS) {
@@ -346,22 +381,30 @@ __global__ void k_check_finite_f(const float* p, int64_t n, unsigned int* bad) {
 }  // namespace
 
 hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
-                            const int64_t* cell_base, double* bend) {
+                            const int64_t* cell_base, double* bend, int lds_limit) {
   if (ncells <= 0) return hipSuccess;
   dim3 g((unsigned)ncells);
+  const bool visc = (T.model != HC_MODEL_RBC) || (T.eta_m != 0.0);
+  const size_t lds = sizeof(double) * (size_t)T.nv * (visc ? 9 : 6);
+  const bool staged = lds <= (size_t)std::min(lds_limit, 65536);
+  const size_t dyn = staged ? lds : 0;
+#define HC_MECH(M, B)                                                                                           \
+  if (staged) hipLaunchKernelGGL((k_mechanics<M, B, true>), g, dim3(B), dyn, s, V, T, cells, cell_base, bend);   \
+  else hipLaunchKernelGGL((k_mechanics<M, B, false>), g, dim3(B), 0, s, V, T, cells, cell_base, bend);
   switch (T.model) {
     case HC_MODEL_RBC:
-      hipLaunchKernelGGL((k_mechanics<HC_MODEL_RBC, 256>), g, dim3(256), 0, s, V, T, cells, cell_base, bend);
+      HC_MECH(HC_MODEL_RBC, 256)
       break;
     case HC_MODEL_PLT:
-      hipLaunchKernelGGL((k_mechanics<HC_MODEL_PLT, 64>), g, dim3(64), 0, s, V, T, cells, cell_base, bend);
+      HC_MECH(HC_MODEL_PLT, 64)
       break;
     case HC_MODEL_WBC:
-      hipLaunchKernelGGL((k_mechanics<HC_MODEL_WBC, 256>), g, dim3(256), 0, s, V, T, cells, cell_base, bend);
+      HC_MECH(HC_MODEL_WBC, 256)
       break;
     default:
       return hipErrorInvalidValue;
   }
+#undef HC_MECH
   return hipGetLastError();
 }
 
