@@ -73,10 +73,17 @@ struct hc_ctx {
   bool cells_dirty = true;
   int lds_limit = 65536;        // dynamic LDS per workgroup available on the device
   bool spread_per_cell = true;  // LDS-staged spread (false: one thread per vertex, global atomics)
+  int spread_lds = 0;           // LDS box budget in bytes (0: default per precision)
   // extra (stretch) forces
   int64_t* d_extra_ids = nullptr;
   double* d_extra_f = nullptr;
   int n_extra = 0;
+  double* d_extra_dense = nullptr;  // the same forces per vertex (3 planes), for the fused step
+  std::vector<int64_t> h_extra_ids;
+  std::vector<double> h_extra_f;
+  // fused step: the vertex kernels of step t also spread the forces of step t+1
+  bool fuse = true;
+  bool spread_ready = false;  // Fibm[cur] already holds this step's spread
   // controls
   int64_t iter = 0;
   int pv_timescale = 1;
@@ -237,6 +244,54 @@ hc_status upload(hc_ctx* ctx, TypeBufs& tb, const std::vector<T>& h, const T** o
   return HC_OK;
 }
 
+// Drop a spread done ahead by the fused step (the host changed what it used).
+hc_status invalidate_spread(hc_ctx* ctx) {
+  if (!ctx->spread_ready) return HC_OK;
+  hipError_t e = ctx->precision == HC_FP32_SHIFTED
+                     ? launch_fill_zero<float>(ctx->stream, (float*)ctx->Fibm[ctx->cur], 3 * ctx->L.stride)
+                     : launch_fill_zero<double>(ctx->stream, (double*)ctx->Fibm[ctx->cur], 3 * ctx->L.stride);
+  if (e != hipSuccess) return hipfail(ctx, e, "invalidate spread");
+  ctx->spread_ready = false;
+  return HC_OK;
+}
+
+// Per-vertex planes of the extra forces, read by the fused step.
+hc_status build_dense_extra(hc_ctx* ctx) {
+  if (ctx->d_extra_dense) hipFree(ctx->d_extra_dense);
+  ctx->d_extra_dense = nullptr;
+  if (ctx->h_extra_ids.empty()) return HC_OK;
+  hc_status s = dalloc(ctx, (void**)&ctx->d_extra_dense, sizeof(double) * 3 * ctx->vcap, "extra forces");
+  if (s != HC_OK) return s;
+  std::vector<double> h(3 * ctx->vcap, 0.0);
+  for (size_t k = 0; k < ctx->h_extra_ids.size(); ++k)
+    for (int c = 0; c < 3; ++c) h[c * ctx->vcap + ctx->h_extra_ids[k]] += ctx->h_extra_f[3 * k + c];
+  HC_TRY(hipMemcpy(ctx->d_extra_dense, h.data(), sizeof(double) * 3 * ctx->vcap, hipMemcpyHostToDevice), "hipMemcpy");
+  return HC_OK;
+}
+
+template <typename R>
+hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_MECH, &ev);
+  for (auto& tp : ctx->types) {
+    TypeBufs& tb = *tp;
+    if (tb.ncells == 0) continue;
+    int flags = CELL_FLAG_SPREAD;
+    if (do_interp) flags |= CELL_FLAG_INTERP;
+    if ((ctx->iter % tb.host.timescale) == 0) flags |= CELL_FLAG_MECH;
+    if (ctx->repulsion) flags |= CELL_FLAG_REPULSION;
+    int box = ctx->spread_lds > 0 ? ctx->spread_lds : (sizeof(R) == 4 ? 32768 : 49152);
+    if (tb.host.nv > 1000) box = std::max(box, 98304);
+    HC_TRY(launch_cell_step<R>(ctx->stream, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base,
+                               (const R*)ctx->ufield, ctx->type, ctx->L, (R*)ctx->Fibm[1 - ctx->cur],
+                               ctx->d_extra_dense, ctx->bend, ctx->f_limit, flags, box, ctx->lds_limit - 4096,
+                               ctx->d_wall_hits),
+           "cell_step");
+  }
+  prof_end(ctx, HC_PROF_MECH, ev);
+  return HC_OK;
+}
+
 template <typename R>
 hc_status do_spread(hc_ctx* ctx) {
   hipEvent_t ev;
@@ -244,7 +299,7 @@ hc_status do_spread(hc_ctx* ctx) {
   if (ctx->n_extra > 0) HC_TRY(launch_add_extra_force(ctx->stream, vdev(ctx), ctx->d_extra_ids, ctx->d_extra_f, ctx->n_extra), "extra force");
   if (ctx->spread_per_cell) {
     // LDS budget: room for the support box of a tumbling RBC in FP32, an aligned one in FP64
-    const int want = sizeof(R) == 4 ? 65536 : 98304;
+    const int want = ctx->spread_lds > 0 ? ctx->spread_lds : (sizeof(R) == 4 ? 40960 : 65536);
     HC_TRY(launch_spread_cells<R>(ctx->stream, vdev(ctx), (R*)ctx->Fibm[ctx->cur], ctx->type, ctx->L, ctx->d_cell_base,
                                   ctx->d_cell_nv, (int)ctx->cell_base.size(), ctx->f_limit, ctx->repulsion ? 1 : 0,
                                   std::min(want, ctx->lds_limit)),
@@ -323,15 +378,30 @@ hc_status step_impl(hc_ctx* ctx, int nsteps) {
   for (int k = 0; k < nsteps; ++k) {
     hipEvent_t ev;
     prof_begin(ctx, HC_PROF_STEP, &ev);
+    const bool fused = ctx->fuse && !ctx->repulsion && ctx->nv > 0;
     if (ctx->repulsion && ctx->iter % ctx->rep_timescale == 0) {
       s = hc_stage_repulsion(ctx);
       if (s != HC_OK) return s;
     }
-    if (ctx->nv > 0 && (s = do_spread<R>(ctx)) != HC_OK) return s;
+    if (ctx->nv > 0 && !ctx->spread_ready && (s = do_spread<R>(ctx)) != HC_OK) return s;
     if ((s = do_collide<R>(ctx)) != HC_OK) return s;
-    if ((s = do_interp_advance<R>(ctx, ctx->iter % ctx->pv_timescale == 0, 1)) != HC_OK) return s;
-    if ((s = do_mechanics(ctx, false)) != HC_OK) return s;
+    const int do_interp = ctx->iter % ctx->pv_timescale == 0;
+    if (fused) {
+      if (do_interp) {
+        hipEvent_t e2;
+        prof_begin(ctx, HC_PROF_INTERP, &e2);
+        HC_TRY(launch_velocity_field<R>(ctx->stream, (const R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], (R*)ctx->ufield,
+                                        ctx->L, ctx->phase, ctx->general, ctx->g),
+               "velocity_field");
+        prof_end(ctx, HC_PROF_INTERP, e2);
+      }
+      if ((s = do_cell_step<R>(ctx, do_interp)) != HC_OK) return s;
+    } else {
+      if ((s = do_interp_advance<R>(ctx, do_interp, 1)) != HC_OK) return s;
+      if ((s = do_mechanics(ctx, false)) != HC_OK) return s;
+    }
     if ((s = do_end<R>(ctx)) != HC_OK) return s;
+    ctx->spread_ready = fused;
     prof_end(ctx, HC_PROF_STEP, ev);
     if (ctx->prof.on && ctx->prof.used > 4096) prof_flush(ctx);
   }
@@ -375,6 +445,10 @@ hc_status hc_create(int device, hc_ctx** out) {
     ctx->lds_limit = lds - 256;  // the kernel's own static LDS
   const char* sp = std::getenv("HC_SPREAD");
   if (sp && std::string(sp) == "vertex") ctx->spread_per_cell = false;
+  const char* fu = std::getenv("HC_FUSE");  // tuning: 0 runs the unfused vertex kernels
+  if (fu && std::string(fu) == "0") ctx->fuse = false;
+  const char* sl = std::getenv("HC_SPREAD_LDS");  // tuning: LDS bytes of the per-cell box
+  if (sl) ctx->spread_lds = std::atoi(sl);
   *out = ctx;
   return HC_OK;
 }
@@ -383,7 +457,7 @@ void hc_destroy(hc_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
+  void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->d_extra_dense, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
                 ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag};
   for (void* p : ps)
     if (p) hipFree(p);
@@ -533,6 +607,13 @@ hc_status hc_add_cell_type(hc_ctx* ctx, int model, int nv, int nt, const int32_t
   UP(vbend_off, vbend_off) UP(vbend, vbend) UP(vedge_off, vedge_off) UP(vedge, vedge) UP(vinner_off, vinner_off)
   UP(vinner, vinner)
 #undef UP
+  {
+    const int32_t* p4 = nullptr;
+    if ((s = upload(ctx, *tb, T.vtri4, &p4)) != HC_OK) return s;
+    D.vtri4 = (const int4*)p4;
+    if ((s = upload(ctx, *tb, T.vedge4, &p4)) != HC_OK) return s;
+    D.vedge4 = (const int4*)p4;
+  }
   D.model = model;
   D.nv = T.nv;
   D.nt = T.nt;
@@ -562,8 +643,11 @@ hc_status hc_add_cells(hc_ctx* ctx, int type_id, int ncells, const double* xyz, 
   if (ncells == 0) return HC_OK;
   const int nvt = ctx->types[type_id]->host.nv;
   const int64_t add = (int64_t)ncells * nvt;
-  hc_status s = grow_vertices(ctx, ctx->nv + add);
-  if (s != HC_OK) return s;
+  hc_status s;
+  if (ctx->lattice_ready && (s = invalidate_spread(ctx)) != HC_OK) return s;
+  const int64_t old_cap = ctx->vcap;
+  if ((s = grow_vertices(ctx, ctx->nv + add)) != HC_OK) return s;
+  if (ctx->vcap != old_cap && (s = build_dense_extra(ctx)) != HC_OK) return s;
   std::vector<double> plane(add);
   for (int c = 0; c < 3; ++c) {
     for (int64_t k = 0; k < add; ++k) plane[k] = xyz[3 * k + c];
@@ -593,6 +677,10 @@ hc_status hc_set_material_timescale(hc_ctx* ctx, int type_id, int ts) {
 }
 hc_status hc_set_force_limit(hc_ctx* ctx, double f_limit) {
   if (!ctx || !(f_limit > 0)) return fail(ctx, HC_INVALID_ARG, "force limit must be > 0");
+  if (ctx->lattice_ready) {
+    hc_status s = invalidate_spread(ctx);
+    if (s != HC_OK) return s;
+  }
   ctx->f_limit = f_limit;
   return HC_OK;
 }
@@ -606,12 +694,18 @@ hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int time
 
 hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* ids, const double* forces) {
   if (!ctx || n < 0 || (n > 0 && (!ids || !forces))) return fail(ctx, HC_INVALID_ARG, "bad extra-force arguments");
+  if (ctx->lattice_ready) {
+    hc_status s0 = invalidate_spread(ctx);
+    if (s0 != HC_OK) return s0;
+  }
   if (ctx->d_extra_ids) hipFree(ctx->d_extra_ids);
   if (ctx->d_extra_f) hipFree(ctx->d_extra_f);
   ctx->d_extra_ids = nullptr;
   ctx->d_extra_f = nullptr;
   ctx->n_extra = 0;
-  if (n == 0) return HC_OK;
+  ctx->h_extra_ids.clear();
+  ctx->h_extra_f.clear();
+  if (n == 0) return build_dense_extra(ctx);
   for (int k = 0; k < n; ++k)
     if (ids[k] < 0 || ids[k] >= ctx->nv) return fail(ctx, HC_INVALID_ARG, "extra-force vertex id out of range");
   hc_status s;
@@ -620,7 +714,9 @@ hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* ids, cons
   HC_TRY(hipMemcpy(ctx->d_extra_ids, ids, sizeof(int64_t) * n, hipMemcpyHostToDevice), "hipMemcpy");
   HC_TRY(hipMemcpy(ctx->d_extra_f, forces, sizeof(double) * 3 * n, hipMemcpyHostToDevice), "hipMemcpy");
   ctx->n_extra = n;
-  return HC_OK;
+  ctx->h_extra_ids.assign(ids, ids + n);
+  ctx->h_extra_f.assign(forces, forces + 3 * n);
+  return build_dense_extra(ctx);
 }
 
 hc_status hc_step(hc_ctx* ctx, int nsteps) {
@@ -641,27 +737,32 @@ int64_t hc_iteration(const hc_ctx* ctx) { return ctx ? ctx->iter : -1; }
 hc_status hc_stage_spread(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
   if (ctx->nv == 0) return HC_OK;
   return fp32(ctx) ? do_spread<float>(ctx) : do_spread<double>(ctx);
 }
 hc_status hc_stage_collide_stream(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
   return fp32(ctx) ? do_collide<float>(ctx) : do_collide<double>(ctx);
 }
 hc_status hc_stage_interpolate(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
   return fp32(ctx) ? do_interp_advance<float>(ctx, 1, 0) : do_interp_advance<double>(ctx, 1, 0);
 }
 hc_status hc_stage_advance(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
   return fp32(ctx) ? do_interp_advance<float>(ctx, 0, 1) : do_interp_advance<double>(ctx, 0, 1);
 }
 hc_status hc_stage_mechanics(hc_ctx* ctx, int forced) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
   return do_mechanics(ctx, forced != 0);
 }
 hc_status hc_stage_repulsion(hc_ctx* ctx) {
@@ -671,6 +772,7 @@ hc_status hc_stage_repulsion(hc_ctx* ctx) {
 hc_status hc_stage_end(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
   return fp32(ctx) ? do_end<float>(ctx) : do_end<double>(ctx);
 }
 
@@ -717,8 +819,11 @@ hc_status hc_get_force_field(hc_ctx* ctx, double* F) {
   const int64_t S = ctx->L.stride, N = ctx->L.N;
   double* d = nullptr;
   if ((s = dalloc(ctx, (void**)&d, sizeof(double) * 3 * S, "force readback")) != HC_OK) return s;
-  hipError_t e = fp32(ctx) ? launch_to_double<float>(ctx->stream, (const float*)ctx->Fibm[ctx->cur], d, 3 * S)
-                           : launch_to_double<double>(ctx->stream, (const double*)ctx->Fibm[ctx->cur], d, 3 * S);
+  // after a fused step the buffer already holds the next step's spread; the
+  // reference's field at this point is the reset one
+  hipError_t e = ctx->spread_ready ? hipMemsetAsync(d, 0, sizeof(double) * 3 * S, ctx->stream)
+                 : fp32(ctx) ? launch_to_double<float>(ctx->stream, (const float*)ctx->Fibm[ctx->cur], d, 3 * S)
+                             : launch_to_double<double>(ctx->stream, (const double*)ctx->Fibm[ctx->cur], d, 3 * S);
   for (int c = 0; c < 3 && e == hipSuccess; ++c)
     e = hipMemcpyAsync(F + c * N, d + c * S, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -741,12 +846,11 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u) {
   a.omega = 1.0 / ctx->tau;
   for (int c = 0; c < 3; ++c) a.g[c] = ctx->g[c];
   std::memcpy(a.wall_u, ctx->wall_u, sizeof(a.wall_u));
-  hipError_t e = fp32(ctx) ? launch_fluid_moments<float>(ctx->stream, (const float*)ctx->f,
-                                                         (const float*)ctx->Fibm[ctx->cur], ctx->type, ctx->L,
-                                                         ctx->phase, a, dr, du)
-                           : launch_fluid_moments<double>(ctx->stream, (const double*)ctx->f,
-                                                          (const double*)ctx->Fibm[ctx->cur], ctx->type, ctx->L,
-                                                          ctx->phase, a, dr, du);
+  const void* Fc = ctx->spread_ready ? nullptr : ctx->Fibm[ctx->cur];  // reset field after a fused step
+  hipError_t e = fp32(ctx) ? launch_fluid_moments<float>(ctx->stream, (const float*)ctx->f, (const float*)Fc,
+                                                         ctx->type, ctx->L, ctx->phase, a, dr, du)
+                           : launch_fluid_moments<double>(ctx->stream, (const double*)ctx->f, (const double*)Fc,
+                                                          ctx->type, ctx->L, ctx->phase, a, dr, du);
   if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, dr, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess && u) e = hipMemcpyAsync(u, du, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -786,8 +890,9 @@ hc_status hc_get_vertices(hc_ctx* ctx, double* x, double* v, double* f) {
 }
 hc_status hc_set_vertices(hc_ctx* ctx, const double* x, const double* v, const double* f) {
   if (!ctx) return HC_INVALID_ARG;
-  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
   hc_status s;
+  if (ctx->lattice_ready && (s = invalidate_spread(ctx)) != HC_OK) return s;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
   if (x && (s = set_planes(ctx, ctx->x, x)) != HC_OK) return s;
   if (v && (s = set_planes(ctx, ctx->v, v)) != HC_OK) return s;
   if (f && (s = set_planes(ctx, ctx->fv, f)) != HC_OK) return s;

@@ -1,0 +1,331 @@
+// hc_cell.hip -- the vertex half of the IB-LBM step, fused per cell.
+//
+// One workgroup per cell runs, in the order of HemoCell::iterate()
+// (core/hemoCell.cpp:299-376):
+//   1. interpolateFluidVelocity + advanceParticles of step t
+//      (core/hemoCellParticleField.cpp:566-588, :819-839): the cell's vertices
+//      read the fluid velocity of their phi2 support, move, and land in LDS;
+//   2. applyConstitutiveModel of step t (:633-675 and the model's
+//      ParticleMechanics) on the positions in LDS;
+//   3. spreadParticleForce of step t+1 (:841-863): the forces just computed (plus
+//      any extra force, capped to f_limit in place) are spread from the same
+//      positions into the force buffer of the next step, through an LDS box
+//      covering the cell's support and one global atomic per non-zero node.
+// Step t+1's spread needs exactly x_t and F(x_t), which steps 1-2 produce, so
+// doing it here saves reloading the cell.  Positions stay in LDS for the whole
+// kernel; per-vertex bending forces and velocities too when they fit.
+#include "hc_ibm_dev.h"
+#include "hc_mech_dev.h"
+
+namespace hcg {
+namespace {
+
+enum { CELL_INTERP = 1, CELL_MECH = 2, CELL_SPREAD = 4, CELL_REPULSION = 8 };
+
+template <typename R, int MODEL, int BS, bool STAGED>
+__global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
+                                                  const int64_t* __restrict__ cell_base, const R* __restrict__ uf,
+                                                  const uint8_t* __restrict__ type, LatticeDev L, R* __restrict__ Fnext,
+                                                  const double* __restrict__ extra, double* __restrict__ scratch,
+                                                  double f_limit, int flags, int box_nodes,
+                                                  unsigned long long* wall_hits) {
+  extern __shared__ __align__(16) double sm[];
+  __shared__ double red[BS];
+  __shared__ double vforce_sh;
+  __shared__ int bmin[3], bmax[3], bad;
+  const int cell = cells[blockIdx.x];
+  const int64_t base = cell_base[cell];
+  const int tid = threadIdx.x;
+  const int nv = T.nv;
+  const int64_t vs = V.vs;
+  const bool visc = (MODEL != HC_MODEL_RBC) || (T.eta_m != 0.0);
+  // LDS: positions, then (STAGED) velocities, then a region shared by the bending
+  // forces (phase 2) and the spread box (phase 3)
+  double* px = sm;
+  double* py = sm + nv;
+  double* pz = sm + 2 * nv;
+  double *vx, *vy, *vz, *bx, *by, *bz;
+  double* region;
+  if (STAGED) {
+    vx = sm + 3 * nv;
+    vy = sm + 4 * nv;
+    vz = sm + 5 * nv;
+    region = sm + (visc ? 6 : 3) * nv;
+    bx = region;
+    by = region + nv;
+    bz = region + 2 * nv;
+  } else {
+    vx = V.v + base;
+    vy = V.v + vs + base;
+    vz = V.v + 2 * vs + base;
+    region = sm + 3 * nv;
+    bx = scratch + base;
+    by = scratch + vs + base;
+    bz = scratch + 2 * vs + base;
+  }
+  R* box = (R*)region;
+
+  // ---- 1. interpolation and advance ------------------------------------------
+  for (int k = tid; k < nv; k += BS) {
+    const int64_t i = base + k;
+    double x0 = V.x[i], x1 = V.x[vs + i], x2 = V.x[2 * vs + i];
+    double v0, v1, v2;
+    if (flags & CELL_INTERP) {
+      Support sp;
+      ibm_support(x0, x1, x2, L, type, sp);
+      v0 = 0.0;
+      v1 = 0.0;
+      v2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (sp.w[q] == 0.0) continue;
+        const R* u = uf + 4 * (int64_t)sp.idx[q];
+        v0 += (double)u[0] * sp.w[q];
+        v1 += (double)u[1] * sp.w[q];
+        v2 += (double)u[2] * sp.w[q];
+      }
+      V.v[i] = v0;
+      V.v[vs + i] = v1;
+      V.v[2 * vs + i] = v2;
+    } else {
+      v0 = V.v[i];
+      v1 = V.v[vs + i];
+      v2 = V.v[2 * vs + i];
+    }
+    x0 += v0;
+    x1 += v1;
+    x2 += v2;
+    V.x[i] = x0;
+    V.x[vs + i] = x1;
+    V.x[2 * vs + i] = x2;
+    px[k] = x0;
+    py[k] = x1;
+    pz[k] = x2;
+    if (STAGED && visc) {
+      vx[k] = v0;
+      vy[k] = v1;
+      vz[k] = v2;
+    }
+    if (type && isfinite(x0) && isfinite(x1) && isfinite(x2)) {
+      const double p[3] = {x0, x1, x2};
+      const int dims[3] = {L.nx, L.ny, L.nz};
+      int64_t wc[3];
+      bool out = false;
+      for (int a = 0; a < 3; ++a) {
+        const int64_t cc = (int64_t)floor(p[a] + 0.5);
+        if (cc >= 0 && cc < dims[a]) wc[a] = cc;
+        else if (L.periodic[a]) wc[a] = wrap_mod(cc, dims[a]);
+        else out = true;
+      }
+      if (!out && type[wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0])] != HC_NODE_FLUID)
+        atomicAdd(wall_hits, 1ull);
+    }
+  }
+  __syncthreads();
+  const CellView c{px, py, pz};
+
+  // ---- 2. mechanics ------------------------------------------------------------
+  if (flags & CELL_MECH) {
+    red[tid] = volume_partial<BS>(c, T, tid);
+    __syncthreads();
+    for (int h = BS / 2; h > 0; h >>= 1) {
+      if (tid < h) red[tid] += red[tid + h];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double volume = red[0] * (1.0 / 6.0);
+      const double vf = (volume - T.volume_eq) / T.volume_eq;
+      vforce_sh = -T.k_volume * vf / fabs(MaxCellVolumetricChange - vf * vf);
+    }
+    if (MODEL != HC_MODEL_PLT) {
+      for (int k = tid; k < nv; k += BS) {
+        const D3 b = bending_force(c, T, k);
+        bx[k] = b.x;
+        by[k] = b.y;
+        bz[k] = b.z;
+      }
+    }
+    __syncthreads();
+    const double vforce = vforce_sh;
+    for (int k = tid; k < nv; k += BS) {
+      const D3 F = vertex_force<MODEL>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
+      const int64_t i = base + k;
+      V.f[i] = F.x;
+      V.f[vs + i] = F.y;
+      V.f[2 * vs + i] = F.z;
+    }
+    __syncthreads();  // forces visible to the whole workgroup; the region is free again
+  }
+  if (!(flags & CELL_SPREAD)) return;
+
+  // ---- 3. spread of the next step ---------------------------------------------
+  if (tid == 0) {
+    for (int a = 0; a < 3; ++a) {
+      bmin[a] = 0x7fffffff;
+      bmax[a] = -0x7fffffff;
+    }
+    bad = 0;
+  }
+  __syncthreads();
+  {
+    int lmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, lmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
+    bool lbad = false;
+    for (int k = tid; k < nv; k += BS) {
+      const double p[3] = {px[k], py[k], pz[k]};
+      for (int a = 0; a < 3; ++a) {
+        if (!(fabs(p[a]) < 1.0e9)) {
+          lbad = true;
+          continue;
+        }
+        const int cc = (int)floor(p[a]);
+        lmin[a] = min(lmin[a], cc);
+        lmax[a] = max(lmax[a], cc);
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&bmin[a], lmin[a]);
+      atomicMax(&bmax[a], lmax[a]);
+    }
+    if (lbad) atomicOr(&bad, 1);
+  }
+  __syncthreads();
+  const int o0 = bmin[0], o1 = bmin[1], o2 = bmin[2];
+  const int s0 = bmax[0] - bmin[0] + 2, s1 = bmax[1] - bmin[1] + 2, s2 = bmax[2] - bmin[2] + 2;
+  const int64_t total = (int64_t)s0 * s1 * s2;
+  const bool direct = bad || total > box_nodes || s0 <= 0;
+  if (!direct)
+    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = R(0);
+  __syncthreads();
+  const int64_t S = L.stride;
+  const int per = (nv + 63) / 64;
+  for (int k = tid; k < 64 * per; k += BS) {
+    // lanes of one wavefront take vertices ~nv/64 apart (fewer LDS same-address adds)
+    const int vtx = (k & 63) * per + (k >> 6);
+    if (vtx >= nv) continue;
+    const int64_t i = base + vtx;
+    double f0 = V.f[i], f1 = V.f[vs + i], f2 = V.f[2 * vs + i];
+    bool store = false;
+    if (extra) {
+      f0 += extra[i];
+      f1 += extra[vs + i];
+      f2 += extra[2 * vs + i];
+      store = true;
+    }
+    const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+    if (fm > f_limit) {
+      const double sc = f_limit / fm;
+      f0 *= sc;
+      f1 *= sc;
+      f2 *= sc;
+      store = true;
+    }
+    if (store) {
+      V.f[i] = f0;
+      V.f[vs + i] = f1;
+      V.f[2 * vs + i] = f2;
+    }
+    if (flags & CELL_REPULSION) {
+      f0 = V.frep[i] + f0;
+      f1 = V.frep[vs + i] + f1;
+      f2 = V.frep[2 * vs + i] + f2;
+    }
+    Support sp;
+    ibm_support(px[vtx], py[vtx], pz[vtx], L, type, sp);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (sp.w[q] == 0.0) continue;
+      if (direct) {
+        const int64_t n = sp.idx[q];
+        atomic_add(&Fnext[n], (R)(f0 * sp.w[q]));
+        atomic_add(&Fnext[S + n], (R)(f1 * sp.w[q]));
+        atomic_add(&Fnext[2 * S + n], (R)(f2 * sp.w[q]));
+      } else {
+        const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * (sp.lo[0] + (q >> 2) - o0));
+        lds_add(&box[e], (R)(f0 * sp.w[q]));
+        lds_add(&box[total + e], (R)(f1 * sp.w[q]));
+        lds_add(&box[2 * total + e], (R)(f2 * sp.w[q]));
+      }
+    }
+  }
+  if (direct) return;
+  __syncthreads();
+  const int dims[3] = {L.nx, L.ny, L.nz};
+  for (int64_t e = tid; e < total; e += BS) {
+    const R a = box[e], b = box[total + e], cz = box[2 * total + e];
+    if (a == R(0) && b == R(0) && cz == R(0)) continue;
+    const int lz = (int)(e % s2);
+    const int64_t r = e / s2;
+    const int ly = (int)(r % s1), lx = (int)(r / s1);
+    const int64_t g[3] = {(int64_t)o0 + lx, (int64_t)o1 + ly, (int64_t)o2 + lz};
+    int64_t wc[3];
+    for (int q = 0; q < 3; ++q) wc[q] = (g[q] >= 0 && g[q] < dims[q]) ? g[q] : wrap_mod(g[q], dims[q]);
+    const int64_t n = wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0]);
+    if (a != R(0)) atomic_add(&Fnext[n], a);
+    if (b != R(0)) atomic_add(&Fnext[S + n], b);
+    if (cz != R(0)) atomic_add(&Fnext[2 * S + n], cz);
+  }
+}
+
+}  // namespace
+
+// Fixed LDS of the kernel (bytes) for a type: positions, plus velocities and bending
+// forces when staged.
+static size_t cell_lds_fixed(const TypeDev& T, bool staged) {
+  const bool visc = (T.model != HC_MODEL_RBC) || (T.eta_m != 0.0);
+  const size_t nv = (size_t)T.nv;
+  if (!staged) return 3 * nv * sizeof(double);
+  return ((visc ? 6 : 3) * nv) * sizeof(double);
+}
+
+template <typename R>
+hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
+                            const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L,
+                            R* Fnext, const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
+                            int lds_limit, unsigned long long* wall_hits) {
+  if (ncells <= 0) return hipSuccess;
+  const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * sizeof(double) : 0;
+  // staged: positions, velocities and a region shared by bending forces and the box
+  bool staged = true;
+  size_t fixed = cell_lds_fixed(T, true);
+  size_t region = std::max(bend, (size_t)box_bytes);
+  if (fixed + region > (size_t)lds_limit) {
+    staged = false;
+    fixed = cell_lds_fixed(T, false);
+    region = std::min((size_t)box_bytes, (size_t)lds_limit > fixed ? (size_t)lds_limit - fixed : 0);
+  }
+  if (fixed > (size_t)lds_limit) return hipErrorInvalidConfiguration;
+  const int box_nodes = (int)(region / (3 * sizeof(R)));
+  const size_t dyn = fixed + region;
+  dim3 g((unsigned)ncells);
+#define HC_CELL(M, B)                                                                                        \
+  if (staged)                                                                                                \
+    hipLaunchKernelGGL((k_cell_step<R, M, B, true>), g, dim3(B), dyn, s, V, T, cells, cell_base, uf, type, L,  \
+                       Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);                       \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_cell_step<R, M, B, false>), g, dim3(B), dyn, s, V, T, cells, cell_base, uf, type, L, \
+                       Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
+  switch (T.model) {
+    case HC_MODEL_RBC:
+      HC_CELL(HC_MODEL_RBC, 256)
+      break;
+    case HC_MODEL_PLT:
+      HC_CELL(HC_MODEL_PLT, 64)
+      break;
+    case HC_MODEL_WBC:
+      HC_CELL(HC_MODEL_WBC, 256)
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef HC_CELL
+  return hipGetLastError();
+}
+
+template hipError_t launch_cell_step<float>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
+                                            const int64_t*, const float*, const uint8_t*, const LatticeDev&, float*,
+                                            const double*, double*, double, int, int, int, unsigned long long*);
+template hipError_t launch_cell_step<double>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
+                                             const int64_t*, const double*, const uint8_t*, const LatticeDev&, double*,
+                                             const double*, double*, double, int, int, int, unsigned long long*);
+
+}  // namespace hcg

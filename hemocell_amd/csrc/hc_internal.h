@@ -4,6 +4,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/hemocell_gpu.h"
 
 namespace hcg {
@@ -30,6 +32,8 @@ struct TypeHost {
   std::vector<int32_t> vbend_off, vbend;    // contributing vertex i (own term when i == k)
   std::vector<int32_t> vedge_off, vedge;    // e*4 + kind (0 first end, 1 second end, 2 opposite vertex)
   std::vector<int32_t> vinner_off, vinner;  // ie*2 + end (0 first, 1 second)
+  std::vector<int32_t> vtri4;   // per vtri entry: triangle, its three corners
+  std::vector<int32_t> vedge4;  // per vedge entry: e*4 + kind, the edge's two ends, 0
   hc_model_params p{};
   int timescale = 1;
 };
@@ -44,6 +48,7 @@ struct TypeDev {
   const int32_t *tri, *edge, *edge_tris, *edge_outer, *ring, *nring, *inner;
   const double *edge_len_eq, *edge_angle_eq, *tri_area_eq, *patch_dist_eq, *inner_len_eq;
   const int32_t *vtri_off, *vtri, *vbend_off, *vbend, *vedge_off, *vedge, *vinner_off, *vinner;
+  const int4 *vtri4, *vedge4;
   double volume_eq, area_mean_eq, edge_mean_eq;
   double k_link, k_bend, k_volume, k_area, eta_m, k_inner_rigid, k_cyto, core_radius, radius;
 };

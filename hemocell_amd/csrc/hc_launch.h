@@ -55,6 +55,14 @@ hipError_t launch_interp_advance(hipStream_t s, const VertexDev& V, const R* uf,
                                  const LatticeDev& L, int do_interp, int do_advance, unsigned long long* wall_hits);
 hipError_t launch_add_extra_force(hipStream_t s, const VertexDev& V, const int64_t* ids, const double* forces, int n);
 
+// fused per-cell step (hc_cell.hip): interpolation + advance, mechanics, next spread
+enum { CELL_FLAG_INTERP = 1, CELL_FLAG_MECH = 2, CELL_FLAG_SPREAD = 4, CELL_FLAG_REPULSION = 8 };
+template <typename R>
+hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
+                            const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L,
+                            R* Fnext, const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
+                            int lds_limit, unsigned long long* wall_hits);
+
 // mechanics (hc_mech.hip)
 hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                             const int64_t* cell_base, double* bend_scratch, int lds_limit);

@@ -288,9 +288,10 @@ __global__ void k_moments(const R* f, const R* Fcur, const uint8_t* type, Lattic
   rhobar_j(fl, rhoBar, j0, j1, j2);
   const R invRho = R(1) / (rhoBar + R(1));
   const int64_t S = L.stride;
-  double u0 = (double)(j0 * invRho + (R(g0) + Fcur[n]) / R(2));
-  double u1 = (double)(j1 * invRho + (R(g1) + Fcur[S + n]) / R(2));
-  double u2 = (double)(j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2));
+  const R F0 = Fcur ? Fcur[n] : R(0), F1 = Fcur ? Fcur[S + n] : R(0), F2 = Fcur ? Fcur[2 * S + n] : R(0);
+  double u0 = (double)(j0 * invRho + (R(g0) + F0) / R(2));
+  double u1 = (double)(j1 * invRho + (R(g1) + F1) / R(2));
+  double u2 = (double)(j2 * invRho + (R(g2) + F2) / R(2));
   double rb = (double)rhoBar;
   if (type && type[n] >= HC_NODE_VELOCITY) {
     const int face = type[n] - HC_NODE_VELOCITY, axis = face >> 1, orient = (face & 1) ? 1 : -1;

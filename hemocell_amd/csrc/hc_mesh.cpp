@@ -391,6 +391,16 @@ bool build_type_tables(TypeHost& T, std::string& err) {
     L[T.inner[2 * k + 1]].push_back(2 * k + 1);
   }
   csr(L, T.vinner_off, T.vinner);
+  T.vtri4.clear();
+  for (int32_t e : T.vtri) {
+    const int t = e >> 2;
+    T.vtri4.insert(T.vtri4.end(), {t, T.tri[3 * t], T.tri[3 * t + 1], T.tri[3 * t + 2]});
+  }
+  T.vedge4.clear();
+  for (int32_t e : T.vedge) {
+    const int ed = e >> 2;
+    T.vedge4.insert(T.vedge4.end(), {e, T.edge[2 * ed], T.edge[2 * ed + 1], 0});
+  }
   return true;
 }
 

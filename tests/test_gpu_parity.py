@@ -277,3 +277,28 @@ def test_stretch_extra_force_fp64():
     xg, _, _ = g.vertices()
     xo, _, _ = o.vertices()
     assert cases.rel_l2(xg, xo) < 1e-10
+
+
+def test_fused_and_unfused_steps_agree(monkeypatch):
+    """The fused vertex kernel (interpolation + advance + mechanics + next spread)
+    against the separate kernels, on a mixed periodic case with a stretch force."""
+    u = cases.units()
+    nx, ny, nz = 40, 30, 36
+    sims = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("HC_FUSE", fuse)
+        g = hc.Simulation(0)
+        g.lattice_init(nx, ny, nz, u.tau)
+        _mixed([g])
+        g.set_extra_vertex_force(np.array([3, 500], dtype=np.int64), np.array([[1e-4, 0, 0], [-1e-4, 0, 0]]))
+        sims.append(g)
+    for _ in range(4):
+        for g in sims:
+            g.step(5)
+    (xa, va, fa), (xb, vb, fb) = sims[0].vertices(), sims[1].vertices()
+    assert cases.rel_l2(xa, xb) < 1e-12
+    assert cases.rel_l2(va, vb) < 1e-8
+    _, ua = sims[0].fluid()
+    _, ub = sims[1].fluid()
+    assert cases.rel_l2(ua, ub) < 1e-10
+    assert np.array_equal(sims[0].force_field(), np.zeros_like(ub.T))
