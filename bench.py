@@ -33,6 +33,24 @@ METRIC = "MLUPS & % HBM roofline, FP32-shifted vs FP64 at Ht 30%; p50/p99 case l
 BYTES_PER_LUP = {"fp32": 44 * 4, "fp64": 44 * 8}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "step_suspension_256x256x512_ht30.md")
+
+
+def profile_traffic(label):
+    """Mean HBM bytes per launch of the collide-stream kernel (even and odd steps) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, or None."""
+    rtype = "float" if label == "fp32" else "double"
+    tot = []
+    try:
+        for line in open(PROFILE):
+            if line.startswith("| `k_collide_stream<%s," % rtype) and line.count("|") == 5:
+                cols = [c.strip() for c in line.strip().strip("|").split("|")]
+                tot.append(float(cols[3]) * 1e6)
+    except OSError:
+        return None
+    return sum(tot) / len(tot) if tot else None
+
+
 def build_case(hc, susp, shape, precision, seed=20261202):
     units = hc.lbm_base_parameters(5e-7, 1e-7)
     types = susp.cell_types(units)
@@ -75,19 +93,32 @@ def run_gpu(hc, susp, shape, precision, steps, warmup):
                     ("velocity_field_or_interp", hc.PROF_INTERP), ("cell_step_or_mechanics", hc.PROF_MECH)):
         ms, cnt = sim.profile_read(k)
         stages[name] = ms / steps
+        if k == hc.PROF_COLLIDE:
+            collide_ms = ms / max(cnt, 1)
     sim.profile_enable(False)
     sim.check_finite()
     N = int(np.prod(shape))
     res = dict(label=label, ms_per_step=1e3 * dt / steps, mlups=N * steps / dt / 1e6,
                vertex_updates_per_s=nv * steps / dt, stage_ms=stages, vertices=nv, nodes=N,
                cells={k: int(len(v)) for k, v in cells.items()})
-    ms, cnt = sim.profile_read(hc.PROF_COLLIDE)
-    res["collide_ms"] = ms / max(cnt, 1)
+    res["collide_ms"] = collide_ms
     res["collide_gbs"] = BYTES_PER_LUP[label] * N / (stages["collide_stream"] * 1e-3) / 1e9
     print("[bench] %s: %.3f ms/step, %.1f MLUPS, collide %.3f ms = %.0f GB/s" % (
         label, res["ms_per_step"], res["mlups"], res["collide_ms"], res["collide_gbs"]), file=sys.stderr, flush=True)
     sim.close()
     return res
+
+
+def whole_step(r):
+    """Algorithmic HBM bytes of one step over the step time: collide-stream (176/352 B per
+    node), velocity field (22 reads + 4 writes per node) and the vertex state of the cell
+    kernel (x read, x, v and f written: 96 B per vertex); the spread's atomic updates are
+    not counted."""
+    rs = 4 if r["label"] == "fp32" else 8
+    b = (BYTES_PER_LUP[r["label"]] + 26 * rs) * r["nodes"] + 96 * r["vertices"]
+    gbs = b / (r["ms_per_step"] * 1e-3) / 1e9
+    return {"algorithmic_bytes": b, "achieved": round(gbs, 1), "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "cell_kernel_ms": round(r["stage_ms"]["cell_step_or_mechanics"], 4)}
 
 
 def cpu_baseline(hc, susp, seconds=12.0):
@@ -154,6 +185,9 @@ def main():
     if args.precision in ("both", "fp64"):
         res["fp64"] = run_gpu(hc, susp, shape, hc.HC_FP64, args.steps, args.warmup)
     head = res.get("fp32") or res["fp64"]
+    traffic = profile_traffic(head["label"])
+    if traffic is not None and shape != (256, 256, 512):
+        traffic = None  # the committed summary is for the 256x256x512 workload only
     line = {
         "metric": METRIC,
         "value": round(head["mlups"], 2),
@@ -171,9 +205,11 @@ def main():
                                "WBC 2562v 1:500, seeded non-overlapping" % shape,
                    "cells": head["cells"], "vertices": head["vertices"], "precision": head["label"]},
         "roofline": {"bound": "hbm", "achieved": round(head["collide_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["collide_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(head["collide_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_collide_stream", "mean_ms": round(head["collide_ms"], 4),
-                     "bytes_per_lup": BYTES_PER_LUP[head["label"]]},
+                     "bytes_per_lup": BYTES_PER_LUP[head["label"]],
+                     "algorithmic_bytes": BYTES_PER_LUP[head["label"]] * head["nodes"]},
+        "whole_step": whole_step(head),
         "vertex_updates_per_s": round(head["vertex_updates_per_s"], 1),
         "stage_ms": {k: round(v, 4) for k, v in head["stage_ms"].items()},
     }

@@ -95,7 +95,9 @@ SIGNATURES = {
     "hc_get_vertices": (_ST, [_P, _D, _D, _D]),
     "hc_set_vertices": (_ST, [_P, _D, _D, _D]),
     "hc_get_repulsion_force": (_ST, [_P, _D]),
+    "hc_get_vertex_bins": (_ST, [_P, _I64]),
     "hc_get_cell_volume_area": (_ST, [_P, _D, _D]),
+    "hc_get_cell_stats": (_ST, [_P, _D, _D, _D, _D, _D, _D]),
     "hc_wall_hits": (ctypes.c_int64, [_P]),
     "hc_check_finite": (_ST, [_P]),
     "hc_profile_enable": (_ST, [_P, _INT]),
@@ -235,6 +237,9 @@ class Simulation:
     def set_force_limit(self, fl):
         self._chk(self.lib.hc_set_force_limit(self.h, fl))
 
+    def set_repulsion(self, k_rep, cutoff_lu, timescale=1):
+        self._chk(self.lib.hc_set_repulsion(self.h, k_rep, cutoff_lu, timescale))
+
     def set_extra_vertex_force(self, ids, forces):
         ids = np.ascontiguousarray(ids, dtype=np.int64)
         f = np.ascontiguousarray(forces, dtype=np.float64)
@@ -289,11 +294,30 @@ class Simulation:
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (x, v, f)]
         self._chk(self.lib.hc_set_vertices(self.h, *[_dp(a) for a in arrs]))
 
+    def repulsion_force(self):
+        fr = np.zeros((self.num_vertices, 3))
+        self._chk(self.lib.hc_get_repulsion_force(self.h, _dp(fr)))
+        return fr
+
+    def vertex_bins(self):
+        b = np.zeros(self.num_vertices, dtype=np.int64)
+        self._chk(self.lib.hc_get_vertex_bins(self.h, b.ctypes.data_as(_I64)))
+        return b
+
     def cell_volume_area(self):
         nc = self.lib.hc_num_cells(self.h)
         vol, area = np.zeros(nc), np.zeros(nc)
         self._chk(self.lib.hc_get_cell_volume_area(self.h, _dp(vol), _dp(area)))
         return vol, area
+
+    def cell_stats(self):
+        """Per-cell dict of arrays: volume, area, centroid, velocity, bbox, stretch (lattice units)."""
+        nc = self.lib.hc_num_cells(self.h)
+        out = dict(volume=np.zeros(nc), area=np.zeros(nc), centroid=np.zeros((nc, 3)), velocity=np.zeros((nc, 3)),
+                   bbox=np.zeros((nc, 6)), stretch=np.zeros(nc))
+        self._chk(self.lib.hc_get_cell_stats(self.h, *[_dp(out[k]) for k in
+                                                       ("volume", "area", "centroid", "velocity", "bbox", "stretch")]))
+        return out
 
     def wall_hits(self):
         return self.lib.hc_wall_hits(self.h)

@@ -91,6 +91,10 @@ struct hc_ctx {
   bool repulsion = false;
   double rep_k = 0, rep_cutoff = 0;
   int rep_timescale = 1;
+  // binning and repulsion buffers, built on first use and whenever the vertex count changed
+  RepulsionBufs rb{};
+  int64_t rb_nv = -1;
+  int32_t* d_vcell = nullptr;
   unsigned long long* d_wall_hits = nullptr;
   unsigned int* d_flag = nullptr;
   Prof prof;
@@ -269,6 +273,56 @@ hc_status build_dense_extra(hc_ctx* ctx) {
   return HC_OK;
 }
 
+void free_repulsion(hc_ctx* ctx) {
+  void* ps[] = {ctx->rb.keys_in, ctx->rb.keys_out, ctx->rb.ids_in, ctx->rb.ids_out, ctx->rb.temp, ctx->d_vcell};
+  for (void* p : ps)
+    if (p) hipFree(p);
+  RepulsionBufs z{};
+  z.bins = ctx->rb.bins;
+  ctx->rb = z;
+  ctx->d_vcell = nullptr;
+  ctx->rb_nv = -1;
+}
+
+// Sort buffers sized for the current vertices, the cell index of every vertex,
+// and the per-node bin bounds (zeroed once; the repulsion launch leaves them zero).
+hc_status ensure_repulsion(hc_ctx* ctx) {
+  hc_status s;
+  if (!ctx->rb.bins) {
+    if ((s = dalloc(ctx, (void**)&ctx->rb.bins, sizeof(int2) * ctx->L.N, "repulsion bins")) != HC_OK) return s;
+    HC_TRY(hipMemset(ctx->rb.bins, 0, sizeof(int2) * ctx->L.N), "hipMemset");
+  }
+  if (ctx->rb_nv == ctx->nv) return HC_OK;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  free_repulsion(ctx);
+  const int64_t nv = std::max<int64_t>(ctx->nv, 1);
+  void** bufs[4] = {(void**)&ctx->rb.keys_in, (void**)&ctx->rb.keys_out, (void**)&ctx->rb.ids_in, (void**)&ctx->rb.ids_out};
+  for (void** b : bufs)
+    if ((s = dalloc(ctx, b, 4 * nv, "repulsion sort buffers")) != HC_OK) return s;
+  ctx->rb.temp_bytes = repulsion_temp_bytes(ctx->nv, ctx->L.N);
+  if ((s = dalloc(ctx, &ctx->rb.temp, std::max<size_t>(ctx->rb.temp_bytes, 16), "radix-sort scratch")) != HC_OK) return s;
+  std::vector<int32_t> vcell((size_t)nv, 0);
+  for (size_t c = 0; c < ctx->cell_base.size(); ++c) {
+    const int n = ctx->types[ctx->cell_type[c]]->host.nv;
+    for (int k = 0; k < n; ++k) vcell[(size_t)ctx->cell_base[c] + k] = (int32_t)c;
+  }
+  if ((s = dalloc(ctx, (void**)&ctx->d_vcell, sizeof(int32_t) * nv, "vertex cells")) != HC_OK) return s;
+  HC_TRY(hipMemcpy(ctx->d_vcell, vcell.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice), "hipMemcpy");
+  ctx->rb.vcell = ctx->d_vcell;
+  ctx->rb_nv = ctx->nv;
+  return HC_OK;
+}
+
+hc_status do_repulsion(hc_ctx* ctx) {
+  hc_status s = ensure_repulsion(ctx);
+  if (s != HC_OK) return s;
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_REPULSION, &ev);
+  HC_TRY(launch_repulsion(ctx->stream, vdev(ctx), ctx->L, ctx->rb, ctx->rep_k, ctx->rep_cutoff), "repulsion");
+  prof_end(ctx, HC_PROF_REPULSION, ev);
+  return HC_OK;
+}
+
 template <typename R>
 hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
   hipEvent_t ev;
@@ -379,10 +433,8 @@ hc_status step_impl(hc_ctx* ctx, int nsteps) {
     hipEvent_t ev;
     prof_begin(ctx, HC_PROF_STEP, &ev);
     const bool fused = ctx->fuse && !ctx->repulsion && ctx->nv > 0;
-    if (ctx->repulsion && ctx->iter % ctx->rep_timescale == 0) {
-      s = hc_stage_repulsion(ctx);
-      if (s != HC_OK) return s;
-    }
+    if (ctx->repulsion && ctx->nv > 0 && ctx->iter % ctx->rep_timescale == 0 && (s = do_repulsion(ctx)) != HC_OK)
+      return s;
     if (ctx->nv > 0 && !ctx->spread_ready && (s = do_spread<R>(ctx)) != HC_OK) return s;
     if ((s = do_collide<R>(ctx)) != HC_OK) return s;
     const int do_interp = ctx->iter % ctx->pv_timescale == 0;
@@ -465,6 +517,8 @@ void hc_destroy(hc_ctx* ctx) {
     for (void* p : tp->allocs) hipFree(p);
     if (tp->d_cells) hipFree(tp->d_cells);
   }
+  free_repulsion(ctx);
+  if (ctx->rb.bins) hipFree(ctx->rb.bins);
   for (hipEvent_t e : ctx->prof.pool) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -686,10 +740,17 @@ hc_status hc_set_force_limit(hc_ctx* ctx, double f_limit) {
 }
 hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale) {
   if (!ctx) return HC_INVALID_ARG;
-  (void)k_rep;
-  (void)cutoff_lu;
-  (void)timescale;
-  return fail(ctx, HC_STATE, "cell-cell repulsion is not available in this build");
+  if (!std::isfinite(k_rep) || !(cutoff_lu > 0.0) || !std::isfinite(cutoff_lu) || timescale < 1)
+    return fail(ctx, HC_INVALID_ARG, "repulsion needs a finite constant, a cut-off > 0 and a timescale >= 1");
+  if (ctx->lattice_ready) {
+    hc_status s = invalidate_spread(ctx);
+    if (s != HC_OK) return s;
+  }
+  ctx->repulsion = true;
+  ctx->rep_k = k_rep;
+  ctx->rep_cutoff = cutoff_lu;
+  ctx->rep_timescale = timescale;
+  return HC_OK;
 }
 
 hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* ids, const double* forces) {
@@ -766,8 +827,11 @@ hc_status hc_stage_mechanics(hc_ctx* ctx, int forced) {
   return do_mechanics(ctx, forced != 0);
 }
 hc_status hc_stage_repulsion(hc_ctx* ctx) {
-  if (!ctx) return HC_INVALID_ARG;
-  return fail(ctx, HC_STATE, "cell-cell repulsion is not available in this build");
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  if (!ctx->repulsion) return fail(ctx, HC_STATE, "repulsion not enabled (hc_set_repulsion)");
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
+  return ctx->nv > 0 ? do_repulsion(ctx) : HC_OK;
 }
 hc_status hc_stage_end(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
@@ -904,31 +968,56 @@ hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep) {
   return get_planes(ctx, ctx->frep, frep);
 }
 
-hc_status hc_get_cell_volume_area(hc_ctx* ctx, double* volume, double* area) {
+hc_status hc_get_cell_stats(hc_ctx* ctx, double* volume, double* area, double* centroid, double* velocity,
+                            double* bbox, double* stretch) {
   if (!ctx) return HC_INVALID_ARG;
   hc_status s = sync_cells(ctx);
   if (s != HC_OK) return s;
   const int nc = (int)ctx->cell_base.size();
   if (nc == 0) return HC_OK;
-  double *dv = nullptr, *da = nullptr;
-  if ((s = dalloc(ctx, (void**)&dv, sizeof(double) * nc, "volume")) != HC_OK) return s;
-  if ((s = dalloc(ctx, (void**)&da, sizeof(double) * nc, "area")) != HC_OK) {
-    hipFree(dv);
-    return s;
-  }
+  constexpr int NQ = 15;
+  double* d = nullptr;
+  if ((s = dalloc(ctx, (void**)&d, sizeof(double) * NQ * nc, "cell statistics")) != HC_OK) return s;
   hipError_t e = hipSuccess;
   for (auto& tp : ctx->types)
     if (e == hipSuccess && tp->ncells)
-      e = launch_cell_volume_area(ctx->stream, vdev(ctx), tp->dev, tp->d_cells, tp->ncells, ctx->d_cell_base, dv, da);
-  std::vector<double> hv(nc), ha(nc);
-  if (e == hipSuccess) e = hipMemcpyAsync(hv.data(), dv, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(ha.data(), da, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream);
+      e = launch_cell_stats(ctx->stream, vdev(ctx), tp->dev, tp->d_cells, tp->ncells, ctx->d_cell_base, d);
+  std::vector<double> h((size_t)NQ * nc);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(double) * NQ * nc, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  hipFree(dv);
-  hipFree(da);
-  if (e != hipSuccess) return hipfail(ctx, e, "cell volume/area");
-  if (volume) std::copy(hv.begin(), hv.end(), volume);
-  if (area) std::copy(ha.begin(), ha.end(), area);
+  hipFree(d);
+  if (e != hipSuccess) return hipfail(ctx, e, "cell statistics");
+  for (int c = 0; c < nc; ++c) {
+    const double* q = &h[(size_t)NQ * c];
+    if (volume) volume[c] = q[0];
+    if (area) area[c] = q[1];
+    for (int k = 0; k < 3; ++k) {
+      if (centroid) centroid[3 * c + k] = q[2 + k];
+      if (velocity) velocity[3 * c + k] = q[5 + k];
+    }
+    if (bbox)
+      for (int k = 0; k < 6; ++k) bbox[6 * c + k] = q[8 + k];
+    if (stretch) stretch[c] = q[14];
+  }
+  return HC_OK;
+}
+
+hc_status hc_get_cell_volume_area(hc_ctx* ctx, double* volume, double* area) {
+  return hc_get_cell_stats(ctx, volume, area, nullptr, nullptr, nullptr, nullptr);
+}
+
+hc_status hc_get_vertex_bins(hc_ctx* ctx, int64_t* bins) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  if (!bins) return fail(ctx, HC_INVALID_ARG, "null output");
+  if (ctx->nv == 0) return HC_OK;
+  int64_t* d = nullptr;
+  if ((s = dalloc(ctx, (void**)&d, sizeof(int64_t) * ctx->nv, "vertex bins")) != HC_OK) return s;
+  hipError_t e = launch_vertex_bins(ctx->stream, vdev(ctx), ctx->L, d);
+  if (e == hipSuccess) e = hipMemcpyAsync(bins, d, sizeof(int64_t) * ctx->nv, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return hipfail(ctx, e, "vertex bins");
   return HC_OK;
 }
 

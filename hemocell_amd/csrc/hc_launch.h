@@ -63,11 +63,25 @@ hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T,
                             R* Fnext, const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
                             int lds_limit, unsigned long long* wall_hits);
 
+// binning and cell-cell repulsion (hc_repulsion.hip)
+struct RepulsionBufs {
+  uint32_t *keys_in, *keys_out;  // nearest-node key per vertex, before / after the sort
+  int32_t *ids_in, *ids_out;     // vertex ids, sorted alongside
+  int2* bins;                    // [start, end) per node into ids_out; all zero between launches
+  const int32_t* vcell;          // cell index of each vertex
+  void* temp;                    // radix-sort scratch
+  size_t temp_bytes;
+};
+size_t repulsion_temp_bytes(int64_t nv, int64_t N);
+hipError_t launch_repulsion(hipStream_t s, const VertexDev& V, const LatticeDev& L, const RepulsionBufs& B,
+                            double k_rep, double r_c);
+hipError_t launch_vertex_bins(hipStream_t s, const VertexDev& V, const LatticeDev& L, int64_t* out);
+
 // mechanics (hc_mech.hip)
 hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                             const int64_t* cell_base, double* bend_scratch, int lds_limit);
-hipError_t launch_cell_volume_area(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells,
-                                   int ncells, const int64_t* cell_base, double* vol, double* area);
+hipError_t launch_cell_stats(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
+                             const int64_t* cell_base, double* out);
 hipError_t launch_check_finite(hipStream_t s, const double* p, int64_t n, unsigned int* bad);
 hipError_t launch_check_finite_f(hipStream_t s, const float* p, int64_t n, unsigned int* bad);
 

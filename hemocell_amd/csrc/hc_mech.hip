@@ -116,35 +116,73 @@ __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const 
   }
 }
 
+// Per-cell statistics of CellInformationFunctionals (helper/cellInfo.cpp:170-266):
+// volume (sum over triangles of the signed triple product / 6), area, centroid
+// and mean velocity (vertex means), bounding box and the largest vertex-vertex
+// distance.  out[15*c + ...] = vol, area, centroid[3], velocity[3], bbox[6]
+// (xmin, xmax, ymin, ymax, zmin, zmax), stretch.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_volume_area(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
-                                                    const int64_t* __restrict__ cell_base, double* vol, double* area) {
-  __shared__ double rv[BS], ra[BS];
+__global__ __launch_bounds__(BS) void k_cell_stats(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
+                                                   const int64_t* __restrict__ cell_base, double* __restrict__ out) {
+  constexpr int NQ = 15;
+  __shared__ double red[NQ][BS];
   const int cell = cells[blockIdx.x];
   const int64_t base = cell_base[cell];
-  const CellView c{V.x + base, V.x + V.vs + base, V.x + 2 * V.vs + base};
+  const int64_t vs = V.vs;
+  const CellView c{V.x + base, V.x + vs + base, V.x + 2 * vs + base};
   const int tid = threadIdx.x;
-  double pv = 0.0, pa = 0.0;
+  double q[NQ];
+  for (int k = 0; k < NQ; ++k) q[k] = 0.0;
+  q[8] = q[10] = q[12] = INFINITY;
+  q[9] = q[11] = q[13] = -INFINITY;
   for (int t = tid; t < T.nt; t += BS) {
-    D3 v0, v1, v2, n;
-    double a;
-    tri_geom(c, T, t, v0, v1, v2, a, n);
-    pv += dotp(v0, crossp(v1, v2));
-    pa += a;
+    const D3 v0 = c.pos(T.tri[3 * t]), v1 = c.pos(T.tri[3 * t + 1]), v2 = c.pos(T.tri[3 * t + 2]);
+    const double v210 = v2.x * v1.y * v0.z;
+    const double v120 = v1.x * v2.y * v0.z;
+    const double v201 = v2.x * v0.y * v1.z;
+    const double v021 = v0.x * v2.y * v1.z;
+    const double v102 = v1.x * v0.y * v2.z;
+    const double v012 = v0.x * v1.y * v2.z;
+    q[0] += (1.0 / 6.0) * (-v210 + v120 + v201 - v021 - v102 + v012);
+    q[1] += 0.5 * normp(crossp(v1 - v0, v2 - v0));
   }
-  rv[tid] = pv;
-  ra[tid] = pa;
+  for (int i = tid; i < T.nv; i += BS) {
+    const D3 p = c.pos(i);
+    q[2] += p.x;
+    q[3] += p.y;
+    q[4] += p.z;
+    q[5] += V.v[base + i];
+    q[6] += V.v[vs + base + i];
+    q[7] += V.v[2 * vs + base + i];
+    q[8] = fmin(q[8], p.x);
+    q[9] = fmax(q[9], p.x);
+    q[10] = fmin(q[10], p.y);
+    q[11] = fmax(q[11], p.y);
+    q[12] = fmin(q[12], p.z);
+    q[13] = fmax(q[13], p.z);
+    for (int j = i + 1; j < T.nv; ++j) {
+      const D3 r = c.pos(j);
+      const double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+      q[14] = fmax(q[14], sqrt(dx * dx + dy * dy + dz * dz));
+    }
+  }
+  for (int k = 0; k < NQ; ++k) red[k][tid] = q[k];
   __syncthreads();
   for (int h = BS / 2; h > 0; h >>= 1) {
     if (tid < h) {
-      rv[tid] += rv[tid + h];
-      ra[tid] += ra[tid + h];
+      for (int k = 0; k < 8; ++k) red[k][tid] += red[k][tid + h];
+      for (int k = 8; k < 14; k += 2) {
+        red[k][tid] = fmin(red[k][tid], red[k][tid + h]);
+        red[k + 1][tid] = fmax(red[k + 1][tid], red[k + 1][tid + h]);
+      }
+      red[14][tid] = fmax(red[14][tid], red[14][tid + h]);
     }
     __syncthreads();
   }
-  if (tid == 0) {
-    vol[cell] = rv[0] / 6.0;
-    area[cell] = ra[0];
+  if (tid < NQ) {
+    double r = red[tid][0];
+    if (tid >= 2 && tid < 8) r /= (double)T.nv;
+    out[(int64_t)NQ * cell + tid] = r;
   }
 }
 
@@ -193,10 +231,10 @@ hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T,
   return hipGetLastError();
 }
 
-hipError_t launch_cell_volume_area(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells,
-                                   int ncells, const int64_t* cell_base, double* vol, double* area) {
+hipError_t launch_cell_stats(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
+                             const int64_t* cell_base, double* out) {
   if (ncells <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_volume_area<256>), dim3((unsigned)ncells), dim3(256), 0, s, V, T, cells, cell_base, vol, area);
+  hipLaunchKernelGGL((k_cell_stats<128>), dim3((unsigned)ncells), dim3(128), 0, s, V, T, cells, cell_base, out);
   return hipGetLastError();
 }
 

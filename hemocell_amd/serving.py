@@ -1,0 +1,183 @@
+"""Serving many small single-cell cases on one GPU.
+
+Each case is one cell (RBC, PLT or WBC) in a cubic box, either sheared between
+two velocity walls (the Couette set-up of helper/hemocellInit.hh:58-92) or
+pulled by a stretching force on 7 vertices per end inside zero-velocity walls
+(helper/hemoCellStretch.cpp:30-110, as in tests/validation/stretch_cell).
+Cases arrive as a Poisson stream; every case gets its own context (its own
+HIP stream and device buffers), so cases never share state and a case's
+result does not depend on what else runs.  One host thread admits arrivals,
+feeds every active case a chunk of steps at a time (the launches are
+asynchronous) and retires the cases whose stream has drained; the latency of
+a case is its completion time minus its arrival time, set-up included.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import hemocell_amd as hc
+from hemocell_amd import suspension
+
+
+@dataclass
+class Case:
+    ident: int
+    kind: str            # "shear" or "stretch"
+    cell: str            # "RBC", "PLT" or "WBC"
+    edge: int            # lattice nodes per side
+    steps: int
+    shear_rate: float = 0.0  # 1/s (shear)
+    force_pN: float = 0.0    # total pulling force per end (stretch)
+    arrival: float = 0.0     # s after the start of the stream
+    # filled in by the server
+    started: float = math.nan
+    finished: float = math.nan
+    result: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def make_cases(n, rate, seed=7, edge=(32, 96), steps=(2000, 20000)):
+    """A seeded stream of n cases arriving at `rate` cases/s (Poisson): shear rate
+    log-uniform in 10..1000 1/s or stretch force uniform in 0..200 pN, cell kind,
+    cubic edge and step count uniform in their ranges."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        kind = "shear" if rng.uniform() < 0.5 else "stretch"
+        cell = ("RBC", "PLT", "WBC")[int(rng.integers(0, 3))]
+        c = Case(ident=i, kind=kind, cell=cell, edge=int(rng.integers(edge[0], edge[1] + 1)),
+                 steps=int(rng.integers(steps[0], steps[1] + 1)), arrival=t)
+        if kind == "shear":
+            c.shear_rate = float(10.0 ** rng.uniform(1.0, 3.0))
+        else:
+            c.force_pN = float(rng.uniform(0.0, 200.0))
+        out.append(c)
+    return out
+
+
+_TYPES = {}
+
+
+def _cell_types(units):
+    key = (units.dx, units.dt)
+    if key not in _TYPES:
+        _TYPES[key] = suspension.cell_types(units, wbc_triangles=600)
+    return _TYPES[key]
+
+
+def build(case, units, precision=hc.HC_FP32_SHIFTED, device=0):
+    """The context of one case, set up and ready to step."""
+    E = case.edge
+    types = _cell_types(units)
+    t = types[case.cell]
+    sim = hc.Simulation(device)
+    mask = np.zeros((E, E, E), dtype=np.uint8)
+    if case.kind == "shear":
+        # velocity walls z = 0 and z = E-1, periodic in x and y
+        mask[:, :, 0] = hc.HC_NODE_VELOCITY + 4
+        mask[:, :, E - 1] = hc.HC_NODE_VELOCITY + 5
+        sim.lattice_init(E, E, E, units.tau, periodic=(1, 1, 0), node_type=mask.reshape(-1), precision=precision)
+        v_half = (E - 1) * case.shear_rate * units.dt * 0.5
+        sim.set_wall_velocity(4, (v_half, 0.0, 0.0))
+        sim.set_wall_velocity(5, (-v_half, 0.0, 0.0))
+    else:
+        # zero-velocity walls on every face (the stretch validation's box)
+        for a in range(3):
+            for side, face in ((0, 2 * a), (E - 1, 2 * a + 1)):
+                sl = [slice(None)] * 3
+                sl[a] = side
+                face_mask = mask[tuple(sl)]
+                face_mask[face_mask == 0] = hc.HC_NODE_VELOCITY + face
+        sim.lattice_init(E, E, E, units.tau, periodic=(0, 0, 0), node_type=mask.reshape(-1), precision=precision)
+    sim.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    sim.set_force_limit(units.f_limit)
+    tid = sim.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner)
+    x = t.verts - 0.5 * (t.verts.min(axis=0) + t.verts.max(axis=0)) + (E - 1) / 2.0
+    sim.add_cells(tid, x[None])
+    if case.kind == "stretch":
+        order = np.argsort(x[:, 0], kind="stable")
+        n = 7
+        f = case.force_pN * 1e-12 / units.df / n
+        ids = np.concatenate([order[:n], order[::-1][:n]]).astype(np.int64)
+        forces = np.zeros((2 * n, 3))
+        forces[:n, 0] = -f
+        forces[n:, 0] = f
+        sim.set_extra_vertex_force(ids, forces)
+    return sim
+
+
+def summarise(sim):
+    st = sim.cell_stats()
+    return dict(volume=float(st["volume"][0]), stretch=float(st["stretch"][0]),
+                bbox=[float(b) for b in st["bbox"][0]], centroid=[float(c) for c in st["centroid"][0]])
+
+
+def run_alone(case, units, precision=hc.HC_FP32_SHIFTED):
+    sim = build(case, units, precision)
+    sim.step(case.steps)
+    sim.synchronize()
+    r = summarise(sim)
+    x = sim.vertices()[0]
+    sim.close()
+    return r, x
+
+
+def serve(cases, units, max_active=16, chunk=500, precision=hc.HC_FP32_SHIFTED, keep_vertices=False,
+          progress=None):
+    """Run the stream in real time; returns the cases with latencies and results."""
+    pending = sorted(cases, key=lambda c: c.arrival)
+    active = []  # (case, sim, steps_left)
+    done = []
+    t0 = time.perf_counter()
+    k = 0
+    while k < len(pending) or active:
+        now = time.perf_counter() - t0
+        while k < len(pending) and pending[k].arrival <= now and len(active) < max_active:
+            c = pending[k]
+            k += 1
+            c.started = time.perf_counter() - t0
+            active.append([c, build(c, units, precision), c.steps])
+        idle = True
+        for a in active:
+            c, sim, left = a
+            if left > 0 and not sim.busy():
+                n = min(chunk, left)
+                sim.step(n)
+                a[2] = left - n
+                idle = False
+        still = []
+        for a in active:
+            c, sim, left = a
+            if left == 0 and not sim.busy():
+                c.finished = time.perf_counter() - t0
+                c.result = summarise(sim)
+                if keep_vertices:
+                    c.result["x"] = sim.vertices()[0]
+                sim.close()
+                done.append(c)
+                if progress:
+                    progress(c, len(done))
+            else:
+                still.append(a)
+        active = still
+        if idle and k < len(pending) and not active:
+            # nothing in flight: wait for the next arrival
+            time.sleep(max(0.0, min(0.01, pending[k].arrival - (time.perf_counter() - t0))))
+        elif idle:
+            time.sleep(0.0002)
+    return done, time.perf_counter() - t0
+
+
+def latency_stats(done, wall):
+    lat = np.array([c.latency for c in done])
+    return dict(cases=len(done), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), cases_per_s=len(done) / wall,
+                steps_per_s=float(sum(c.steps for c in done) / wall),
+                lups_per_s=float(sum(c.steps * c.edge ** 3 for c in done) / wall))

@@ -174,8 +174,17 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u /* 3*N, n*3+c */);
 hc_status hc_get_vertices(hc_ctx* ctx, double* x, double* v, double* f);
 hc_status hc_set_vertices(hc_ctx* ctx, const double* x, const double* v, const double* f);
 hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep);
+/* nearest-node bin of every vertex, node index z + nz*(y + ny*x) or -1 outside the domain.
+ * Replaces: HemoCellParticleField::update_pg (core/hemoCellParticleField.cpp:137-168). */
+hc_status hc_get_vertex_bins(hc_ctx* ctx, int64_t* bins);
 /* per-cell FP64 volume and area of the current shape (helper/cellInfo.cpp:39-120) */
 hc_status hc_get_cell_volume_area(hc_ctx* ctx, double* volume, double* area);
+/* Replaces: CellInformationFunctionals::allCellInformation (helper/cellInfo.cpp:170-266): per
+ * cell (in cell order) volume, area, centroid[3] and mean velocity[3] (vertex means),
+ * bounding box[6] (xmin, xmax, ymin, ymax, zmin, zmax) and the largest vertex-vertex
+ * distance ("stretch"); lattice units; any pointer may be NULL. */
+hc_status hc_get_cell_stats(hc_ctx* ctx, double* volume, double* area, double* centroid,
+                            double* velocity, double* bbox, double* stretch);
 /* vertices whose nearest node became a wall node during advance (the reference deletes
  * their cells, hemoCellParticleField.cpp:570-584) */
 int64_t hc_wall_hits(hc_ctx* ctx);

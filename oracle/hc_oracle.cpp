@@ -957,6 +957,20 @@ void ho_set_vertices(void* h, const double* x, const double* v, const double* f)
   if (v) std::copy(v, v + o->v.size(), o->v.begin());
   if (f) std::copy(f, f + o->force.size(), o->force.begin());
 }
+// Nearest-node bin of every vertex as update_pg forms it (hemoCellParticleField.cpp:150-163),
+// -1 for a vertex outside the domain.
+void ho_vertex_bins(void* h, long* out) {
+  Oracle* o = (Oracle*)h;
+  for (long i = 0; i < o->NV(); ++i) {
+    int w3[3];
+    bool out_of = false;
+    for (int a = 0; a < 3; ++a) {
+      w3[a] = o->wrap(a, (long)std::floor(o->x[3 * i + a] + 0.5));
+      if (w3[a] < 0) out_of = true;
+    }
+    out[i] = out_of ? -1 : o->idx(w3[0], w3[1], w3[2]);
+  }
+}
 void ho_get_repulsion(void* h, double* fr) { Oracle* o = (Oracle*)h; std::copy(o->frep.begin(), o->frep.end(), fr); }
 
 // Individual stages, in iterate() order, for stage-level parity tests.

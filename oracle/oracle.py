@@ -44,6 +44,7 @@ _SIG = {
     "ho_set_vertices": (None, [_P, _D, _D, _D]),
     "ho_get_repulsion": (None, [_P, _D]),
     "ho_stage_repulsion": (None, [_P]),
+    "ho_vertex_bins": (None, [_P, ctypes.POINTER(ctypes.c_long)]),
     "ho_stage_spread": (None, [_P]),
     "ho_stage_collide_stream": (None, [_P]),
     "ho_stage_interpolate": (None, [_P]),
@@ -117,6 +118,19 @@ class OracleSim:
 
     def set_force_limit(self, fl):
         self.lib.ho_set_f_limit(self.h, fl)
+
+    def set_repulsion(self, k_rep, cutoff_lu, timescale=1):
+        self.lib.ho_set_repulsion(self.h, k_rep, cutoff_lu, timescale)
+
+    def repulsion_force(self):
+        fr = np.zeros((self.num_vertices, 3))
+        self.lib.ho_get_repulsion(self.h, _dp(fr))
+        return fr
+
+    def vertex_bins(self):
+        b = np.zeros(self.num_vertices, dtype=np.int64)
+        self.lib.ho_vertex_bins(self.h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        return b
 
     def set_velocity_timescale(self, ts):
         self.lib.ho_set_timescales(self.h, ts)

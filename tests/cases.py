@@ -145,3 +145,130 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def crowd(kind, n, shape, rng, u, builder, margin=0.0):
+    """n cells of one kind at seeded random centres and orientations in a box of
+    the given shape (overlaps allowed): (model, verts, tris, params, inner, xyz[n, nv, 3])."""
+    model, v, t, p, inner = mesh(kind, u, builder)
+    lo = np.full(3, margin)
+    hi = np.asarray(shape, dtype=np.float64) - margin
+    xs = np.stack([place(v, rng.uniform(lo, hi), rotation(rng)) for _ in range(n)])
+    return model, v, t, p, inner, xs
+
+
+def repulsion_brute_force(x, cell, shape, periodic, k_rep, r_c):
+    """Cell-cell repulsion of core/hemoCellParticleField.cpp:677-743 restated as a
+    sum over all vertex pairs: vertices are binned at their nearest node; a pair
+    of vertices of different cells closer than r_c acts once when their bins are
+    distinct neighbours (one visit from the half stencil) and twice when they
+    share a bin (both orders are visited).  Needs every periodic axis >= 3."""
+    n = np.asarray(shape)
+    b = np.floor(x + 0.5).astype(np.int64)
+    inside = np.ones(len(x), dtype=bool)
+    for a in range(3):
+        if periodic[a]:
+            b[:, a] %= n[a]
+        else:
+            inside &= (b[:, a] >= 0) & (b[:, a] < n[a])
+    F = np.zeros_like(x)
+    idx = np.flatnonzero(inside)
+    for i in idx:
+        d = x[i] - x[idx]
+        db = b[i] - b[idx]
+        for a in range(3):
+            if periodic[a]:
+                d[:, a] -= n[a] * np.floor(d[:, a] / n[a] + 0.5)
+                db[:, a] = (db[:, a] + 1) % n[a] - 1
+        dist = np.sqrt((d * d).sum(axis=1))
+        nb = np.all(np.abs(db) <= 1, axis=1)
+        same = np.all(db == 0, axis=1)
+        ok = nb & (cell[idx] != cell[i]) & (dist < r_c)
+        mult = np.where(same, 2.0, 1.0)[ok]
+        s = k_rep * (r_c / dist[ok])
+        F[i] = (mult[:, None] * s[:, None] * d[ok] / dist[ok, None]).sum(axis=0)
+    return F
+
+
+def cell_stats_reference(x, v, tris):
+    """CellInformationFunctionals of one cell (helper/cellInfo.cpp:170-266) in numpy."""
+    v0, v1, v2 = x[tris[:, 0]], x[tris[:, 1]], x[tris[:, 2]]
+    trip = (-v2[:, 0] * v1[:, 1] * v0[:, 2] + v1[:, 0] * v2[:, 1] * v0[:, 2] + v2[:, 0] * v0[:, 1] * v1[:, 2]
+            - v0[:, 0] * v2[:, 1] * v1[:, 2] - v1[:, 0] * v0[:, 1] * v2[:, 2] + v0[:, 0] * v1[:, 1] * v2[:, 2])
+    area = 0.5 * np.linalg.norm(np.cross(v1 - v0, v2 - v0), axis=1)
+    d = x[:, None, :] - x[None, :, :]
+    stretch = np.sqrt((d * d).sum(axis=2)).max()
+    bbox = np.array([x[:, 0].min(), x[:, 0].max(), x[:, 1].min(), x[:, 1].max(), x[:, 2].min(), x[:, 2].max()])
+    return dict(volume=(trip / 6.0).sum(), area=area.sum(), centroid=x.mean(axis=0), velocity=v.mean(axis=0),
+                bbox=bbox, stretch=stretch)
+
+
+def material_xml(mat, name, inner_edges=0):
+    """A <MaterialModel> file with the given constants (SI, as the reference templates)."""
+    rows = "".join("    <%s> %r </%s>\n" % (k, v, k) for k, v in mat.items())
+    if inner_edges:
+        rows += "    <InnerEdges>\n" + "".join("        <Edge> %d %d </Edge>\n" % (2 * i, 2 * i + 1)
+                                             for i in range(inner_edges)) + "    </InnerEdges>\n"
+    return ('<?xml version="1.0" ?>\n<hemocell>\n<MaterialModel>\n    <name>%s</name>\n%s</MaterialModel>\n'
+            '</hemocell>\n' % (name, rows))
+
+
+def config_xml(dx=5e-7, dt=1e-7, nu_p=1.1e-6, rho_p=1025.0, kBT=KBT):
+    return ('<?xml version="1.0" ?>\n<hemocell>\n<domain>\n    <!-- lattice units -->\n'
+            '    <rhoP> %r </rhoP>\n    <nuP> %r </nuP>\n    <dx> %r </dx>\n    <dt> %r </dt>\n'
+            '    <kBT> %r </kBT>\n    <particleEnvelope>20</particleEnvelope>\n</domain>\n'
+            '<sim> <tmax> 10 </tmax> </sim>\n</hemocell>\n' % (rho_p, nu_p, dx, dt, kBT))
+
+
+def pos_placement(mesh_v, lines, dx, boundary, deny_um, first_id):
+    """Cells placed as io/readPositionsBloodCells.cpp:36-361 does: the mesh is
+    centred on its bounding box, rotated by the negated angles (degrees) with
+    the matrices of rotateTriangularMeshXYZ about its bounding-box centre and
+    moved to the listed centre (um).  A cell is dropped when the nearest node
+    int(v + 0.5) of one of its vertices, or a node within the deny layer
+    around it, is a boundary node.  Returns (ids, positions) of the kept cells."""
+    lo, hi = mesh_v.min(axis=0), mesh_v.max(axis=0)
+    base = mesh_v - (lo + hi) / 2.0
+    ctr = (base.max(axis=0) + base.min(axis=0)) * 0.5
+    ratio = 1e-6 / dx
+    deny = int((deny_um * 1e-6) / dx)
+    dims = boundary.shape
+    ids, cells = [], []
+    for k, (p, ang) in enumerate(lines):
+        al, be, ga = [a * (math.pi / 180.0) * -1.0 for a in ang]
+        A = [[1.0, 0.0, 0.0], [0.0, math.cos(al), math.sin(al)], [0.0, -math.sin(al), math.cos(al)]]
+        B = [[math.cos(be), 0.0, -math.sin(be)], [0.0, 1.0, 0.0], [math.sin(be), 0.0, math.cos(be)]]
+        C = [[0.0] * 3 for _ in range(3)]
+        for i in range(3):
+            for j in range(3):
+                for q in range(3):
+                    C[i][j] += A[q][j] * B[i][q]
+        Z = [[math.cos(ga), math.sin(ga), 0.0], [-math.sin(ga), math.cos(ga), 0.0], [0.0, 0.0, 1.0]]
+        M = [[0.0] * 3 for _ in range(3)]
+        for i in range(3):
+            for j in range(3):
+                for q in range(3):
+                    M[i][j] += C[q][j] * Z[i][q]
+        q = base + -1.0 * ctr
+        rq = np.zeros_like(q)
+        for i in range(3):
+            acc = np.zeros(len(q))
+            for j in range(3):
+                acc = acc + M[i][j] * q[:, j]
+            rq[:, i] = acc
+        v = np.asarray(p, dtype=np.float64) * ratio + (rq + ctr)
+        keep = True
+        for x in v:
+            n = [int(c + 0.5) for c in x]
+            for dxx in range(-deny, deny + 1):
+                for dyy in range(-deny, deny + 1):
+                    for dzz in range(-deny, deny + 1):
+                        m = (n[0] + dxx, n[1] + dyy, n[2] + dzz)
+                        if all(0 <= m[a] < dims[a] for a in range(3)) and boundary[m]:
+                            keep = False
+            if not keep:
+                break
+        if keep:
+            ids.append(first_id + k)
+            cells.append(v)
+    return ids, cells

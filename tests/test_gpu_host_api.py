@@ -1,0 +1,91 @@
+"""The hemo::HemoCell host layer (hemocell_amd/libhemocell.so) on the GPU.
+
+* load_particles: cells read from .pos files and placed through the host API
+  land on the device exactly where io/readPositionsBloodCells.cpp puts them
+  (bounding-box centring, X-Y-Z rotation by the negated angles, micrometre
+  positions), cells touching a wall or its deny layer are dropped, and cell
+  ids run over all types in registration order (integer data: bit-exact).
+* stretch_cell: the reference's own validation, an RBC pulled by 25, 75 and
+  125 pN on 7 vertices per end for 10000 steps in a 52x26x26 box with
+  zero-velocity walls; the axial and transverse diameters must fall in the
+  bands of tests/validation/stretch_cell/test_stretch_cell.cpp:162-165 and the
+  volume stay within 2% (:145-146).
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+import hemocell_amd as hc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "hemocell_amd", "bin")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _run(args, cwd, timeout=300):
+    print("[gpu-host]", " ".join(args), flush=True)
+    r = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _read_out(path):
+    with open(path, "rb") as f:
+        nc, nv = np.frombuffer(f.read(16), dtype=np.int64)
+        ids = np.frombuffer(f.read(4 * nc), dtype=np.int32)
+        types = np.frombuffer(f.read(4 * nc), dtype=np.int32)
+        x = np.frombuffer(f.read(24 * nv), dtype=np.float64).reshape(nv, 3)
+    return ids, types, x
+
+
+def test_load_particles_places_cells_like_the_reference(tmp_path):
+    u = cases.units()
+    nx, ny, nz = 48, 30, 40
+    (tmp_path / "config.xml").write_text(cases.config_xml())
+    (tmp_path / "RBC.xml").write_text(cases.material_xml(cases.RBC_MAT, "RBC"))
+    (tmp_path / "PLT.xml").write_text(cases.material_xml(cases.PLT_MAT, "PLT", inner_edges=21))
+    rng = np.random.default_rng(21)
+    rbc = [((6.0, 7.5, 6.0), (90, 0, 0)), ((15.3, 7.4, 11.1), tuple(rng.uniform(0, 360, 3))),
+           ((12.0, 1.2, 10.0), (0, 0, 0)),  # reaches the y = 0 wall: dropped
+           ((7.0, 8.0, 15.0), tuple(rng.uniform(0, 360, 3)))]
+    plt = [((3.0, 12.0, 3.0), tuple(rng.uniform(0, 360, 3))), ((20.0, 1.4, 4.0), (10, 20, 30)),  # deny layer: dropped
+           ((19.5, 7.0, 17.5), (0, 45, 0)), ((2.5, 7.0, 18.0), tuple(rng.uniform(0, 360, 3)))]
+    for name, lines in (("RBC", rbc), ("PLT", plt)):
+        with open(tmp_path / (name + ".pos"), "w") as f:
+            f.write("%d\n" % len(lines))
+            for p, a in lines:
+                f.write(" ".join("%.17g" % float(c) for c in (*p, *a)) + "\n")
+    out = tmp_path / "cells.bin"
+    res = _run([os.path.join(BIN, "load_particles"), "config.xml", str(nx), str(ny), str(nz), "bb-y", str(out),
+                "RBC:RBC", "PLT:PLT@1.0"], cwd=tmp_path)
+    ids, types, x = _read_out(out)
+    assert res["listed"] == len(rbc) + len(plt)
+    boundary = np.zeros((nx, ny, nz), dtype=bool)
+    boundary[:, 0, :] = boundary[:, ny - 1, :] = True
+    mr = hc.build_mesh(1, cases.RBC_MAT["radius"] / u.dx, cases.RBC_MAT["minNumTriangles"])[0]
+    mp = hc.build_mesh(6, cases.PLT_MAT["radius"] / u.dx, cases.PLT_MAT["minNumTriangles"],
+                       cases.PLT_MAT["aspectRatio"])[0]
+    ids_r, cells_r = cases.pos_placement(mr, rbc, u.dx, boundary, 0.0, 0)
+    ids_p, cells_p = cases.pos_placement(mp, plt, u.dx, boundary, 1.0, len(rbc))
+    assert ids_r == [0, 1, 3] and ids_p == [4, 6, 7]
+    assert list(ids) == ids_r + ids_p
+    assert list(types) == [0] * len(ids_r) + [1] * len(ids_p)
+    assert np.array_equal(x, np.concatenate(cells_r + cells_p))
+
+
+@pytest.mark.parametrize("force,transverse,axial", [(25, (7.3, 7.9), (9.2, 9.7)), (75, (7.0, 7.5), (11.0, 12.0)),
+                                                    (125, (6.5, 7.0), (12.25, 12.75))])
+def test_stretch_cell_reference_bands(force, transverse, axial):
+    res = _run([os.path.join(BIN, "stretch_cell"), os.path.join(GOLDEN, "stretch_cell"), str(force)], cwd=ROOT)
+    print(res)
+    assert res["iterations"] == 10000 and res["cells"] == 1
+    assert res["finite"] == 1 and res["wall_hits"] == 0
+    assert transverse[0] <= res["transverse_um"] <= transverse[1]
+    assert axial[0] <= res["axial_um"] <= axial[1]
+    assert 0.98 < res["volume_ratio"] <= 1.02

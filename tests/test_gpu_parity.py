@@ -302,3 +302,83 @@ def test_fused_and_unfused_steps_agree(monkeypatch):
     _, ub = sims[1].fluid()
     assert cases.rel_l2(ua, ub) < 1e-10
     assert np.array_equal(sims[0].force_field(), np.zeros_like(ub.T))
+
+
+# ------------------------------------------------------ binning and repulsion --
+def _crowded_pair(shape, periodic, seed, n_plt, n_rbc, precision=hc.HC_FP64):
+    rng = np.random.default_rng(seed)
+    u = cases.units()
+    g, o = pair(*shape, tau=u.tau, periodic=periodic, precision=precision)
+    for kind, n in (("PLT", n_plt), ("RBC", n_rbc)):
+        if n:
+            model, v, t, p, inner, xs = cases.crowd(kind, n, shape, rng, u, hc.build_mesh)
+            for s in (g, o):
+                tid = s.add_cell_type(model, v, t, p, inner)
+                s.add_cells(tid, xs)
+    return g, o, u
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (1, 0, 1)])
+def test_vertex_bins_bit_exact(periodic):
+    g, o, _ = _crowded_pair((14, 12, 16), periodic, seed=11, n_plt=40, n_rbc=2)
+    bg, bo = g.vertex_bins(), o.vertex_bins()
+    assert np.array_equal(bg, bo)
+    assert (bo >= 0).any() and (periodic == (1, 1, 1)) == bool((bo >= 0).all())
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 1)])
+def test_repulsion_stage_fp64(periodic):
+    g, o, _ = _crowded_pair((12, 10, 13), periodic, seed=12, n_plt=40, n_rbc=1)
+    both(g, o, "set_repulsion", 2.5e-3, 0.8, 1)
+    g.stage("repulsion")
+    o.stage("repulsion")
+    fg, fo = g.repulsion_force(), o.repulsion_force()
+    assert np.abs(fo).max() > 0
+    assert cases.rel_l2(fg, fo) < 1e-14
+    # a second call after moving the cells reuses the (cleared) bins
+    xg = g.vertices()[0] + 0.37
+    both(g, o, "set_vertices", xg, None, None)
+    g.stage("repulsion")
+    o.stage("repulsion")
+    assert cases.rel_l2(g.repulsion_force(), o.repulsion_force()) < 1e-14
+
+
+def test_full_step_with_repulsion_fp64():
+    g, o, u = _crowded_pair((18, 16, 20), (1, 1, 1), seed=13, n_plt=30, n_rbc=1)
+    both(g, o, "set_body_force", [0, 0, 1e-6])
+    both(g, o, "set_force_limit", u.f_limit)
+    both(g, o, "set_repulsion", 2e-5, 0.9, 2)
+    for _ in range(3):
+        both(g, o, "step", 5)
+    assert np.abs(o.repulsion_force()).max() > 0
+    xg, vg, _ = g.vertices()
+    xo, vo, _ = o.vertices()
+    assert cases.rel_l2(xg, xo) < 1e-10
+    assert cases.rel_l2(g.repulsion_force(), o.repulsion_force()) < 1e-9
+
+
+# ---------------------------------------------------------- cell statistics --
+def test_cell_stats_match_reference_formulas():
+    u = cases.units()
+    g = hc.Simulation(0)
+    g.lattice_init(40, 30, 36, u.tau)
+    _mixed([g])
+    g.step(20)
+    x, v, _ = g.vertices()
+    st = g.cell_stats()
+    base, c = 0, 0
+    # cells in the order _mixed adds them
+    for kind, count in (("RBC", 2), ("PLT", 2), ("WBC", 1)):
+        _, mv, mt, _, _ = cases.mesh(kind, u, hc.build_mesh)
+        for _ in range(count):
+            sl = slice(base, base + len(mv))
+            ref = cases.cell_stats_reference(x[sl], v[sl], mt)
+            assert abs(st["volume"][c] - ref["volume"]) <= 1e-12 * abs(ref["volume"])
+            assert abs(st["area"][c] - ref["area"]) <= 1e-12 * ref["area"]
+            assert np.allclose(st["centroid"][c], ref["centroid"], rtol=1e-13, atol=0)
+            assert np.allclose(st["velocity"][c], ref["velocity"], rtol=1e-10, atol=1e-20)
+            assert np.array_equal(st["bbox"][c], ref["bbox"])
+            assert abs(st["stretch"][c] - ref["stretch"]) <= 1e-15 * ref["stretch"]
+            base += len(mv)
+            c += 1
+    assert c == len(st["volume"])
