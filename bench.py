@@ -109,6 +109,24 @@ def run_gpu(hc, susp, shape, precision, steps, warmup):
     return res
 
 
+def serve_stream(n, rate):
+    """p50/p99 latency of a seeded Poisson stream of single-cell cases (shear 10-1000 1/s
+    or stretch 0-200 pN on one RBC, PLT or WBC; cubic edge 32-96; 2000-20000 steps;
+    FP32-shifted), each case in its own context and stream."""
+    import hemocell_amd as hc
+    from hemocell_amd import serving
+    units = hc.lbm_base_parameters(5e-7, 1e-7)
+    stream = serving.make_cases(n, rate, seed=7)
+    print("[bench] serving %d cases at %.1f/s" % (n, rate), file=sys.stderr, flush=True)
+    done, wall = serving.serve(stream, units, max_active=16, chunk=1000,
+                               progress=lambda c, k: print("[bench] case %d done (%d/%d) %.2fs" % (
+                                   c.ident, k, n, c.latency), file=sys.stderr, flush=True) if k % 8 == 0 else None)
+    st = serving.latency_stats(done, wall)
+    st.update(arrival_rate=rate, max_active=16, workload="Poisson seed 7, shear or stretch, one RBC/PLT/WBC, "
+                                                        "edge 32-96, 2000-20000 steps, FP32-shifted")
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
+
+
 def whole_step(r):
     """Algorithmic HBM bytes of one step over the step time: collide-stream (176/352 B per
     node), velocity field (22 reads + 4 writes per node) and the vertex state of the cell
@@ -161,6 +179,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp32", "fp64"])
     ap.add_argument("--shape", default="256,256,512")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", type=int, default=32, metavar="N",
+                    help="cases of the serving stream (0 skips the latency measurement)")
+    ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate, cases/s")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a device-to-device copy of MIB MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -220,6 +241,8 @@ def main():
                         "collide_frac": round(f64["collide_gbs"] / HBM_PEAK_GBS, 4),
                         "stage_ms": {k: round(v, 4) for k, v in f64["stage_ms"].items()}}
         line["fp32_over_fp64"] = round(res["fp32"]["mlups"] / f64["mlups"], 3)
+    if args.serve > 0:
+        line["serving"] = serve_stream(args.serve, args.serve_rate)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(hc, susp)
     print(json.dumps(line), flush=True)

@@ -84,6 +84,7 @@ SIGNATURES = {
     "hc_stage_advance": (_ST, [_P]),
     "hc_stage_mechanics": (_ST, [_P, _INT]),
     "hc_stage_repulsion": (_ST, [_P]),
+    "hc_busy": (_INT, [_P]),
     "hc_stage_end": (_ST, [_P]),
     "hc_num_nodes": (ctypes.c_int64, [_P]),
     "hc_num_vertices": (ctypes.c_int64, [_P]),
@@ -251,6 +252,12 @@ class Simulation:
 
     def synchronize(self):
         self._chk(self.lib.hc_synchronize(self.h))
+
+    def busy(self):
+        r = self.lib.hc_busy(self.h)
+        if r < 0:
+            raise HcError(HC_DEVICE, self.lib.hc_last_error(self.h).decode())
+        return r == 1
 
     def stage(self, name, *args):
         self._chk(getattr(self.lib, "hc_stage_" + name)(self.h, *args))

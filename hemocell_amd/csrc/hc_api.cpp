@@ -74,6 +74,7 @@ struct hc_ctx {
   int lds_limit = 65536;        // dynamic LDS per workgroup available on the device
   bool spread_per_cell = true;  // LDS-staged spread (false: one thread per vertex, global atomics)
   int spread_lds = 0;           // LDS box budget in bytes (0: default per precision)
+  int cell_block = 256;         // threads per RBC / WBC cell in the fused vertex kernel
   // extra (stretch) forces
   int64_t* d_extra_ids = nullptr;
   double* d_extra_f = nullptr;
@@ -84,6 +85,12 @@ struct hc_ctx {
   // fused step: the vertex kernels of step t also spread the forces of step t+1
   bool fuse = true;
   bool spread_ready = false;  // Fibm[cur] already holds this step's spread
+  // steady fused steps are replayed from a captured graph of two steps (the layout
+  // parity and the force buffers alternate, so two steps bring every argument back)
+  bool use_graphs = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> graph_sig;  // launch arguments the graph was captured with
   // controls
   int64_t iter = 0;
   int pv_timescale = 1;
@@ -339,7 +346,7 @@ hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
     HC_TRY(launch_cell_step<R>(ctx->stream, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base,
                                (const R*)ctx->ufield, ctx->type, ctx->L, (R*)ctx->Fibm[1 - ctx->cur],
                                ctx->d_extra_dense, ctx->bend, ctx->f_limit, flags, box, ctx->lds_limit - 4096,
-                               ctx->d_wall_hits),
+                               ctx->d_wall_hits, ctx->cell_block),
            "cell_step");
   }
   prof_end(ctx, HC_PROF_MECH, ev);
@@ -427,35 +434,160 @@ hc_status do_end(hc_ctx* ctx) {
 }
 
 template <typename R>
+hc_status one_step(hc_ctx* ctx) {
+  hc_status s;
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_STEP, &ev);
+  const bool fused = ctx->fuse && !ctx->repulsion && ctx->nv > 0;
+  if (ctx->repulsion && ctx->nv > 0 && ctx->iter % ctx->rep_timescale == 0 && (s = do_repulsion(ctx)) != HC_OK)
+    return s;
+  if (ctx->nv > 0 && !ctx->spread_ready && (s = do_spread<R>(ctx)) != HC_OK) return s;
+  if ((s = do_collide<R>(ctx)) != HC_OK) return s;
+  const int do_interp = ctx->iter % ctx->pv_timescale == 0;
+  if (fused) {
+    if (do_interp) {
+      hipEvent_t e2;
+      prof_begin(ctx, HC_PROF_INTERP, &e2);
+      HC_TRY(launch_velocity_field<R>(ctx->stream, (const R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], (R*)ctx->ufield,
+                                      ctx->L, ctx->phase, ctx->general, ctx->g),
+             "velocity_field");
+      prof_end(ctx, HC_PROF_INTERP, e2);
+    }
+    if ((s = do_cell_step<R>(ctx, do_interp)) != HC_OK) return s;
+  } else {
+    if ((s = do_interp_advance<R>(ctx, do_interp, 1)) != HC_OK) return s;
+    if ((s = do_mechanics(ctx, false)) != HC_OK) return s;
+  }
+  if ((s = do_end<R>(ctx)) != HC_OK) return s;
+  ctx->spread_ready = fused;
+  prof_end(ctx, HC_PROF_STEP, ev);
+  if (ctx->prof.on && ctx->prof.used > 4096) prof_flush(ctx);
+  return HC_OK;
+}
+
+template <typename V>
+void put(std::vector<unsigned char>& sig, const V& x) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&x);
+  sig.insert(sig.end(), p, p + sizeof(V));
+}
+
+// Everything a fused step passes to its kernels: equal signatures mean equal launches.
+std::vector<unsigned char> launch_signature(const hc_ctx* c) {
+  std::vector<unsigned char> v;
+  put(v, c->phase);
+  put(v, c->cur);
+  put(v, c->other_clean);
+  put(v, c->spread_ready);
+  put(v, c->precision);
+  put(v, c->tau);
+  put(v, c->g);
+  put(v, c->wall_u);
+  put(v, c->f_limit);
+  put(v, c->L);
+  put(v, c->general);
+  put(v, c->f);
+  put(v, c->Fibm);
+  put(v, c->ufield);
+  put(v, c->type);
+  put(v, c->vel_nodes);
+  put(v, c->n_vel);
+  put(v, c->x);
+  put(v, c->v);
+  put(v, c->fv);
+  put(v, c->frep);
+  put(v, c->bend);
+  put(v, c->nv);
+  put(v, c->vcap);
+  put(v, c->d_cell_base);
+  put(v, c->d_extra_dense);
+  put(v, c->d_wall_hits);
+  put(v, c->lds_limit);
+  put(v, c->spread_lds);
+  put(v, c->cell_block);
+  put(v, c->pv_timescale);
+  for (const auto& tp : c->types) {
+    put(v, tp->dev);
+    put(v, tp->d_cells);
+    put(v, tp->ncells);
+    put(v, tp->host.timescale);
+  }
+  return v;
+}
+
+// Two steps repeat exactly when the step is fused and every timescale is 1.
+bool graph_eligible(const hc_ctx* c) {
+  if (!c->use_graphs || c->prof.on || !c->fuse || c->repulsion || c->nv == 0 || !c->spread_ready) return false;
+  if (c->pv_timescale != 1) return false;
+  for (const auto& tp : c->types)
+    if (tp->ncells && tp->host.timescale != 1) return false;
+  return true;
+}
+
+void drop_graph(hc_ctx* c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_sig.clear();
+}
+
+// Capture two steps into a graph.  The host-side state the capture advanced is
+// put back: the captured work has not run yet.
+template <typename R>
+hc_status capture_pair(hc_ctx* ctx, std::vector<unsigned char> sig) {
+  drop_graph(ctx);
+  const int64_t iter0 = ctx->iter;
+  const int phase0 = ctx->phase, cur0 = ctx->cur;
+  const bool clean0 = ctx->other_clean, ready0 = ctx->spread_ready;
+  HC_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  hc_status s = one_step<R>(ctx);
+  if (s == HC_OK) s = one_step<R>(ctx);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  const bool periodic = ctx->phase == phase0 && ctx->cur == cur0 && ctx->other_clean == clean0 &&
+                        ctx->spread_ready == ready0;
+  ctx->iter = iter0;
+  ctx->phase = phase0;
+  ctx->cur = cur0;
+  ctx->other_clean = clean0;
+  ctx->spread_ready = ready0;
+  if (s != HC_OK || e != hipSuccess || !periodic) {
+    if (g) hipGraphDestroy(g);
+    if (s != HC_OK) return s;
+    if (e != hipSuccess) return hipfail(ctx, e, "hipStreamEndCapture");
+    ctx->use_graphs = false;  // not a two-step cycle: stay eager
+    return HC_OK;
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    hipGraphDestroy(g);
+    return hipfail(ctx, ie, "hipGraphInstantiate");
+  }
+  ctx->graph = g;
+  ctx->gexec = x;
+  ctx->graph_sig = std::move(sig);
+  return HC_OK;
+}
+
+template <typename R>
 hc_status step_impl(hc_ctx* ctx, int nsteps) {
   hc_status s;
-  for (int k = 0; k < nsteps; ++k) {
-    hipEvent_t ev;
-    prof_begin(ctx, HC_PROF_STEP, &ev);
-    const bool fused = ctx->fuse && !ctx->repulsion && ctx->nv > 0;
-    if (ctx->repulsion && ctx->nv > 0 && ctx->iter % ctx->rep_timescale == 0 && (s = do_repulsion(ctx)) != HC_OK)
-      return s;
-    if (ctx->nv > 0 && !ctx->spread_ready && (s = do_spread<R>(ctx)) != HC_OK) return s;
-    if ((s = do_collide<R>(ctx)) != HC_OK) return s;
-    const int do_interp = ctx->iter % ctx->pv_timescale == 0;
-    if (fused) {
-      if (do_interp) {
-        hipEvent_t e2;
-        prof_begin(ctx, HC_PROF_INTERP, &e2);
-        HC_TRY(launch_velocity_field<R>(ctx->stream, (const R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], (R*)ctx->ufield,
-                                        ctx->L, ctx->phase, ctx->general, ctx->g),
-               "velocity_field");
-        prof_end(ctx, HC_PROF_INTERP, e2);
+  int k = 0;
+  while (k < nsteps) {
+    if (nsteps - k >= 2 && graph_eligible(ctx)) {
+      std::vector<unsigned char> sig = launch_signature(ctx);
+      if (!ctx->gexec || sig != ctx->graph_sig) {
+        if ((s = capture_pair<R>(ctx, std::move(sig))) != HC_OK) return s;
+        if (!ctx->gexec) continue;
       }
-      if ((s = do_cell_step<R>(ctx, do_interp)) != HC_OK) return s;
-    } else {
-      if ((s = do_interp_advance<R>(ctx, do_interp, 1)) != HC_OK) return s;
-      if ((s = do_mechanics(ctx, false)) != HC_OK) return s;
+      HC_TRY(hipGraphLaunch(ctx->gexec, ctx->stream), "hipGraphLaunch");
+      ctx->iter += 2;
+      k += 2;
+      continue;
     }
-    if ((s = do_end<R>(ctx)) != HC_OK) return s;
-    ctx->spread_ready = fused;
-    prof_end(ctx, HC_PROF_STEP, ev);
-    if (ctx->prof.on && ctx->prof.used > 4096) prof_flush(ctx);
+    if ((s = one_step<R>(ctx)) != HC_OK) return s;
+    ++k;
   }
   return HC_OK;
 }
@@ -499,6 +631,10 @@ hc_status hc_create(int device, hc_ctx** out) {
   if (sp && std::string(sp) == "vertex") ctx->spread_per_cell = false;
   const char* fu = std::getenv("HC_FUSE");  // tuning: 0 runs the unfused vertex kernels
   if (fu && std::string(fu) == "0") ctx->fuse = false;
+  const char* cb = std::getenv("HC_CELL_BLOCK");  // tuning: 256, 512 or 1024 threads per cell
+  if (cb) ctx->cell_block = std::atoi(cb);
+  const char* gr = std::getenv("HC_GRAPH");  // 0: launch every step eagerly
+  if (gr && std::string(gr) == "0") ctx->use_graphs = false;
   const char* sl = std::getenv("HC_SPREAD_LDS");  // tuning: LDS bytes of the per-cell box
   if (sl) ctx->spread_lds = std::atoi(sl);
   *out = ctx;
@@ -517,6 +653,7 @@ void hc_destroy(hc_ctx* ctx) {
     for (void* p : tp->allocs) hipFree(p);
     if (tp->d_cells) hipFree(tp->d_cells);
   }
+  drop_graph(ctx);
   free_repulsion(ctx);
   if (ctx->rb.bins) hipFree(ctx->rb.bins);
   for (hipEvent_t e : ctx->prof.pool) hipEventDestroy(e);
@@ -785,6 +922,15 @@ hc_status hc_step(hc_ctx* ctx, int nsteps) {
   if (s != HC_OK) return s;
   if (nsteps < 0) return fail(ctx, HC_INVALID_ARG, "nsteps must be >= 0");
   return fp32(ctx) ? step_impl<float>(ctx, nsteps) : step_impl<double>(ctx, nsteps);
+}
+
+int hc_busy(hc_ctx* ctx) {
+  if (!ctx) return -1;
+  const hipError_t e = hipStreamQuery(ctx->stream);
+  if (e == hipSuccess) return 0;
+  if (e == hipErrorNotReady) return 1;
+  ctx->err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+  return -1;
 }
 
 hc_status hc_synchronize(hc_ctx* ctx) {

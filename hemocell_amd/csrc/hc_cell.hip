@@ -281,7 +281,7 @@ template <typename R>
 hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                             const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L,
                             R* Fnext, const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
-                            int lds_limit, unsigned long long* wall_hits) {
+                            int lds_limit, unsigned long long* wall_hits, int block) {
   if (ncells <= 0) return hipSuccess;
   const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * sizeof(double) : 0;
   // staged: positions, velocities and a region shared by bending forces and the box
@@ -304,15 +304,28 @@ hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T,
   else                                                                                                       \
     hipLaunchKernelGGL((k_cell_step<R, M, B, false>), g, dim3(B), dyn, s, V, T, cells, cell_base, uf, type, L, \
                        Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
+  // threads per cell for the RBC and WBC meshes: 256, 512 or 1024
   switch (T.model) {
     case HC_MODEL_RBC:
-      HC_CELL(HC_MODEL_RBC, 256)
+      if (block >= 1024) {
+        HC_CELL(HC_MODEL_RBC, 1024)
+      } else if (block >= 512) {
+        HC_CELL(HC_MODEL_RBC, 512)
+      } else {
+        HC_CELL(HC_MODEL_RBC, 256)
+      }
       break;
     case HC_MODEL_PLT:
       HC_CELL(HC_MODEL_PLT, 64)
       break;
     case HC_MODEL_WBC:
-      HC_CELL(HC_MODEL_WBC, 256)
+      if (block >= 1024) {
+        HC_CELL(HC_MODEL_WBC, 1024)
+      } else if (block >= 512) {
+        HC_CELL(HC_MODEL_WBC, 512)
+      } else {
+        HC_CELL(HC_MODEL_WBC, 256)
+      }
       break;
     default:
       return hipErrorInvalidValue;
@@ -323,9 +336,9 @@ hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T,
 
 template hipError_t launch_cell_step<float>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
                                             const int64_t*, const float*, const uint8_t*, const LatticeDev&, float*,
-                                            const double*, double*, double, int, int, int, unsigned long long*);
+                                            const double*, double*, double, int, int, int, unsigned long long*, int);
 template hipError_t launch_cell_step<double>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
                                              const int64_t*, const double*, const uint8_t*, const LatticeDev&, double*,
-                                             const double*, double*, double, int, int, int, unsigned long long*);
+                                             const double*, double*, double, int, int, int, unsigned long long*, int);
 
 }  // namespace hcg

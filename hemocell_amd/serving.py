@@ -81,8 +81,8 @@ def build(case, units, precision=hc.HC_FP32_SHIFTED, device=0):
     mask = np.zeros((E, E, E), dtype=np.uint8)
     if case.kind == "shear":
         # velocity walls z = 0 and z = E-1, periodic in x and y
-        mask[:, :, 0] = hc.HC_NODE_VELOCITY + 4
-        mask[:, :, E - 1] = hc.HC_NODE_VELOCITY + 5
+        mask[:, :, 0] = hc.NODE_VELOCITY + 4
+        mask[:, :, E - 1] = hc.NODE_VELOCITY + 5
         sim.lattice_init(E, E, E, units.tau, periodic=(1, 1, 0), node_type=mask.reshape(-1), precision=precision)
         v_half = (E - 1) * case.shear_rate * units.dt * 0.5
         sim.set_wall_velocity(4, (v_half, 0.0, 0.0))
@@ -94,7 +94,7 @@ def build(case, units, precision=hc.HC_FP32_SHIFTED, device=0):
                 sl = [slice(None)] * 3
                 sl[a] = side
                 face_mask = mask[tuple(sl)]
-                face_mask[face_mask == 0] = hc.HC_NODE_VELOCITY + face
+                face_mask[face_mask == 0] = hc.NODE_VELOCITY + face
         sim.lattice_init(E, E, E, units.tau, periodic=(0, 0, 0), node_type=mask.reshape(-1), precision=precision)
     sim.init_equilibrium(1.0, (0.0, 0.0, 0.0))
     sim.set_force_limit(units.f_limit)

@@ -148,6 +148,10 @@ hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* vertex_id
  * on the context stream; errors of the launches are reported. */
 hc_status hc_step(hc_ctx* ctx, int nsteps);
 hc_status hc_synchronize(hc_ctx* ctx);
+/* Non-blocking: 1 while work queued on the context is still running, 0 when idle, -1 on
+ * error.  No counterpart in the reference (its iterate() is synchronous); used by hosts
+ * that drive several contexts from one thread. */
+int hc_busy(hc_ctx* ctx);
 int64_t hc_iteration(const hc_ctx* ctx);
 
 /* Single stages of iterate(), in its order (parity tests and partial drivers). */

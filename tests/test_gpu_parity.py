@@ -382,3 +382,36 @@ def test_cell_stats_match_reference_formulas():
             base += len(mv)
             c += 1
     assert c == len(st["volume"])
+
+
+# ------------------------------------------------------------- step graphs --
+def test_graph_replays_match_eager_steps(monkeypatch):
+    """Steady fused steps replayed from a captured two-step graph against the eager
+    launches, replay after replay, with host reads and a changed body force (which
+    forces a new capture) in between.  The spread's float atomics commit in no fixed
+    order, so two eager runs already differ at round-off: the graphed run must stay
+    within the fused-step tolerances of the eager one, and as close to it as a second
+    eager run is, up to a factor."""
+    u = cases.units()
+    sims = []
+    for graph in ("0", "0", "1"):
+        monkeypatch.setenv("HC_GRAPH", graph)
+        g = hc.Simulation(0)
+        g.lattice_init(40, 30, 36, u.tau)
+        _mixed([g])
+        g.set_extra_vertex_force(np.array([3, 500], dtype=np.int64), np.array([[1e-4, 0, 0], [-1e-4, 0, 0]]))
+        sims.append(g)
+    eager, eager2, graphed = sims
+    for rnd, n in enumerate((1, 6, 6, 7, 4, 9)):
+        if rnd == 3:
+            for g in sims:
+                g.set_body_force([0, 3e-6, 1e-6])
+        for g in sims:
+            g.step(n)
+        (xa, va, _), (x2, v2, _), (xb, vb, _) = eager.vertices(), eager2.vertices(), graphed.vertices()
+        assert cases.rel_l2(xa, xb) < 1e-12, rnd
+        assert cases.rel_l2(va, vb) < 1e-8, rnd
+        assert cases.rel_l2(va, vb) <= 100 * max(cases.rel_l2(va, v2), 1e-14), rnd
+        fa, fb = eager.populations(), graphed.populations()
+        assert cases.rel_l2(fa, fb) < 1e-10, rnd
+    assert eager.iteration == graphed.iteration == 33
