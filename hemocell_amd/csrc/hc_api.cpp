@@ -44,6 +44,10 @@ struct Prof {
 struct hc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the cell kernels of different types run side by side: two more streams that
+  // fork from and join back into the context stream within each step
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
   std::string err;
   // lattice
   bool lattice_ready = false;
@@ -72,9 +76,11 @@ struct hc_ctx {
   int32_t* d_cell_nv = nullptr;
   bool cells_dirty = true;
   int lds_limit = 65536;        // dynamic LDS per workgroup available on the device
+  int lds_device = 65536;       // LDS per workgroup of the device (static + dynamic)
   bool spread_per_cell = true;  // LDS-staged spread (false: one thread per vertex, global atomics)
   int spread_lds = 0;           // LDS box budget in bytes (0: default per precision)
-  int cell_block = 256;         // threads per RBC / WBC cell in the fused vertex kernel
+  int cell_block = 256;         // threads per RBC cell in the fused vertex kernel
+  int wbc_block = 1024;         // threads per WBC cell (falls back when the LDS does not fit)
   // extra (stretch) forces
   int64_t* d_extra_ids = nullptr;
   double* d_extra_f = nullptr;
@@ -244,6 +250,12 @@ hc_status sync_cells(hc_ctx* ctx) {
   return HC_OK;
 }
 
+// tuning: HC_MECH_TABLES=csr keeps the mechanics on the variable-length lists
+bool padded_off(const hc_ctx*) {
+  const char* e = std::getenv("HC_MECH_TABLES");
+  return e && std::string(e) == "csr";
+}
+
 template <typename T>
 hc_status upload(hc_ctx* ctx, TypeBufs& tb, const std::vector<T>& h, const T** out) {
   void* p = nullptr;
@@ -334,20 +346,36 @@ template <typename R>
 hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_MECH, &ev);
-  for (auto& tp : ctx->types) {
-    TypeBufs& tb = *tp;
-    if (tb.ncells == 0) continue;
+  // the type with the most vertices runs on the context stream, the others beside it
+  std::vector<TypeBufs*> act;
+  for (auto& tp : ctx->types)
+    if (tp->ncells) act.push_back(tp.get());
+  std::stable_sort(act.begin(), act.end(), [](const TypeBufs* a, const TypeBufs* b) {
+    return (int64_t)a->ncells * a->host.nv > (int64_t)b->ncells * b->host.nv;
+  });
+  const int nside = std::min<int>((int)act.size() - 1, ctx->side[0] ? 2 : 0);
+  if (nside > 0) {
+    HC_TRY(hipEventRecord(ctx->fork_ev, ctx->stream), "hipEventRecord");
+    for (int j = 0; j < nside; ++j) HC_TRY(hipStreamWaitEvent(ctx->side[j], ctx->fork_ev, 0), "hipStreamWaitEvent");
+  }
+  for (size_t k = 0; k < act.size(); ++k) {
+    TypeBufs& tb = *act[k];
+    hipStream_t st = (k == 0 || nside == 0) ? ctx->stream : ctx->side[(k - 1) % nside];
     int flags = CELL_FLAG_SPREAD;
     if (do_interp) flags |= CELL_FLAG_INTERP;
     if ((ctx->iter % tb.host.timescale) == 0) flags |= CELL_FLAG_MECH;
     if (ctx->repulsion) flags |= CELL_FLAG_REPULSION;
-    int box = ctx->spread_lds > 0 ? ctx->spread_lds : (sizeof(R) == 4 ? 32768 : 49152);
-    if (tb.host.nv > 1000) box = std::max(box, 98304);
-    HC_TRY(launch_cell_step<R>(ctx->stream, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base,
-                               (const R*)ctx->ufield, ctx->type, ctx->L, (R*)ctx->Fibm[1 - ctx->cur],
-                               ctx->d_extra_dense, ctx->bend, ctx->f_limit, flags, box, ctx->lds_limit - 4096,
-                               ctx->d_wall_hits, ctx->cell_block),
+    // spread box in LDS; larger supports are spread in slabs of x-planes
+    const int box = ctx->spread_lds > 0 ? ctx->spread_lds : 16384;
+    const int block = tb.host.model == HC_MODEL_WBC ? ctx->wbc_block : ctx->cell_block;
+    HC_TRY(launch_cell_step<R>(st, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base, (const R*)ctx->ufield,
+                               ctx->type, ctx->L, (R*)ctx->Fibm[1 - ctx->cur], ctx->d_extra_dense, ctx->bend,
+                               ctx->f_limit, flags, box, ctx->lds_device, ctx->d_wall_hits, block),
            "cell_step");
+  }
+  for (int j = 0; j < nside; ++j) {
+    HC_TRY(hipEventRecord(ctx->join_ev[j], ctx->side[j]), "hipEventRecord");
+    HC_TRY(hipStreamWaitEvent(ctx->stream, ctx->join_ev[j], 0), "hipStreamWaitEvent");
   }
   prof_end(ctx, HC_PROF_MECH, ev);
   return HC_OK;
@@ -502,8 +530,10 @@ std::vector<unsigned char> launch_signature(const hc_ctx* c) {
   put(v, c->d_extra_dense);
   put(v, c->d_wall_hits);
   put(v, c->lds_limit);
+  put(v, c->lds_device);
   put(v, c->spread_lds);
   put(v, c->cell_block);
+  put(v, c->wbc_block);
   put(v, c->pv_timescale);
   for (const auto& tp : c->types) {
     put(v, tp->dev);
@@ -617,6 +647,15 @@ hc_status hc_create(int device, hc_ctx** out) {
     delete ctx;
     return HC_DEVICE;
   }
+  bool side_ok = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess;
+  for (int j = 0; j < 2; ++j)
+    side_ok = side_ok && hipStreamCreate(&ctx->side[j]) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->join_ev[j], hipEventDisableTiming) == hipSuccess;
+  if (!side_ok) {  // run every type on the context stream
+    for (int j = 0; j < 2; ++j)
+      if (ctx->side[j]) hipStreamDestroy(ctx->side[j]);
+    ctx->side[0] = ctx->side[1] = nullptr;
+  }
   if (hipMalloc((void**)&ctx->d_wall_hits, sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc((void**)&ctx->d_flag, sizeof(unsigned int)) != hipSuccess) {
     hipStreamDestroy(ctx->stream);
@@ -626,13 +665,18 @@ hc_status hc_create(int device, hc_ctx** out) {
   hipMemset(ctx->d_wall_hits, 0, sizeof(unsigned long long));
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+  {
     ctx->lds_limit = lds - 256;  // the kernel's own static LDS
+    ctx->lds_device = lds;
+  }
   const char* sp = std::getenv("HC_SPREAD");
   if (sp && std::string(sp) == "vertex") ctx->spread_per_cell = false;
   const char* fu = std::getenv("HC_FUSE");  // tuning: 0 runs the unfused vertex kernels
   if (fu && std::string(fu) == "0") ctx->fuse = false;
   const char* cb = std::getenv("HC_CELL_BLOCK");  // tuning: 256, 512 or 1024 threads per cell
   if (cb) ctx->cell_block = std::atoi(cb);
+  const char* wb = std::getenv("HC_WBC_BLOCK");  // tuning: threads per WBC cell
+  if (wb) ctx->wbc_block = std::atoi(wb);
   const char* gr = std::getenv("HC_GRAPH");  // 0: launch every step eagerly
   if (gr && std::string(gr) == "0") ctx->use_graphs = false;
   const char* sl = std::getenv("HC_SPREAD_LDS");  // tuning: LDS bytes of the per-cell box
@@ -657,6 +701,11 @@ void hc_destroy(hc_ctx* ctx) {
   free_repulsion(ctx);
   if (ctx->rb.bins) hipFree(ctx->rb.bins);
   for (hipEvent_t e : ctx->prof.pool) hipEventDestroy(e);
+  for (int j = 0; j < 2; ++j) {
+    if (ctx->side[j]) hipStreamDestroy(ctx->side[j]);
+    if (ctx->join_ev[j]) hipEventDestroy(ctx->join_ev[j]);
+  }
+  if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -804,6 +853,15 @@ hc_status hc_add_cell_type(hc_ctx* ctx, int model, int nv, int nt, const int32_t
     D.vtri4 = (const int4*)p4;
     if ((s = upload(ctx, *tb, T.vedge4, &p4)) != HC_OK) return s;
     D.vedge4 = (const int4*)p4;
+    D.vtri4p = D.vedge4p = nullptr;
+    D.vbendp = nullptr;
+    if (T.padded && !padded_off(ctx)) {
+      if ((s = upload(ctx, *tb, T.vtri4p, &p4)) != HC_OK) return s;
+      D.vtri4p = (const int4*)p4;
+      if ((s = upload(ctx, *tb, T.vedge4p, &p4)) != HC_OK) return s;
+      D.vedge4p = (const int4*)p4;
+      if ((s = upload(ctx, *tb, T.vbendp, &D.vbendp)) != HC_OK) return s;
+    }
   }
   D.model = model;
   D.nv = T.nv;

@@ -191,78 +191,105 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
   __syncthreads();
   const int o0 = bmin[0], o1 = bmin[1], o2 = bmin[2];
   const int s0 = bmax[0] - bmin[0] + 2, s1 = bmax[1] - bmin[1] + 2, s2 = bmax[2] - bmin[2] + 2;
-  const int64_t total = (int64_t)s0 * s1 * s2;
-  const bool direct = bad || total > box_nodes || s0 <= 0;
-  if (!direct)
-    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = R(0);
-  __syncthreads();
+  const int64_t plane = (int64_t)s1 * s2;
+  // the box is spread in slabs of whole x-planes when it does not fit at once;
+  // only a cell whose single x-plane does not fit uses global atomics per vertex
+  const bool direct = bad || s0 <= 0 || plane > box_nodes;
   const int64_t S = L.stride;
   const int per = (nv + 63) / 64;
-  for (int k = tid; k < 64 * per; k += BS) {
-    // lanes of one wavefront take vertices ~nv/64 apart (fewer LDS same-address adds)
-    const int vtx = (k & 63) * per + (k >> 6);
-    if (vtx >= nv) continue;
-    const int64_t i = base + vtx;
-    double f0 = V.f[i], f1 = V.f[vs + i], f2 = V.f[2 * vs + i];
-    bool store = false;
-    if (extra) {
-      f0 += extra[i];
-      f1 += extra[vs + i];
-      f2 += extra[2 * vs + i];
-      store = true;
-    }
-    const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
-    if (fm > f_limit) {
-      const double sc = f_limit / fm;
-      f0 *= sc;
-      f1 *= sc;
-      f2 *= sc;
-      store = true;
-    }
-    if (store) {
-      V.f[i] = f0;
-      V.f[vs + i] = f1;
-      V.f[2 * vs + i] = f2;
+  // the force each vertex spreads: its force plus any extra force, capped to
+  // f_limit and written back in place when that changed it (hemoCellParticleField.cpp:846-852)
+  auto spread_force = [&](int64_t i, double& f0, double& f1, double& f2, bool prepare) {
+    f0 = V.f[i];
+    f1 = V.f[vs + i];
+    f2 = V.f[2 * vs + i];
+    if (prepare) {
+      bool store = false;
+      if (extra) {
+        f0 += extra[i];
+        f1 += extra[vs + i];
+        f2 += extra[2 * vs + i];
+        store = true;
+      }
+      const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+      if (fm > f_limit) {
+        const double sc = f_limit / fm;
+        f0 *= sc;
+        f1 *= sc;
+        f2 *= sc;
+        store = true;
+      }
+      if (store) {
+        V.f[i] = f0;
+        V.f[vs + i] = f1;
+        V.f[2 * vs + i] = f2;
+      }
     }
     if (flags & CELL_REPULSION) {
       f0 = V.frep[i] + f0;
       f1 = V.frep[vs + i] + f1;
       f2 = V.frep[2 * vs + i] + f2;
     }
-    Support sp;
-    ibm_support(px[vtx], py[vtx], pz[vtx], L, type, sp);
+  };
+  if (direct) {
+    for (int k = tid; k < nv; k += BS) {
+      const int64_t i = base + k;
+      double f0, f1, f2;
+      spread_force(i, f0, f1, f2, true);
+      Support sp;
+      ibm_support(px[k], py[k], pz[k], L, type, sp);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      if (sp.w[q] == 0.0) continue;
-      if (direct) {
+      for (int q = 0; q < 8; ++q) {
+        if (sp.w[q] == 0.0) continue;
         const int64_t n = sp.idx[q];
         atomic_add(&Fnext[n], (R)(f0 * sp.w[q]));
         atomic_add(&Fnext[S + n], (R)(f1 * sp.w[q]));
         atomic_add(&Fnext[2 * S + n], (R)(f2 * sp.w[q]));
-      } else {
-        const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * (sp.lo[0] + (q >> 2) - o0));
+      }
+    }
+    return;
+  }
+  const int sx = (int)std::min<int64_t>(s0, box_nodes / plane);  // x-planes per slab
+  const int dims[3] = {L.nx, L.ny, L.nz};
+  for (int xs = 0; xs < s0; xs += sx) {
+    const int nxp = std::min(sx, s0 - xs);
+    const int64_t total = (int64_t)nxp * plane;
+    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = R(0);
+    __syncthreads();
+    for (int k = tid; k < 64 * per; k += BS) {
+      // lanes of one wavefront take vertices ~nv/64 apart (fewer LDS same-address adds)
+      const int vtx = (k & 63) * per + (k >> 6);
+      if (vtx >= nv) continue;
+      double f0, f1, f2;
+      spread_force(base + vtx, f0, f1, f2, xs == 0);
+      Support sp;
+      ibm_support(px[vtx], py[vtx], pz[vtx], L, type, sp);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int lx = sp.lo[0] + (q >> 2) - o0 - xs;
+        if (sp.w[q] == 0.0 || lx < 0 || lx >= nxp) continue;
+        const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * lx);
         lds_add(&box[e], (R)(f0 * sp.w[q]));
         lds_add(&box[total + e], (R)(f1 * sp.w[q]));
         lds_add(&box[2 * total + e], (R)(f2 * sp.w[q]));
       }
     }
-  }
-  if (direct) return;
-  __syncthreads();
-  const int dims[3] = {L.nx, L.ny, L.nz};
-  for (int64_t e = tid; e < total; e += BS) {
-    const R a = box[e], b = box[total + e], cz = box[2 * total + e];
-    if (a == R(0) && b == R(0) && cz == R(0)) continue;
-    const int lz = (int)(e % s2);
-    const int64_t r = e / s2;
-    const int ly = (int)(r % s1), lx = (int)(r / s1);
-    const int64_t g[3] = {(int64_t)o0 + lx, (int64_t)o1 + ly, (int64_t)o2 + lz};
-    int64_t wc[3];
-    for (int q = 0; q < 3; ++q) wc[q] = (g[q] >= 0 && g[q] < dims[q]) ? g[q] : wrap_mod(g[q], dims[q]);
-    const int64_t n = wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0]);
-    if (a != R(0)) atomic_add(&Fnext[n], a);
-    if (b != R(0)) atomic_add(&Fnext[S + n], b);
-    if (cz != R(0)) atomic_add(&Fnext[2 * S + n], cz);
+    __syncthreads();
+    for (int64_t e = tid; e < total; e += BS) {
+      const R a = box[e], b = box[total + e], cz = box[2 * total + e];
+      if (a == R(0) && b == R(0) && cz == R(0)) continue;
+      const int lz = (int)(e % s2);
+      const int64_t r = e / s2;
+      const int ly = (int)(r % s1), lx = (int)(r / s1);
+      const int64_t g[3] = {(int64_t)o0 + xs + lx, (int64_t)o1 + ly, (int64_t)o2 + lz};
+      int64_t wc[3];
+      for (int q = 0; q < 3; ++q) wc[q] = (g[q] >= 0 && g[q] < dims[q]) ? g[q] : wrap_mod(g[q], dims[q]);
+      const int64_t n = wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0]);
+      if (a != R(0)) atomic_add(&Fnext[n], a);
+      if (b != R(0)) atomic_add(&Fnext[S + n], b);
+      if (cz != R(0)) atomic_add(&Fnext[2 * S + n], cz);
+    }
+    __syncthreads();  // the box is zeroed again by the next slab
   }
 }
 
@@ -277,61 +304,78 @@ static size_t cell_lds_fixed(const TypeDev& T, bool staged) {
   return ((visc ? 6 : 3) * nv) * sizeof(double);
 }
 
+// Static LDS of one kernel instantiation (the reduction array grows with the block).
+template <typename K>
+static size_t static_lds(K kernel) {
+  hipFuncAttributes a{};
+  if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) != hipSuccess) return 32768;
+  return a.sharedSizeBytes;
+}
+
+// Lay out the dynamic LDS of one cell type for block size B and launch: staged
+// (positions, velocities and a region shared by bending forces and the box) when
+// it fits beside the kernel's static LDS, else positions plus as much box as fits.
+template <typename R, int M, int B>
+static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
+                              const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L, R* Fnext,
+                              const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
+                              int lds_device, unsigned long long* wall_hits) {
+  static const size_t st_staged = static_lds(k_cell_step<R, M, B, true>);
+  static const size_t st_plain = static_lds(k_cell_step<R, M, B, false>);
+  const size_t margin = 256, dev = (size_t)lds_device;
+  const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * sizeof(double) : 0;
+  bool staged = true;
+  size_t fixed = cell_lds_fixed(T, true);
+  size_t region = std::max(bend, (size_t)box_bytes);
+  if (fixed + region + st_staged + margin > dev) {
+    staged = false;
+    fixed = cell_lds_fixed(T, false);
+    const size_t used = fixed + st_plain + margin;
+    region = std::min((size_t)box_bytes, dev > used ? dev - used : 0);
+  }
+  if (fixed + region + (staged ? st_staged : st_plain) + margin > dev) return hipErrorInvalidConfiguration;
+  const int box_nodes = (int)(region / (3 * sizeof(R)));
+  const size_t dyn = fixed + region;
+  if (staged)
+    hipLaunchKernelGGL((k_cell_step<R, M, B, true>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells, cell_base,
+                       uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
+  else
+    hipLaunchKernelGGL((k_cell_step<R, M, B, false>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells, cell_base,
+                       uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                             const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L,
                             R* Fnext, const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
-                            int lds_limit, unsigned long long* wall_hits, int block) {
+                            int lds_device, unsigned long long* wall_hits, int block) {
   if (ncells <= 0) return hipSuccess;
-  const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * sizeof(double) : 0;
-  // staged: positions, velocities and a region shared by bending forces and the box
-  bool staged = true;
-  size_t fixed = cell_lds_fixed(T, true);
-  size_t region = std::max(bend, (size_t)box_bytes);
-  if (fixed + region > (size_t)lds_limit) {
-    staged = false;
-    fixed = cell_lds_fixed(T, false);
-    region = std::min((size_t)box_bytes, (size_t)lds_limit > fixed ? (size_t)lds_limit - fixed : 0);
-  }
-  if (fixed > (size_t)lds_limit) return hipErrorInvalidConfiguration;
-  const int box_nodes = (int)(region / (3 * sizeof(R)));
-  const size_t dyn = fixed + region;
-  dim3 g((unsigned)ncells);
-#define HC_CELL(M, B)                                                                                        \
-  if (staged)                                                                                                \
-    hipLaunchKernelGGL((k_cell_step<R, M, B, true>), g, dim3(B), dyn, s, V, T, cells, cell_base, uf, type, L,  \
-                       Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);                       \
-  else                                                                                                       \
-    hipLaunchKernelGGL((k_cell_step<R, M, B, false>), g, dim3(B), dyn, s, V, T, cells, cell_base, uf, type, L, \
-                       Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
-  // threads per cell for the RBC and WBC meshes: 256, 512 or 1024
+  // threads per cell for the RBC and WBC meshes: 256, 512 or 1024; a block whose
+  // static LDS leaves too little room for the cell falls back to a smaller one
+  hipError_t e = hipErrorInvalidConfiguration;
+#define HC_CELL(M, B)                                                                                                \
+  if (e == hipErrorInvalidConfiguration)                                                                             \
+    e = launch_cell<R, M, B>(s, V, T, cells, ncells, cell_base, uf, type, L, Fnext, extra, scratch, f_limit, flags, \
+                             box_bytes, lds_device, wall_hits);
   switch (T.model) {
     case HC_MODEL_RBC:
-      if (block >= 1024) {
-        HC_CELL(HC_MODEL_RBC, 1024)
-      } else if (block >= 512) {
-        HC_CELL(HC_MODEL_RBC, 512)
-      } else {
-        HC_CELL(HC_MODEL_RBC, 256)
-      }
-      break;
+      if (block >= 1024) HC_CELL(HC_MODEL_RBC, 1024)
+      if (block >= 512) HC_CELL(HC_MODEL_RBC, 512)
+      HC_CELL(HC_MODEL_RBC, 256)
+      return e;
     case HC_MODEL_PLT:
       HC_CELL(HC_MODEL_PLT, 64)
-      break;
+      return e;
     case HC_MODEL_WBC:
-      if (block >= 1024) {
-        HC_CELL(HC_MODEL_WBC, 1024)
-      } else if (block >= 512) {
-        HC_CELL(HC_MODEL_WBC, 512)
-      } else {
-        HC_CELL(HC_MODEL_WBC, 256)
-      }
-      break;
+      if (block >= 1024) HC_CELL(HC_MODEL_WBC, 1024)
+      if (block >= 512) HC_CELL(HC_MODEL_WBC, 512)
+      HC_CELL(HC_MODEL_WBC, 256)
+      return e;
     default:
       return hipErrorInvalidValue;
   }
 #undef HC_CELL
-  return hipGetLastError();
 }
 
 template hipError_t launch_cell_step<float>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,

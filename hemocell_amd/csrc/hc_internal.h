@@ -10,6 +10,11 @@
 
 namespace hcg {
 
+// Fixed-width per-vertex copies of the gather lists (entries past the end have a
+// negative first field): a vertex then issues all its table loads at once.
+constexpr int kPadW = 7;  // incident triangles / edge entries
+constexpr int kPadB = 8;  // bending contributors (the vertex and its 1-ring)
+
 // Rest-geometry tables of one cell type, in the layout the mechanics kernels read.
 // The first block mirrors CommonCellConstants (mechanics/commonCellConstants.cpp:70-409);
 // the *_off/_list pairs are per-vertex gather lists that visit every force
@@ -34,6 +39,8 @@ struct TypeHost {
   std::vector<int32_t> vinner_off, vinner;  // ie*2 + end (0 first, 1 second)
   std::vector<int32_t> vtri4;   // per vtri entry: triangle, its three corners
   std::vector<int32_t> vedge4;  // per vedge entry: e*4 + kind, the edge's two ends, 0
+  bool padded = false;          // every list fits kPadW / kPadB
+  std::vector<int32_t> vtri4p, vedge4p, vbendp;  // nv*kPadW int4, nv*kPadW int4, nv*kPadB int
   hc_model_params p{};
   int timescale = 1;
 };
@@ -49,6 +56,8 @@ struct TypeDev {
   const double *edge_len_eq, *edge_angle_eq, *tri_area_eq, *patch_dist_eq, *inner_len_eq;
   const int32_t *vtri_off, *vtri, *vbend_off, *vbend, *vedge_off, *vedge, *vinner_off, *vinner;
   const int4 *vtri4, *vedge4;
+  const int4 *vtri4p, *vedge4p;  // fixed-width copies (null when a list does not fit)
+  const int32_t* vbendp;
   double volume_eq, area_mean_eq, edge_mean_eq;
   double k_link, k_bend, k_volume, k_area, eta_m, k_inner_rigid, k_cyto, core_radius, radius;
 };

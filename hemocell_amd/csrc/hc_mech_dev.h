@@ -126,8 +126,7 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
   // (all area terms, then all volume terms) up to the grouping of that sum
   {
     double A0 = 0.0, A1 = 0.0, A2 = 0.0, B0 = 0.0, B1 = 0.0, B2 = 0.0;
-    for (int e = T.vtri_off[k]; e < T.vtri_off[k + 1]; ++e) {
-      const int4 q = T.vtri4[e];
+    auto tri_term = [&](const int4 q) {
       const int t = q.x;
       const D3 v0 = c.pos(q.y), v1 = c.pos(q.z), v2 = c.pos(q.w);
       D3 n = crossp(v1 - v0, v2 - v0);
@@ -152,6 +151,16 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
       B0 += (vforce * n.x) * sc;
       B1 += (vforce * n.y) * sc;
       B2 += (vforce * n.z) * sc;
+    };
+    if (T.vtri4p) {
+      int4 q[kPadW];
+#pragma unroll
+      for (int j = 0; j < kPadW; ++j) q[j] = T.vtri4p[kPadW * k + j];
+#pragma unroll
+      for (int j = 0; j < kPadW; ++j)
+        if (q[j].x >= 0) tri_term(q[j]);
+    } else {
+      for (int e = T.vtri_off[k]; e < T.vtri_off[k + 1]; ++e) tri_term(T.vtri4[e]);
     }
     F0 = A0 + B0;
     F1 = A1 + B1;
@@ -159,8 +168,7 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
   }
   // bending: own force and -F_i/n_i of every vertex i whose 1-ring holds k, ascending i
   if (MODEL != HC_MODEL_PLT) {
-    for (int e = T.vbend_off[k]; e < T.vbend_off[k + 1]; ++e) {
-      const int i = T.vbend[e];
+    auto bend_term = [&](const int i) {
       const double b0 = bx[i], b1 = by[i], b2 = bz[i];
       if (i == k) {
         F0 += b0;
@@ -172,11 +180,20 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
         F1 += -b1 / dn;
         F2 += -b2 / dn;
       }
+    };
+    if (T.vbendp) {
+      int b[kPadB];
+#pragma unroll
+      for (int j = 0; j < kPadB; ++j) b[j] = T.vbendp[kPadB * k + j];
+#pragma unroll
+      for (int j = 0; j < kPadB; ++j)
+        if (b[j] >= 0) bend_term(b[j]);
+    } else {
+      for (int e = T.vbend_off[k]; e < T.vbend_off[k + 1]; ++e) bend_term(T.vbend[e]);
     }
   }
   // edges: link, membrane viscosity, (PLT) dihedral bending; edge order
-  for (int e = T.vedge_off[k]; e < T.vedge_off[k + 1]; ++e) {
-    const int4 q = T.vedge4[e];
+  auto edge_term = [&](const int4 q) {
     const int ed = q.x >> 2, kind = q.x & 3;
     const int a = q.y, b = q.z;
     const D3 pa = c.pos(a), pb = c.pos(b);
@@ -217,6 +234,16 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
       if (kind != 2) { F0 += bf.x; F1 += bf.y; F2 += bf.z; }
       else { F0 -= bf.x; F1 -= bf.y; F2 -= bf.z; }
     }
+  };
+  if (MODEL != HC_MODEL_PLT && T.vedge4p) {
+    int4 q[kPadW];
+#pragma unroll
+    for (int j = 0; j < kPadW; ++j) q[j] = T.vedge4p[kPadW * k + j];
+#pragma unroll
+    for (int j = 0; j < kPadW; ++j)
+      if (q[j].x >= 0) edge_term(q[j]);
+  } else {
+    for (int e = T.vedge_off[k]; e < T.vedge_off[k + 1]; ++e) edge_term(T.vedge4[e]);
   }
   // inner links, inner-edge order
   for (int e = T.vinner_off[k]; e < T.vinner_off[k + 1]; ++e) {

@@ -401,6 +401,26 @@ bool build_type_tables(TypeHost& T, std::string& err) {
     const int ed = e >> 2;
     T.vedge4.insert(T.vedge4.end(), {e, T.edge[2 * ed], T.edge[2 * ed + 1], 0});
   }
+  T.padded = true;
+  for (int v = 0; v < nv; ++v)
+    if (T.vtri_off[v + 1] - T.vtri_off[v] > kPadW || T.vedge_off[v + 1] - T.vedge_off[v] > kPadW ||
+        T.vbend_off[v + 1] - T.vbend_off[v] > kPadB)
+      T.padded = false;
+  T.vtri4p.clear();
+  T.vedge4p.clear();
+  T.vbendp.clear();
+  if (T.padded) {
+    T.vtri4p.assign((size_t)nv * kPadW * 4, -1);
+    T.vedge4p.assign((size_t)nv * kPadW * 4, -1);
+    T.vbendp.assign((size_t)nv * kPadB, -1);
+    for (int v = 0; v < nv; ++v) {
+      for (int e = T.vtri_off[v], j = 0; e < T.vtri_off[v + 1]; ++e, ++j)
+        std::copy(&T.vtri4[4 * (size_t)e], &T.vtri4[4 * (size_t)e] + 4, &T.vtri4p[4 * ((size_t)v * kPadW + j)]);
+      for (int e = T.vedge_off[v], j = 0; e < T.vedge_off[v + 1]; ++e, ++j)
+        std::copy(&T.vedge4[4 * (size_t)e], &T.vedge4[4 * (size_t)e] + 4, &T.vedge4p[4 * ((size_t)v * kPadW + j)]);
+      for (int e = T.vbend_off[v], j = 0; e < T.vbend_off[v + 1]; ++e, ++j) T.vbendp[(size_t)v * kPadB + j] = T.vbend[e];
+    }
+  }
   return true;
 }
 

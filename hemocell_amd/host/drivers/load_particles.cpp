@@ -9,6 +9,8 @@
 //          the material and <name>.pos the positions, as for addCellType(name).
 // out.bin: int64 ncells, int64 nvertices, int32 cell id[ncells], int32 cell
 //          type[ncells], float64 x[nvertices][3].
+// With HEMOCELL_OUTPUT_AFTER=<n> the run steps n times and calls writeOutput()
+// (all cell and fluid outputs) before writing out.bin.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -67,6 +69,16 @@ static int run(int argc, char* argv[]) {
     if (dist >= 0) hemocell.setInitialMinimumDistanceFromSolid(name, dist);
   }
   hemocell.loadParticles();
+  if (const char* steps = std::getenv("HEMOCELL_OUTPUT_AFTER")) {
+    // step, then write the HDF5 / CSV output of the reference layout under ./tmp
+    for (auto& fp : hemocell.cellfields->fields)
+      hemocell.setOutputs(fp->name, {hemo::OUTPUT_POSITION, hemo::OUTPUT_VELOCITY, hemo::OUTPUT_FORCE,
+                                     hemo::OUTPUT_TRIANGLES, hemo::OUTPUT_CELL_ID, hemo::OUTPUT_VERTEX_ID});
+    hemocell.setFluidOutputs({hemo::OUTPUT_VELOCITY, hemo::OUTPUT_DENSITY, hemo::OUTPUT_BOUNDARY});
+    const int n = std::atoi(steps);
+    for (int k = 0; k < n; ++k) hemocell.iterate();
+    hemocell.writeOutput();
+  }
   const int64_t nc = hc_num_cells(hemocell.context());
   const int64_t nv = hc_num_vertices(hemocell.context());
   std::vector<double> x((size_t)nv * 3);

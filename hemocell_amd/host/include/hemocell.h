@@ -14,8 +14,10 @@
 //   * hemo::Lattice replaces plb::MultiBlockLattice3D: geometry is a node mask
 //     (fluid, bounce-back, velocity wall) set with defineBounceBack() and the
 //     boundary-condition helpers below, not with Palabos dynamics objects.
-//   * Output (writeOutput, setOutputs, HDF5), checkpoints and load balancing
-//     are not produced; the calls are accepted and do nothing.
+//   * writeOutput() writes the HDF5 and CSV files of the reference's layout
+//     (positions, velocities, total and repulsion forces, ids, triangles; fluid
+//     velocity, density and boundary); per-term force splits, checkpoints and
+//     load balancing are not produced (those calls are accepted and do nothing).
 //   * Errors that the reference reports with a log line and exit(1) do the same
 //     here; errors from the device return as hemo::HemoCellError.
 #pragma once
@@ -262,8 +264,9 @@ class HemoCellField {
   std::unique_ptr<MeshMetrics> meshmetric;
   hc_model_params params{};
   HemoCellFields& cellFields;
+  std::vector<int> outputs{OUTPUT_POSITION};  // HDF5 datasets written by writeOutput
   HemoCellField(HemoCellFields& cf, const std::string& name, int ctype, int constructType, int model);
-  void setOutputVariables(const std::vector<int>&) {}
+  void setOutputVariables(const std::vector<int>& o) { outputs = o; }
 };
 
 class HemoCellFields {
@@ -298,8 +301,8 @@ class HemoCell {
     if (!cellfields) initializeCellfield();
     cellfields->addCellType(name, constructType, Mechanics::model);
   }
-  void setOutputs(std::string, std::vector<int>) {}
-  void setFluidOutputs(std::vector<int>) {}
+  void setOutputs(std::string name, std::vector<int> outputs) { (*cellfields)[name]->setOutputVariables(outputs); }
+  void setFluidOutputs(std::vector<int> outputs) { fluidOutputs = outputs; }
   void setRepulsion(T repulsionConstant, T repulsionCutoff);
   void setRepulsionTimeScaleSeperation(unsigned int separation);
   void setMaterialTimeScaleSeparation(std::string name, unsigned int separation);
@@ -309,7 +312,10 @@ class HemoCell {
   void loadParticles();
   void loadCheckPoint() {}
   void saveCheckPoint() {}
-  void writeOutput() {}
+  // HDF5 (per cell type and fluid) and CSV output of the current state in the
+  // reference's file layout
+  void writeOutput();
+  std::string outputDirectory();
   void iterate();
 
   // Device handle (created by the constructor).  Exposed for drivers that
@@ -342,6 +348,8 @@ class HemoCell {
   std::vector<int64_t> extra_ids, pending_ids;
   std::vector<T> extra_f, pending_f;
   bool extra_set = false;
+  std::vector<int> fluidOutputs;
+  std::string outDir;
 };
 
 // -------------------------------------------------------------- cellInfo -----

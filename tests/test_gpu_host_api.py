@@ -89,3 +89,60 @@ def test_stretch_cell_reference_bands(force, transverse, axial):
     assert transverse[0] <= res["transverse_um"] <= transverse[1]
     assert axial[0] <= res["axial_um"] <= axial[1]
     assert 0.98 < res["volume_ratio"] <= 1.02
+
+
+def _h5(path, what):
+    r = subprocess.run([os.path.join(BIN, "h5_read"), str(path), what], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (path, what, r.stderr)
+    return json.loads(r.stdout)
+
+
+def test_write_output_in_the_reference_layout(tmp_path):
+    """writeOutput(): per-type and fluid HDF5 files and per-type CSV files with the
+    reference's names, datasets, attributes and SI scaling
+    (io/ParticleHdf5IO.cpp, io/FluidHdf5IO.hh, io/writeCellInfoCSV.cpp)."""
+    u = cases.units()
+    nx, ny, nz = 40, 30, 36
+    (tmp_path / "config.xml").write_text(cases.config_xml())
+    (tmp_path / "RBC.xml").write_text(cases.material_xml(cases.RBC_MAT, "RBC"))
+    (tmp_path / "PLT.xml").write_text(cases.material_xml(cases.PLT_MAT, "PLT", inner_edges=21))
+    (tmp_path / "RBC.pos").write_text("2\n6 7.5 6 90 0 0\n14 7.5 12 0 30 0\n")
+    (tmp_path / "PLT.pos").write_text("1\n3 11 15 10 20 30\n")
+    env = dict(os.environ, HEMOCELL_OUTPUT_AFTER="3")
+    print("[gpu-host] load_particles with output", flush=True)
+    r = subprocess.run([os.path.join(BIN, "load_particles"), "config.xml", str(nx), str(ny), str(nz), "bb-y",
+                        str(tmp_path / "cells.bin"), "RBC:RBC", "PLT:PLT"], cwd=tmp_path, env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ids, types, x = _read_out(tmp_path / "cells.bin")
+    it = "%012d" % 3
+    d = tmp_path / "tmp" / "hdf5" / it
+    rbc = d / ("RBC.%s.p.0.h5" % it)
+    at = _h5(rbc, "--attrs")
+    assert at["dx"] == [u.dx] and at["dt"] == [u.dt] and at["iteration"] == [3] and at["numberOfParticles"] == [1284]
+    pos = _h5(rbc, "Position")
+    assert pos["dims"] == [1284, 3]
+    want = (x[:1284] * u.dx).astype(np.float32).astype(np.float64)
+    # h5_read prints 9 significant digits, enough to recover each float32 exactly
+    got = np.array(pos["data"]).reshape(-1, 3).astype(np.float32).astype(np.float64)
+    bad = np.flatnonzero(np.any(got != want, axis=1))
+    assert bad.size == 0, (bad[:5], got[bad[:3]], want[bad[:3]], np.abs(got - want).max())
+    tri = _h5(rbc, "Triangles")
+    assert tri["dims"] == [2560, 3] and min(tri["data"]) == 0 and max(tri["data"]) == 1283
+    assert _h5(rbc, "Cell Id")["data"] == [0.0] * 642 + [1.0] * 642
+    assert _h5(rbc, "Vertex Id")["data"][:642] == [float(k) for k in range(642)]
+    assert _h5(d / ("PLT.%s.p.0.h5" % it), "Position")["dims"] == [66, 3]
+    fl = d / ("Fluid.%s.p.0.h5" % it)
+    fa = _h5(fl, "--attrs")
+    assert fa["subdomainSize"] == [nz + 2, ny + 2, nx + 2] and fa["numberOfCells"] == [(nx + 2) * (ny + 2) * (nz + 2)]
+    b = np.array(_h5(fl, "Boundary")["data"]).reshape(nz + 2, ny + 2, nx + 2)
+    assert np.all(b[:, 1, :] == 1) and np.all(b[:, ny, :] == 1) and np.all(b[:, 2:ny, :] == 0)
+    vel = _h5(fl, "Velocity")
+    assert vel["dims"] == [nz + 2, ny + 2, nx + 2, 3] and np.isfinite(vel["data"]).all()
+    rho = np.array(_h5(fl, "Density")["data"])
+    assert np.allclose(rho, u.df / u.dx ** 2, rtol=1e-3)
+    rows = (tmp_path / "tmp" / "csv" / ("RBC.%s.csv" % it)).read_text().splitlines()
+    assert rows[0] == "X,Y,Z,area,volume,atomic_block,cellId,baseCellId,velocity_x,velocity_y,velocity_z"
+    assert len(rows) == 3
+    vol = float(rows[1].split(",")[4])
+    assert abs(vol * 1e18 - 81.116) < 1.0  # um^3, the rest volume (RBC_template.xml:16)
