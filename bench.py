@@ -181,7 +181,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", type=int, default=32, metavar="N",
                     help="cases of the serving stream (0 skips the latency measurement)")
-    ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate, cases/s")
+    ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate, cases/s")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a device-to-device copy of MIB MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()

@@ -753,6 +753,8 @@ hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* no
       }
     }
   }
+  // grouped by face, so that the lanes of a wavefront take the same face branch
+  std::stable_sort(vel.begin(), vel.end(), [&](uint32_t a, uint32_t b) { return node_type[a] < node_type[b]; });
   ctx->general = has_walls || !all_periodic;
   const size_t rs = rsize(ctx);
   hc_status s;

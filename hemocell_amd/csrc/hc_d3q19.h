@@ -12,20 +12,21 @@
 
 namespace hcg {
 
-__host__ __device__ constexpr int cx(int i) {
-  return i == 0 ? 0 : (i <= 9 ? (i == 1 || (i >= 4 && i <= 7) ? -1 : 0) : -cx(i - 9));
+// c_1..c_9 (the c_{i+9} are their negatives); written without recursion so that
+// calls with a loop index inline and fold
+__host__ __device__ __forceinline__ constexpr int cx9(int i) { return (i == 1 || (i >= 4 && i <= 7)) ? -1 : 0; }
+__host__ __device__ __forceinline__ constexpr int cy9(int i) {
+  return (i == 2 || i == 4 || i == 8 || i == 9) ? -1 : (i == 5 ? 1 : 0);
 }
-__host__ __device__ constexpr int cy(int i) {
-  return i == 0 ? 0
-                : (i <= 9 ? (i == 2 || i == 4 || i == 8 || i == 9 ? -1 : (i == 5 ? 1 : 0)) : -cy(i - 9));
+__host__ __device__ __forceinline__ constexpr int cz9(int i) {
+  return (i == 3 || i == 6 || i == 8) ? -1 : ((i == 7 || i == 9) ? 1 : 0);
 }
-__host__ __device__ constexpr int cz(int i) {
-  return i == 0 ? 0
-                : (i <= 9 ? (i == 3 || i == 6 || i == 8 ? -1 : (i == 7 || i == 9 ? 1 : 0)) : -cz(i - 9));
-}
-__host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9); }
+__host__ __device__ __forceinline__ constexpr int cx(int i) { return i == 0 ? 0 : (i <= 9 ? cx9(i) : -cx9(i - 9)); }
+__host__ __device__ __forceinline__ constexpr int cy(int i) { return i == 0 ? 0 : (i <= 9 ? cy9(i) : -cy9(i - 9)); }
+__host__ __device__ __forceinline__ constexpr int cz(int i) { return i == 0 ? 0 : (i <= 9 ? cz9(i) : -cz9(i - 9)); }
+__host__ __device__ __forceinline__ constexpr int opp(int i) { return i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9); }
 template <typename R>
-__host__ __device__ constexpr R wgt(int i) {
+__host__ __device__ __forceinline__ constexpr R wgt(int i) {
   return i == 0 ? R(1) / R(3) : (((i - 1) % 9) < 3 ? R(1) / R(18) : R(1) / R(36));
 }
 

@@ -108,18 +108,21 @@ __device__ __forceinline__ R feq(int i, R rhoBar, R invRho, R j0, R j1, R j2, R 
 // as f_eq(rhoBar, rho u_w) + f_neq(Pi_neq), Pi_neq from the known populations
 // (on-wall ones and those arriving from the interior, the latter counted twice:
 // bounce-back of the non-equilibrium part).
-template <typename R>
-__device__ __forceinline__ void velocity_boundary(R (&f)[19], int axis, int orient, const double uw[3]) {
-  auto cc = [](int i, int a) { return a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i)); };
+__device__ __forceinline__ constexpr int c_axis(int i, int a) { return a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i)); }
+
+template <typename R, int AXIS, int ORIENT>
+__device__ __forceinline__ void velocity_boundary(R (&f)[19], const double uw0, const double uw1, const double uw2) {
+  // the face is a compile-time (axis, orientation) pair, so every lattice table
+  // lookup below folds to a constant
   R rhoOnWall = R(0), rhoNormal = R(0);
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
-    int c = cc(i, axis);
+    const int c = c_axis(i, AXIS);
     if (c == 0) rhoOnWall += f[i];
-    else if (c == orient) rhoNormal += f[i];
+    else if (c == ORIENT) rhoNormal += f[i];
   }
-  const R w0 = R(uw[0]), w1 = R(uw[1]), w2 = R(uw[2]);
-  const R velNormal = R(orient) * (axis == 0 ? w0 : (axis == 1 ? w1 : w2));
+  const R w0 = R(uw0), w1 = R(uw1), w2 = R(uw2);
+  const R velNormal = R(ORIENT) * (AXIS == 0 ? w0 : (AXIS == 1 ? w1 : w2));
   const R rhoBar = (R(2) * rhoNormal + rhoOnWall - velNormal) / (R(1) + velNormal);
   const R rho = rhoBar + R(1);
   const R j0 = rho * w0, j1 = rho * w1, j2 = rho * w2;
@@ -128,35 +131,29 @@ __device__ __forceinline__ void velocity_boundary(R (&f)[19], int axis, int orie
   R fNeq[19];
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
-    int c = cc(i, axis);
-    fNeq[i] = (c == 0 || c == orient) ? f[i] - feq<R>(i, rhoBar, invRho, j0, j1, j2, jSqr) : R(0);
+    const int c = c_axis(i, AXIS);
+    fNeq[i] = (c == 0 || c == ORIENT) ? f[i] - feq<R>(i, rhoBar, invRho, j0, j1, j2, jSqr) : R(0);
   }
-  R Pi[6];
-  int p = 0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-    for (int b = a; b < 3; ++b) {
-      R s = R(0);
-#pragma unroll
-      for (int i = 0; i < 19; ++i)
-        if (cc(i, axis) == 0) s += R(cc(i, a) * cc(i, b)) * fNeq[i];
-#pragma unroll
-      for (int i = 0; i < 19; ++i)
-        if (cc(i, axis) == orient) s += R(2 * cc(i, a) * cc(i, b)) * fNeq[i];
-      Pi[p++] = s;
-    }
+  // Pi_neq (xx, xy, xz, yy, yz, zz): on-wall populations once, incoming ones twice
+  R Pi[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
-    R fn = R(0);
-    int q = 0;
-  #pragma unroll
-  for (int a = 0; a < 3; ++a)
-      for (int b = a; b < 3; ++b) {
-        R t = R(cc(i, a) * cc(i, b));
-        if (a == b) t -= R(1) / R(3);
-        else t *= R(2);
-        fn += t * Pi[q++];
-      }
+    const int c = c_axis(i, AXIS);
+    if (c != 0 && c != ORIENT) continue;
+    const R m = c == 0 ? fNeq[i] : R(2) * fNeq[i];
+    Pi[0] += R(cx(i) * cx(i)) * m;
+    Pi[1] += R(cx(i) * cy(i)) * m;
+    Pi[2] += R(cx(i) * cz(i)) * m;
+    Pi[3] += R(cy(i) * cy(i)) * m;
+    Pi[4] += R(cy(i) * cz(i)) * m;
+    Pi[5] += R(cz(i) * cz(i)) * m;
+  }
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    const R qxx = R(cx(i) * cx(i)) - R(1) / R(3), qyy = R(cy(i) * cy(i)) - R(1) / R(3),
+            qzz = R(cz(i) * cz(i)) - R(1) / R(3);
+    R fn = qxx * Pi[0] + R(2 * cx(i) * cy(i)) * Pi[1] + R(2 * cx(i) * cz(i)) * Pi[2] + qyy * Pi[3] +
+           R(2 * cy(i) * cz(i)) * Pi[4] + qzz * Pi[5];
     fn *= wgt<R>(i) * R(3) * R(3) / R(2);
     f[i] = feq<R>(i, rhoBar, invRho, j0, j1, j2, jSqr) + fn;
   }
@@ -223,8 +220,15 @@ __global__ __launch_bounds__(64) void k_velocity_nodes(R* __restrict__ f, const 
   const NbrOff o = neighbour_offsets(x, y, z, L);
   R fl[19];
   aa_load<R, ODD, true>(f, S, n, o, fl, std::make_integer_sequence<int, 19>{});
-  const int face = type[n] - HC_NODE_VELOCITY;
-  velocity_boundary<R>(fl, face >> 1, (face & 1) ? 1 : -1, a.wall_u[face]);
+  // one compile-time instance per domain face (no runtime-indexed lattice tables)
+  switch (type[n] - HC_NODE_VELOCITY) {
+    case 0: velocity_boundary<R, 0, -1>(fl, a.wall_u[0][0], a.wall_u[0][1], a.wall_u[0][2]); break;
+    case 1: velocity_boundary<R, 0, 1>(fl, a.wall_u[1][0], a.wall_u[1][1], a.wall_u[1][2]); break;
+    case 2: velocity_boundary<R, 1, -1>(fl, a.wall_u[2][0], a.wall_u[2][1], a.wall_u[2][2]); break;
+    case 3: velocity_boundary<R, 1, 1>(fl, a.wall_u[3][0], a.wall_u[3][1], a.wall_u[3][2]); break;
+    case 4: velocity_boundary<R, 2, -1>(fl, a.wall_u[4][0], a.wall_u[4][1], a.wall_u[4][2]); break;
+    default: velocity_boundary<R, 2, 1>(fl, a.wall_u[5][0], a.wall_u[5][1], a.wall_u[5][2]); break;
+  }
   const R F0 = R(a.g[0]) + Fcur[n], F1 = R(a.g[1]) + Fcur[S + n], F2 = R(a.g[2]) + Fcur[2 * S + n];
   collide_guo<R>(fl, F0, F1, F2, R(a.omega));
   Fclr[n] = R(0);
@@ -241,6 +245,7 @@ __global__ void k_init_eq(R* f, const uint8_t* type, LatticeDev L, double rho, d
   const R rhoBar = R(rho - 1.0), invRho = R(1.0 / rho);
   const R j0 = R(rho * u0), j1 = R(rho * u1), j2 = R(rho * u2);
   const R jSqr = j0 * j0 + j1 * j1 + j2 * j2;
+#pragma unroll
   for (int i = 0; i < 19; ++i) f[i * L.stride + n] = bb ? R(0) : feq<R>(i, rhoBar, invRho, j0, j1, j2, jSqr);
 }
 
