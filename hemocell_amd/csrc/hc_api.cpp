@@ -358,7 +358,10 @@ hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
     HC_TRY(hipEventRecord(ctx->fork_ev, ctx->stream), "hipEventRecord");
     for (int j = 0; j < nside; ++j) HC_TRY(hipStreamWaitEvent(ctx->side[j], ctx->fork_ev, 0), "hipStreamWaitEvent");
   }
-  for (size_t k = 0; k < act.size(); ++k) {
+  // the side kernels are launched first: their few, long workgroups (the WBCs)
+  // should not wait for the many short ones of the main type to drain
+  for (size_t kk = 0; kk < act.size(); ++kk) {
+    const size_t k = (kk + 1) % act.size();
     TypeBufs& tb = *act[k];
     hipStream_t st = (k == 0 || nside == 0) ? ctx->stream : ctx->side[(k - 1) % nside];
     int flags = CELL_FLAG_SPREAD;
@@ -673,7 +676,7 @@ hc_status hc_create(int device, hc_ctx** out) {
   if (sp && std::string(sp) == "vertex") ctx->spread_per_cell = false;
   const char* fu = std::getenv("HC_FUSE");  // tuning: 0 runs the unfused vertex kernels
   if (fu && std::string(fu) == "0") ctx->fuse = false;
-  const char* cb = std::getenv("HC_CELL_BLOCK");  // tuning: 256, 512 or 1024 threads per cell
+  const char* cb = std::getenv("HC_CELL_BLOCK");  // tuning: 256, 512 or 1024 threads per RBC cell
   if (cb) ctx->cell_block = std::atoi(cb);
   const char* wb = std::getenv("HC_WBC_BLOCK");  // tuning: threads per WBC cell
   if (wb) ctx->wbc_block = std::atoi(wb);
@@ -857,12 +860,16 @@ hc_status hc_add_cell_type(hc_ctx* ctx, int model, int nv, int nt, const int32_t
     D.vedge4 = (const int4*)p4;
     D.vtri4p = D.vedge4p = nullptr;
     D.vbendp = nullptr;
+    D.vtri_aeq = D.vedge_l0 = D.vbend_n = nullptr;
     if (T.padded && !padded_off(ctx)) {
       if ((s = upload(ctx, *tb, T.vtri4p, &p4)) != HC_OK) return s;
       D.vtri4p = (const int4*)p4;
       if ((s = upload(ctx, *tb, T.vedge4p, &p4)) != HC_OK) return s;
       D.vedge4p = (const int4*)p4;
       if ((s = upload(ctx, *tb, T.vbendp, &D.vbendp)) != HC_OK) return s;
+      if ((s = upload(ctx, *tb, T.vtri_aeq, &D.vtri_aeq)) != HC_OK) return s;
+      if ((s = upload(ctx, *tb, T.vedge_l0, &D.vedge_l0)) != HC_OK) return s;
+      if ((s = upload(ctx, *tb, T.vbend_n, &D.vbend_n)) != HC_OK) return s;
     }
   }
   D.model = model;

@@ -41,6 +41,9 @@ struct TypeHost {
   std::vector<int32_t> vedge4;  // per vedge entry: e*4 + kind, the edge's two ends, 0
   bool padded = false;          // every list fits kPadW / kPadB
   std::vector<int32_t> vtri4p, vedge4p, vbendp;  // nv*kPadW int4, nv*kPadW int4, nv*kPadB int
+  // per padded slot: the rest area of its triangle, the rest length of its edge and
+  // the ring size of its bending contributor (addresses known from the vertex alone)
+  std::vector<double> vtri_aeq, vedge_l0, vbend_n;
   hc_model_params p{};
   int timescale = 1;
 };
@@ -58,6 +61,7 @@ struct TypeDev {
   const int4 *vtri4, *vedge4;
   const int4 *vtri4p, *vedge4p;  // fixed-width copies (null when a list does not fit)
   const int32_t* vbendp;
+  const double *vtri_aeq, *vedge_l0, *vbend_n;
   double volume_eq, area_mean_eq, edge_mean_eq;
   double k_link, k_bend, k_volume, k_area, eta_m, k_inner_rigid, k_cyto, core_radius, radius;
 };

@@ -79,9 +79,15 @@ __device__ __forceinline__ D3 unit_normal(D3 v0, D3 v1, D3 v2) {
 // bending force of vertex i from its ordered 1-ring (rbcHighOrderModel.cpp:127-166)
 __device__ D3 bending_force(const CellView& c, const TypeDev& T, int i) {
   const int nr = T.nring[i];
-  const int32_t* R = T.ring + 6 * i;
+  // the ring (at most 6 entries, stored 6 wide) is loaded at once
+  int R[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) R[j] = T.ring[6 * i + j];
+  const double patch_eq = T.patch_dist_eq[i];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int j = 0; j < nr; ++j) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (j >= nr) break;
     D3 p = c.pos(R[j]);
     s0 += p.x;
     s1 += p.y;
@@ -93,8 +99,10 @@ __device__ D3 bending_force(const CellView& c, const TypeDev& T, int i) {
   D3 pn = {0.0, 0.0, 0.0};
   D3 prev = c.pos(R[0]);
   const D3 first = prev;
-  for (int j = 0; j < nr; ++j) {
-    const D3 nxt = (j + 1 < nr) ? c.pos(R[j + 1]) : first;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (j >= nr) break;
+    const D3 nxt = (j + 1 < nr) ? c.pos(R[j + 1 < 6 ? j + 1 : 0]) : first;
     D3 tn = crossp(prev - xi, nxt - xi);
     const double l = normp(tn);
     pn.x += tn.x / l;
@@ -107,7 +115,7 @@ __device__ D3 bending_force(const CellView& c, const TypeDev& T, int i) {
   pn.y /= l;
   pn.z /= l;
   const double ndev = dotp(pn, dev);
-  const double dDev = (ndev - T.patch_dist_eq[i]) / T.edge_mean_eq;
+  const double dDev = (ndev - patch_eq) / T.edge_mean_eq;
   const double s = T.k_bend * (dDev + dDev / fabs(MaxCellBendingAngle - dDev * dDev));
   return {pn.x * s, pn.y * s, pn.z * s};
 }
@@ -126,8 +134,7 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
   // (all area terms, then all volume terms) up to the grouping of that sum
   {
     double A0 = 0.0, A1 = 0.0, A2 = 0.0, B0 = 0.0, B1 = 0.0, B2 = 0.0;
-    auto tri_term = [&](const int4 q) {
-      const int t = q.x;
+    auto tri_term = [&](const int4 q, const double a_eq) {
       const D3 v0 = c.pos(q.y), v1 = c.pos(q.z), v2 = c.pos(q.w);
       D3 n = crossp(v1 - v0, v2 - v0);
       const double nn = normp(n);
@@ -140,7 +147,6 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
       } else {
         n = {0.0, 0.0, 0.0};
       }
-      const double a_eq = T.tri_area_eq[t];
       const double ar = (area - a_eq) / a_eq;
       const double afm = T.k_area * (ar + ar / fabs(MaxCellSurfaceAreaChange - ar * ar));
       const double c0 = (v0.x + v1.x + v2.x) / 3.0, c1 = (v0.y + v1.y + v2.y) / 3.0, c2 = (v0.z + v1.z + v2.z) / 3.0;
@@ -154,13 +160,17 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
     };
     if (T.vtri4p) {
       int4 q[kPadW];
+      double aeq[kPadW];
 #pragma unroll
-      for (int j = 0; j < kPadW; ++j) q[j] = T.vtri4p[kPadW * k + j];
+      for (int j = 0; j < kPadW; ++j) {
+        q[j] = T.vtri4p[kPadW * k + j];
+        aeq[j] = T.vtri_aeq[kPadW * k + j];
+      }
 #pragma unroll
       for (int j = 0; j < kPadW; ++j)
-        if (q[j].x >= 0) tri_term(q[j]);
+        if (q[j].x >= 0) tri_term(q[j], aeq[j]);
     } else {
-      for (int e = T.vtri_off[k]; e < T.vtri_off[k + 1]; ++e) tri_term(T.vtri4[e]);
+      for (int e = T.vtri_off[k]; e < T.vtri_off[k + 1]; ++e) tri_term(T.vtri4[e], T.tri_area_eq[T.vtri4[e].x]);
     }
     F0 = A0 + B0;
     F1 = A1 + B1;
@@ -168,14 +178,13 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
   }
   // bending: own force and -F_i/n_i of every vertex i whose 1-ring holds k, ascending i
   if (MODEL != HC_MODEL_PLT) {
-    auto bend_term = [&](const int i) {
+    auto bend_term = [&](const int i, const double dn) {
       const double b0 = bx[i], b1 = by[i], b2 = bz[i];
       if (i == k) {
         F0 += b0;
         F1 += b1;
         F2 += b2;
       } else {
-        const double dn = (double)(unsigned)T.nring[i];
         F0 += -b0 / dn;
         F1 += -b1 / dn;
         F2 += -b2 / dn;
@@ -183,17 +192,22 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
     };
     if (T.vbendp) {
       int b[kPadB];
+      double dn[kPadB];
 #pragma unroll
-      for (int j = 0; j < kPadB; ++j) b[j] = T.vbendp[kPadB * k + j];
+      for (int j = 0; j < kPadB; ++j) {
+        b[j] = T.vbendp[kPadB * k + j];
+        dn[j] = T.vbend_n[kPadB * k + j];
+      }
 #pragma unroll
       for (int j = 0; j < kPadB; ++j)
-        if (b[j] >= 0) bend_term(b[j]);
+        if (b[j] >= 0) bend_term(b[j], dn[j]);
     } else {
-      for (int e = T.vbend_off[k]; e < T.vbend_off[k + 1]; ++e) bend_term(T.vbend[e]);
+      for (int e = T.vbend_off[k]; e < T.vbend_off[k + 1]; ++e)
+        bend_term(T.vbend[e], (double)(unsigned)T.nring[T.vbend[e]]);
     }
   }
   // edges: link, membrane viscosity, (PLT) dihedral bending; edge order
-  auto edge_term = [&](const int4 q) {
+  auto edge_term = [&](const int4 q, const double l0) {
     const int ed = q.x >> 2, kind = q.x & 3;
     const int a = q.y, b = q.z;
     const D3 pa = c.pos(a), pb = c.pos(b);
@@ -202,7 +216,7 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
     const D3 uv = {ev.x / len, ev.y / len, ev.z / len};
     if (kind != 2) {
       const double sgn = kind == 0 ? 1.0 : -1.0;
-      const double ef = (len - T.edge_len_eq[ed]) / T.edge_len_eq[ed];
+      const double ef = (len - l0) / l0;
       const double fs = T.k_link * (ef + ef / fabs(MaxCellPersistenceLength - ef * ef));
       const D3 fl = {uv.x * fs, uv.y * fs, uv.z * fs};
       if (sgn > 0) { F0 += fl.x; F1 += fl.y; F2 += fl.z; }
@@ -237,13 +251,17 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
   };
   if (MODEL != HC_MODEL_PLT && T.vedge4p) {
     int4 q[kPadW];
+    double l0[kPadW];
 #pragma unroll
-    for (int j = 0; j < kPadW; ++j) q[j] = T.vedge4p[kPadW * k + j];
+    for (int j = 0; j < kPadW; ++j) {
+      q[j] = T.vedge4p[kPadW * k + j];
+      l0[j] = T.vedge_l0[kPadW * k + j];
+    }
 #pragma unroll
     for (int j = 0; j < kPadW; ++j)
-      if (q[j].x >= 0) edge_term(q[j]);
+      if (q[j].x >= 0) edge_term(q[j], l0[j]);
   } else {
-    for (int e = T.vedge_off[k]; e < T.vedge_off[k + 1]; ++e) edge_term(T.vedge4[e]);
+    for (int e = T.vedge_off[k]; e < T.vedge_off[k + 1]; ++e) edge_term(T.vedge4[e], T.edge_len_eq[T.vedge4[e].x >> 2]);
   }
   // inner links, inner-edge order
   for (int e = T.vinner_off[k]; e < T.vinner_off[k + 1]; ++e) {

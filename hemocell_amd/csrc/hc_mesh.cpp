@@ -413,12 +413,22 @@ bool build_type_tables(TypeHost& T, std::string& err) {
     T.vtri4p.assign((size_t)nv * kPadW * 4, -1);
     T.vedge4p.assign((size_t)nv * kPadW * 4, -1);
     T.vbendp.assign((size_t)nv * kPadB, -1);
+    T.vtri_aeq.assign((size_t)nv * kPadW, 1.0);
+    T.vedge_l0.assign((size_t)nv * kPadW, 1.0);
+    T.vbend_n.assign((size_t)nv * kPadB, 1.0);
     for (int v = 0; v < nv; ++v) {
-      for (int e = T.vtri_off[v], j = 0; e < T.vtri_off[v + 1]; ++e, ++j)
+      for (int e = T.vtri_off[v], j = 0; e < T.vtri_off[v + 1]; ++e, ++j) {
         std::copy(&T.vtri4[4 * (size_t)e], &T.vtri4[4 * (size_t)e] + 4, &T.vtri4p[4 * ((size_t)v * kPadW + j)]);
-      for (int e = T.vedge_off[v], j = 0; e < T.vedge_off[v + 1]; ++e, ++j)
+        T.vtri_aeq[(size_t)v * kPadW + j] = T.tri_area_eq[T.vtri4[4 * (size_t)e]];
+      }
+      for (int e = T.vedge_off[v], j = 0; e < T.vedge_off[v + 1]; ++e, ++j) {
         std::copy(&T.vedge4[4 * (size_t)e], &T.vedge4[4 * (size_t)e] + 4, &T.vedge4p[4 * ((size_t)v * kPadW + j)]);
-      for (int e = T.vbend_off[v], j = 0; e < T.vbend_off[v + 1]; ++e, ++j) T.vbendp[(size_t)v * kPadB + j] = T.vbend[e];
+        T.vedge_l0[(size_t)v * kPadW + j] = T.edge_len_eq[T.vedge4[4 * (size_t)e] >> 2];
+      }
+      for (int e = T.vbend_off[v], j = 0; e < T.vbend_off[v + 1]; ++e, ++j) {
+        T.vbendp[(size_t)v * kPadB + j] = T.vbend[e];
+        T.vbend_n[(size_t)v * kPadB + j] = (double)(unsigned)T.nring[T.vbend[e]];
+      }
     }
   }
   return true;
