@@ -81,6 +81,9 @@ struct hc_ctx {
   int spread_lds = 0;           // LDS box budget in bytes (0: default per precision)
   int cell_block = 256;         // threads per RBC cell in the fused vertex kernel
   int wbc_block = 1024;         // threads per WBC cell (falls back when the LDS does not fit)
+  bool cell_block_env = false;  // the RBC block was set explicitly
+  int n_cu = 256;               // compute units of the device
+  bool fuse_vel = true;         // velocity-wall nodes inside the collide-stream kernel
   // extra (stretch) forces
   int64_t* d_extra_ids = nullptr;
   double* d_extra_f = nullptr;
@@ -370,7 +373,9 @@ hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
     if (ctx->repulsion) flags |= CELL_FLAG_REPULSION;
     // spread box in LDS; larger supports are spread in slabs of x-planes
     const int box = ctx->spread_lds > 0 ? ctx->spread_lds : 16384;
-    const int block = tb.host.model == HC_MODEL_WBC ? ctx->wbc_block : ctx->cell_block;
+    // few cells: their latency, not the occupancy, sets the time: the widest block
+    int block = tb.host.model == HC_MODEL_WBC ? ctx->wbc_block : ctx->cell_block;
+    if (tb.ncells < ctx->n_cu && !(ctx->cell_block_env && tb.host.model == HC_MODEL_RBC)) block = 1024;
     HC_TRY(launch_cell_step<R>(st, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base, (const R*)ctx->ufield,
                                ctx->type, ctx->L, (R*)ctx->Fibm[1 - ctx->cur], ctx->d_extra_dense, ctx->bend,
                                ctx->f_limit, flags, box, ctx->lds_device, ctx->d_wall_hits, block),
@@ -413,7 +418,8 @@ hc_status do_collide(hc_ctx* ctx) {
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_COLLIDE, &ev);
   HC_TRY(launch_collide_stream<R>(ctx->stream, (R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], (R*)ctx->Fibm[1 - ctx->cur],
-                                  ctx->type, ctx->L, a, ctx->phase, ctx->general, ctx->vel_nodes, ctx->n_vel),
+                                  ctx->type, ctx->L, a, ctx->phase, ctx->general, ctx->vel_nodes, ctx->n_vel,
+                                  ctx->fuse_vel),
          "collide_stream");
   prof_end(ctx, HC_PROF_COLLIDE, ev);
   ctx->phase ^= 1;
@@ -537,6 +543,8 @@ std::vector<unsigned char> launch_signature(const hc_ctx* c) {
   put(v, c->spread_lds);
   put(v, c->cell_block);
   put(v, c->wbc_block);
+  put(v, c->n_cu);
+  put(v, c->fuse_vel);
   put(v, c->pv_timescale);
   for (const auto& tp : c->types) {
     put(v, tp->dev);
@@ -677,7 +685,15 @@ hc_status hc_create(int device, hc_ctx** out) {
   const char* fu = std::getenv("HC_FUSE");  // tuning: 0 runs the unfused vertex kernels
   if (fu && std::string(fu) == "0") ctx->fuse = false;
   const char* cb = std::getenv("HC_CELL_BLOCK");  // tuning: 256, 512 or 1024 threads per RBC cell
-  if (cb) ctx->cell_block = std::atoi(cb);
+  if (cb) {
+    ctx->cell_block = std::atoi(cb);
+    ctx->cell_block_env = true;
+  }
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    ctx->n_cu = ncu;
+  const char* fv = std::getenv("HC_FUSE_VEL");  // 0: velocity-wall nodes in their own kernel
+  if (fv && std::string(fv) == "0") ctx->fuse_vel = false;
   const char* wb = std::getenv("HC_WBC_BLOCK");  // tuning: threads per WBC cell
   if (wb) ctx->wbc_block = std::atoi(wb);
   const char* gr = std::getenv("HC_GRAPH");  // 0: launch every step eagerly

@@ -15,7 +15,7 @@ struct CollideArgs {
 template <typename R>
 hipError_t launch_collide_stream(hipStream_t s, R* f, const R* Fcur, R* Fclear, const uint8_t* type,
                                  const LatticeDev& L, const CollideArgs& a, int odd, bool general,
-                                 const uint32_t* vel_nodes, int n_vel);
+                                 const uint32_t* vel_nodes, int n_vel, bool fuse_vel);
 template <typename R>
 hipError_t launch_init_equilibrium(hipStream_t s, R* f, const uint8_t* type, const LatticeDev& L,
                                    double rho, const double u[3]);

@@ -166,10 +166,24 @@ __device__ __forceinline__ void coords(int64_t n, const LatticeDev& L, int& x, i
   x = (int)(t / L.ny);
 }
 
-// GENERAL: node-type mask present and/or some axis non-periodic.  Velocity
-// boundary nodes are skipped here and updated by k_velocity_nodes (each node
-// only touches its own 19 slots, so the two kernels may run in either order).
-template <typename R, bool ODD, bool GENERAL>
+// regularized reconstruction of a velocity node of domain face `face`
+template <typename R>
+__device__ __forceinline__ void velocity_node(R (&fl)[19], int face, const CollideArgs& a) {
+  // one compile-time instance per domain face (no runtime-indexed lattice tables)
+  switch (face) {
+    case 0: velocity_boundary<R, 0, -1>(fl, a.wall_u[0][0], a.wall_u[0][1], a.wall_u[0][2]); break;
+    case 1: velocity_boundary<R, 0, 1>(fl, a.wall_u[1][0], a.wall_u[1][1], a.wall_u[1][2]); break;
+    case 2: velocity_boundary<R, 1, -1>(fl, a.wall_u[2][0], a.wall_u[2][1], a.wall_u[2][2]); break;
+    case 3: velocity_boundary<R, 1, 1>(fl, a.wall_u[3][0], a.wall_u[3][1], a.wall_u[3][2]); break;
+    case 4: velocity_boundary<R, 2, -1>(fl, a.wall_u[4][0], a.wall_u[4][1], a.wall_u[4][2]); break;
+    default: velocity_boundary<R, 2, 1>(fl, a.wall_u[5][0], a.wall_u[5][1], a.wall_u[5][2]); break;
+  }
+}
+
+// GENERAL: node-type mask present and/or some axis non-periodic.  VEL: the
+// velocity-boundary nodes are reconstructed in place (each node only touches its
+// own 19 slots); without VEL they are skipped and k_velocity_nodes updates them.
+template <typename R, bool ODD, bool GENERAL, bool VEL = false>
 __global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, const R* __restrict__ Fcur,
                                                           R* __restrict__ Fclr, const uint8_t* __restrict__ type,
                                                           LatticeDev L, CollideArgs a) {
@@ -177,7 +191,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, con
   if (n >= (uint32_t)L.N) return;
   const int64_t S = L.stride;
   const uint8_t ty = GENERAL && type ? type[n] : (uint8_t)HC_NODE_FLUID;
-  if (GENERAL && ty >= HC_NODE_VELOCITY) return;
+  if (GENERAL && !VEL && ty >= HC_NODE_VELOCITY) return;
   const int z = (int)(n % (uint32_t)L.nz);
   const uint32_t t = n / (uint32_t)L.nz;
   const int y = (int)(t % (uint32_t)L.ny);
@@ -193,6 +207,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, con
       fl[i + 9] = tmp;
     }
   } else {
+    if (VEL && ty >= HC_NODE_VELOCITY) velocity_node<R>(fl, ty - HC_NODE_VELOCITY, a);
     const R F0 = R(a.g[0]) + Fcur[n], F1 = R(a.g[1]) + Fcur[S + n], F2 = R(a.g[2]) + Fcur[2 * S + n];
     collide_guo<R>(fl, F0, F1, F2, R(a.omega));
   }
@@ -220,15 +235,7 @@ __global__ __launch_bounds__(64) void k_velocity_nodes(R* __restrict__ f, const 
   const NbrOff o = neighbour_offsets(x, y, z, L);
   R fl[19];
   aa_load<R, ODD, true>(f, S, n, o, fl, std::make_integer_sequence<int, 19>{});
-  // one compile-time instance per domain face (no runtime-indexed lattice tables)
-  switch (type[n] - HC_NODE_VELOCITY) {
-    case 0: velocity_boundary<R, 0, -1>(fl, a.wall_u[0][0], a.wall_u[0][1], a.wall_u[0][2]); break;
-    case 1: velocity_boundary<R, 0, 1>(fl, a.wall_u[1][0], a.wall_u[1][1], a.wall_u[1][2]); break;
-    case 2: velocity_boundary<R, 1, -1>(fl, a.wall_u[2][0], a.wall_u[2][1], a.wall_u[2][2]); break;
-    case 3: velocity_boundary<R, 1, 1>(fl, a.wall_u[3][0], a.wall_u[3][1], a.wall_u[3][2]); break;
-    case 4: velocity_boundary<R, 2, -1>(fl, a.wall_u[4][0], a.wall_u[4][1], a.wall_u[4][2]); break;
-    default: velocity_boundary<R, 2, 1>(fl, a.wall_u[5][0], a.wall_u[5][1], a.wall_u[5][2]); break;
-  }
+  velocity_node<R>(fl, type[n] - HC_NODE_VELOCITY, a);
   const R F0 = R(a.g[0]) + Fcur[n], F1 = R(a.g[1]) + Fcur[S + n], F2 = R(a.g[2]) + Fcur[2 * S + n];
   collide_guo<R>(fl, F0, F1, F2, R(a.omega));
   Fclr[n] = R(0);
@@ -347,9 +354,13 @@ inline dim3 grid_capped(int64_t n) {
 template <typename R>
 hipError_t launch_collide_stream(hipStream_t s, R* f, const R* Fcur, R* Fclr, const uint8_t* type,
                                  const LatticeDev& L, const CollideArgs& a, int odd, bool general,
-                                 const uint32_t* vel_nodes, int n_vel) {
+                                 const uint32_t* vel_nodes, int n_vel, bool fuse_vel) {
   dim3 g = grid_for(L.N);
-  if (general) {
+  if (general && n_vel > 0 && fuse_vel) {
+    // velocity nodes reconstructed inside the bulk kernel: one launch per step
+    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, true, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+    else hipLaunchKernelGGL((k_collide_stream<R, false, true, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+  } else if (general) {
     if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
     else hipLaunchKernelGGL((k_collide_stream<R, false, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
     if (n_vel > 0) {
@@ -407,7 +418,7 @@ hipError_t launch_to_double(hipStream_t s, const R* in, double* out, int64_t n) 
 
 #define HC_INSTANTIATE(R)                                                                                      \
   template hipError_t launch_collide_stream<R>(hipStream_t, R*, const R*, R*, const uint8_t*, const LatticeDev&, \
-                                               const CollideArgs&, int, bool, const uint32_t*, int);          \
+                                               const CollideArgs&, int, bool, const uint32_t*, int, bool);    \
   template hipError_t launch_init_equilibrium<R>(hipStream_t, R*, const uint8_t*, const LatticeDev&, double,    \
                                                  const double*);                                              \
   template hipError_t launch_natural_populations<R>(hipStream_t, const R*, double*, const LatticeDev&, int);    \

@@ -415,3 +415,35 @@ def test_graph_replays_match_eager_steps(monkeypatch):
         fa, fb = eager.populations(), graphed.populations()
         assert cases.rel_l2(fa, fb) < 1e-10, rnd
     assert eager.iteration == graphed.iteration == 33
+
+
+@pytest.mark.parametrize("block", ["256", "512"])
+def test_threads_per_cell_agree_with_the_oracle(monkeypatch, block):
+    """The fused cell kernel at an explicit number of threads per RBC (the
+    suspension's setting) against the oracle; small cases otherwise run with the
+    widest block."""
+    monkeypatch.setenv("HC_CELL_BLOCK", block)
+    u = cases.units()
+    g, o = pair(40, 30, 36, tau=u.tau)
+    _mixed([g, o])
+    for _ in range(3):
+        both(g, o, "step", 10)
+    xg, vg, _ = g.vertices()
+    xo, vo, _ = o.vertices()
+    assert cases.rel_l2(xg, xo) < 1e-10
+    _, ug = g.fluid()
+    _, uo = o.fluid()
+    assert cases.rel_l2(ug, uo) < 1e-10
+
+
+def test_velocity_walls_in_their_own_kernel(monkeypatch):
+    """Velocity-wall nodes reconstructed by the separate kernel (HC_FUSE_VEL=0)
+    against the oracle, as the in-kernel path is by the Couette tests."""
+    monkeypatch.setenv("HC_FUSE_VEL", "0")
+    g, o, x0 = _shear_case(hc.HC_FP64, steps=20)
+    xg, _, _ = g.vertices()
+    xo, _, _ = o.vertices()
+    assert cases.rel_l2(xg, xo) < 1e-10
+    _, ug = g.fluid()
+    _, uo = o.fluid()
+    assert cases.rel_l2(ug, uo) < 1e-10
