@@ -94,8 +94,8 @@ __device__ D3 bending_force(const CellView& c, const TypeDev& T, int i) {
     s2 += p.z;
   }
   const D3 xi = c.pos(i);
-  const double dn = (double)(unsigned)nr;
-  const D3 dev = {s0 / dn - xi.x, s1 / dn - xi.y, s2 / dn - xi.z};
+  const double inv_n = 1.0 / (double)(unsigned)nr;
+  const D3 dev = {s0 * inv_n - xi.x, s1 * inv_n - xi.y, s2 * inv_n - xi.z};
   D3 pn = {0.0, 0.0, 0.0};
   D3 prev = c.pos(R[0]);
   const D3 first = prev;
@@ -104,16 +104,16 @@ __device__ D3 bending_force(const CellView& c, const TypeDev& T, int i) {
     if (j >= nr) break;
     const D3 nxt = (j + 1 < nr) ? c.pos(R[j + 1 < 6 ? j + 1 : 0]) : first;
     D3 tn = crossp(prev - xi, nxt - xi);
-    const double l = normp(tn);
-    pn.x += tn.x / l;
-    pn.y += tn.y / l;
-    pn.z += tn.z / l;
+    const double il = 1.0 / normp(tn);
+    pn.x += tn.x * il;
+    pn.y += tn.y * il;
+    pn.z += tn.z * il;
     prev = nxt;
   }
-  const double l = normp(pn);
-  pn.x /= l;
-  pn.y /= l;
-  pn.z /= l;
+  const double il = 1.0 / normp(pn);
+  pn.x *= il;
+  pn.y *= il;
+  pn.z *= il;
   const double ndev = dotp(pn, dev);
   const double dDev = (ndev - patch_eq) / T.edge_mean_eq;
   const double s = T.k_bend * (dDev + dDev / fabs(MaxCellBendingAngle - dDev * dDev));
@@ -134,6 +134,7 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
   // (all area terms, then all volume terms) up to the grouping of that sum
   {
     double A0 = 0.0, A1 = 0.0, A2 = 0.0, B0 = 0.0, B1 = 0.0, B2 = 0.0;
+    const double inv_area_mean = 1.0 / T.area_mean_eq;
     auto tri_term = [&](const int4 q, const double a_eq) {
       const D3 v0 = c.pos(q.y), v1 = c.pos(q.z), v2 = c.pos(q.w);
       D3 n = crossp(v1 - v0, v2 - v0);
@@ -141,19 +142,22 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
       double area = 0.0;
       if (nn != 0.0) {
         area = 0.5 * nn;
-        n.x /= nn;
-        n.y /= nn;
-        n.z /= nn;
+        const double inn = 1.0 / nn;
+        n.x *= inn;
+        n.y *= inn;
+        n.z *= inn;
       } else {
         n = {0.0, 0.0, 0.0};
       }
       const double ar = (area - a_eq) / a_eq;
       const double afm = T.k_area * (ar + ar / fabs(MaxCellSurfaceAreaChange - ar * ar));
-      const double c0 = (v0.x + v1.x + v2.x) / 3.0, c1 = (v0.y + v1.y + v2.y) / 3.0, c2 = (v0.z + v1.z + v2.z) / 3.0;
+      constexpr double third = 1.0 / 3.0;
+      const double c0 = (v0.x + v1.x + v2.x) * third, c1 = (v0.y + v1.y + v2.y) * third,
+                   c2 = (v0.z + v1.z + v2.z) * third;
       A0 += (c0 - xk.x) * afm;
       A1 += (c1 - xk.y) * afm;
       A2 += (c2 - xk.z) * afm;
-      const double sc = area / T.area_mean_eq;
+      const double sc = area * inv_area_mean;
       B0 += (vforce * n.x) * sc;
       B1 += (vforce * n.y) * sc;
       B2 += (vforce * n.z) * sc;
@@ -185,9 +189,10 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
         F1 += b1;
         F2 += b2;
       } else {
-        F0 += -b0 / dn;
-        F1 += -b1 / dn;
-        F2 += -b2 / dn;
+        const double in = 1.0 / dn;
+        F0 += -b0 * in;
+        F1 += -b1 * in;
+        F2 += -b2 * in;
       }
     };
     if (T.vbendp) {
@@ -213,7 +218,8 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
     const D3 pa = c.pos(a), pb = c.pos(b);
     const D3 ev = pb - pa;
     const double len = normp(ev);
-    const D3 uv = {ev.x / len, ev.y / len, ev.z / len};
+    const double il = 1.0 / len;
+    const D3 uv = {ev.x * il, ev.y * il, ev.z * il};
     if (kind != 2) {
       const double sgn = kind == 0 ? 1.0 : -1.0;
       const double ef = (len - l0) / l0;
