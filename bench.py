@@ -75,6 +75,7 @@ def run_gpu(hc, susp, shape, precision, steps, warmup):
     t0 = time.time()
     sim, units, types, cells = build_case(hc, susp, shape, precision)
     nv = sim.num_vertices
+    x0 = sim.vertices()[0]
     print("[bench] %d cells, %d vertices, Ht %.3f, setup %.1fs" % (
         sum(len(c) for c in cells.values()), nv, susp.hematocrit(types, cells, shape), time.time() - t0),
         file=sys.stderr, flush=True)
@@ -97,11 +98,13 @@ def run_gpu(hc, susp, shape, precision, steps, warmup):
             collide_ms = ms / max(cnt, 1)
     sim.profile_enable(False)
     sim.check_finite()
+    x1 = sim.vertices()[0]
     N = int(np.prod(shape))
     res = dict(label=label, ms_per_step=1e3 * dt / steps, mlups=N * steps / dt / 1e6,
                vertex_updates_per_s=nv * steps / dt, stage_ms=stages, vertices=nv, nodes=N,
                cells={k: int(len(v)) for k, v in cells.items()})
     res["collide_ms"] = collide_ms
+    res["x0"], res["x1"] = x0, x1
     res["collide_gbs"] = BYTES_PER_LUP[label] * N / (stages["collide_stream"] * 1e-3) / 1e9
     print("[bench] %s: %.3f ms/step, %.1f MLUPS, collide %.3f ms = %.0f GB/s" % (
         label, res["ms_per_step"], res["mlups"], res["collide_ms"], res["collide_gbs"]), file=sys.stderr, flush=True)
@@ -236,6 +239,13 @@ def main():
     }
     if "fp64" in res and "fp32" in res:
         f64 = res["fp64"]
+        # FP32-shifted against FP64 after the same steps from the same suspension
+        d32 = res["fp32"]["x1"] - res["fp32"]["x0"]
+        d64 = f64["x1"] - f64["x0"]
+        line["fp32_accuracy"] = {
+            "vertex_position_rel_l2": float(np.linalg.norm(res["fp32"]["x1"] - f64["x1"]) / np.linalg.norm(f64["x1"])),
+            "vertex_displacement_rel_l2": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
+            "steps": args.steps + args.warmup, "reference": "the FP64 form of this build"}
         line["fp64"] = {"value": round(f64["mlups"], 2), "ms_per_step": round(f64["ms_per_step"], 4),
                         "collide_gbs": round(f64["collide_gbs"], 1),
                         "collide_frac": round(f64["collide_gbs"] / HBM_PEAK_GBS, 4),

@@ -78,6 +78,7 @@ SIGNATURES = {
     "hc_step": (_ST, [_P, _INT]),
     "hc_synchronize": (_ST, [_P]),
     "hc_iteration": (ctypes.c_int64, [_P]),
+    "hc_set_iteration": (_ST, [_P, ctypes.c_int64]),
     "hc_stage_spread": (_ST, [_P]),
     "hc_stage_collide_stream": (_ST, [_P]),
     "hc_stage_interpolate": (_ST, [_P]),
@@ -96,6 +97,8 @@ SIGNATURES = {
     "hc_get_vertices": (_ST, [_P, _D, _D, _D]),
     "hc_set_vertices": (_ST, [_P, _D, _D, _D]),
     "hc_get_repulsion_force": (_ST, [_P, _D]),
+    "hc_set_repulsion_force": (_ST, [_P, _D]),
+    "hc_get_cells": (_ST, [_P, _I32, _I32]),
     "hc_get_vertex_bins": (_ST, [_P, _I64]),
     "hc_get_cell_volume_area": (_ST, [_P, _D, _D]),
     "hc_get_cell_stats": (_ST, [_P, _D, _D, _D, _D, _D, _D]),
@@ -224,10 +227,16 @@ class Simulation:
                                             ctypes.byref(params), timescale, ctypes.byref(tid)))
         return tid.value
 
-    def add_cells(self, type_id, xyz):
+    def add_cells(self, type_id, xyz, cell_ids=None):
         x = np.ascontiguousarray(xyz, dtype=np.float64)
         ncells = x.shape[0] if x.ndim == 3 else 1
-        self._chk(self.lib.hc_add_cells(self.h, type_id, ncells, _dp(x), None))
+        ids = None
+        if cell_ids is not None:
+            ids = np.ascontiguousarray(cell_ids, dtype=np.int32)
+            if ids.shape != (ncells,):
+                raise ValueError("one cell id per cell")
+            ids = ids.ctypes.data_as(_I32)
+        self._chk(self.lib.hc_add_cells(self.h, type_id, ncells, _dp(x), ids))
 
     def set_velocity_timescale(self, ts):
         self._chk(self.lib.hc_set_velocity_timescale(self.h, ts))
@@ -265,6 +274,9 @@ class Simulation:
     @property
     def iteration(self):
         return self.lib.hc_iteration(self.h)
+
+    def set_iteration(self, it):
+        self._chk(self.lib.hc_set_iteration(self.h, int(it)))
 
     # state -------------------------------------------------------------------
     @property
@@ -305,6 +317,18 @@ class Simulation:
         fr = np.zeros((self.num_vertices, 3))
         self._chk(self.lib.hc_get_repulsion_force(self.h, _dp(fr)))
         return fr
+
+    def set_repulsion_force(self, fr):
+        fr = np.ascontiguousarray(fr, dtype=np.float64)
+        self._chk(self.lib.hc_set_repulsion_force(self.h, _dp(fr)))
+
+    def cells(self):
+        """(type id, cell id) of every cell, in cell order."""
+        nc = self.lib.hc_num_cells(self.h)
+        t = np.zeros(nc, dtype=np.int32)
+        c = np.zeros(nc, dtype=np.int32)
+        self._chk(self.lib.hc_get_cells(self.h, t.ctypes.data_as(_I32), c.ctypes.data_as(_I32)))
+        return t, c
 
     def vertex_bins(self):
         b = np.zeros(self.num_vertices, dtype=np.int64)

@@ -1024,6 +1024,14 @@ hc_status hc_synchronize(hc_ctx* ctx) {
 
 int64_t hc_iteration(const hc_ctx* ctx) { return ctx ? ctx->iter : -1; }
 
+hc_status hc_set_iteration(hc_ctx* ctx, int64_t iter) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (iter < 0) return fail(ctx, HC_INVALID_ARG, "iteration must be >= 0");
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->iter = iter;
+  return HC_OK;
+}
+
 hc_status hc_stage_spread(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
@@ -1195,6 +1203,24 @@ hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep) {
   if (!ctx || !frep) return HC_INVALID_ARG;
   HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
   return get_planes(ctx, ctx->frep, frep);
+}
+hc_status hc_set_repulsion_force(hc_ctx* ctx, const double* frep) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (!frep) return fail(ctx, HC_INVALID_ARG, "null input");
+  hc_status s;
+  if (ctx->lattice_ready && (s = invalidate_spread(ctx)) != HC_OK) return s;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  return set_planes(ctx, ctx->frep, frep);
+}
+
+hc_status hc_get_cells(hc_ctx* ctx, int32_t* type_ids, int32_t* cell_ids) {
+  if (!ctx) return HC_INVALID_ARG;
+  const size_t nc = ctx->cell_base.size();
+  for (size_t c = 0; c < nc; ++c) {
+    if (type_ids) type_ids[c] = ctx->cell_type[c];
+    if (cell_ids) cell_ids[c] = ctx->cell_id[c];
+  }
+  return HC_OK;
 }
 
 hc_status hc_get_cell_stats(hc_ctx* ctx, double* volume, double* area, double* centroid, double* velocity,

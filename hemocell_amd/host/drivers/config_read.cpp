@@ -1,6 +1,8 @@
 // config_read -- prints values of a HemoCell XML configuration through hemo::Config
 // (no device needed): config_read <file.xml> <a/b/c>...  One line per path: the
 // element's text, or "<missing>" when an element on the path does not exist.
+// config_read <file.xml> --xml prints the <hemocell> element as a checkpoint
+// file stores it.
 #include <cstdio>
 #include <string>
 
@@ -13,6 +15,12 @@ int main(int argc, char* argv[]) {
   }
   try {
     hemo::Config cfg(argv[1]);
+    if (std::string(argv[2]) == "--xml") {
+      const hemo::XMLNode* n = cfg.hemocellNode();
+      if (!n) throw std::invalid_argument("no <hemocell> element");
+      std::printf("%s", hemo::toXml(*n).c_str());
+      return 0;
+    }
     for (int a = 2; a < argc; ++a) {
       std::string path = argv[a];
       size_t p = path.find('/');

@@ -162,6 +162,14 @@ Config::Config(std::string path) {
   checkpointed = !doc->children.empty() && doc->children.front()->name == "Checkpoint";
 }
 
+const XMLNode* Config::hemocellNode() const {
+  if (checkpointed) {
+    const XMLNode* c = doc->child("Checkpoint");
+    return c ? c->child("hemocell") : nullptr;
+  }
+  return doc->child("hemocell");
+}
+
 XMLElement Config::operator[](std::string name) const {
   if (checkpointed) return XMLElement(doc->child("Checkpoint"))["hemocell"][name];
   const XMLNode* root = doc->child("hemocell");

@@ -83,6 +83,10 @@ class HemoCellError : public std::runtime_error {
   explicit HemoCellError(const std::string& m) : std::runtime_error(m) {}
 };
 
+// File-system helpers (the reference's mkpath / file_exists, io/genericFunctions)
+bool exists(const std::string& path);
+void mkpath(const std::string& path);
+
 // --------------------------------------------------------------- Config -----
 // XML configuration access (config/config.h): cfg["domain"]["dx"].read<T>();
 // a missing element throws std::invalid_argument (config/config.cpp:85).
@@ -134,7 +138,14 @@ class Config {
   bool checkpointed = false;
   explicit Config(std::string paramXmlFilename);
   XMLElement operator[](std::string name) const;
+  // the whole document (above <hemocell> or <Checkpoint>)
+  XMLElement document() const { return XMLElement(doc.get()); }
+  // the <hemocell> element, also inside a checkpoint file
+  const XMLNode* hemocellNode() const;
 };
+
+// XML text of a node and its children, indented two spaces per level
+std::string toXml(const XMLNode& node, int indent = 0);
 
 // ----------------------------------------------------------- Parameters -----
 // SI -> lattice conversion (mechanics/constantConversion.{h,cpp}).
@@ -310,8 +321,16 @@ class HemoCell {
   void setInitialMinimumDistanceFromSolid(std::string name, T distance);
   void setSystemPeriodicity(unsigned int axis, bool bePeriodic);
   void loadParticles();
-  void loadCheckPoint() {}
-  void saveCheckPoint() {}
+  // Checkpoints (core/hemoCell.cpp:197-219, core/hemoCellFields.cpp:240-319): the
+  // directory <out>/<parameters><checkpointDirectory> (default <out>/checkpoint/)
+  // holds checkpoint.xml (the configuration wrapped in <Checkpoint> with
+  // <General><Iteration> and <OutDirectory>), lattice.dat (populations) and
+  // particleField.dat (cells and vertex state); the previous set is renamed to
+  // *.old first.  A run started from checkpoint.xml calls loadCheckPoint() in
+  // place of loadParticles() and continues from the saved iteration.
+  void loadCheckPoint();
+  void saveCheckPoint();
+  std::string checkpointDirectory();
   // HDF5 (per cell type and fluid) and CSV output of the current state in the
   // reference's file layout
   void writeOutput();

@@ -24,14 +24,6 @@
 
 namespace hemo {
 
-namespace {
-
-std::string zero_pad(long num, int w = 12) {
-  std::ostringstream ss;
-  ss << std::setw(w) << std::setfill('0') << num;
-  return ss.str();
-}
-
 bool exists(const std::string& p) {
   struct stat st;
   return stat(p.c_str(), &st) == 0;
@@ -44,6 +36,14 @@ void mkpath(const std::string& p) {
     if (p[i] == '/' || i + 1 == p.size())
       if (!cur.empty() && cur != "/" && cur != "./") mkdir(cur.c_str(), 0777);
   }
+}
+
+namespace {
+
+std::string zero_pad(long num, int w = 12) {
+  std::ostringstream ss;
+  ss << std::setw(w) << std::setfill('0') << num;
+  return ss.str();
 }
 
 // 2-D dataset, chunked by up to 1000 rows and deflated (level 7), as the reference writes

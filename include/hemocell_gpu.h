@@ -153,6 +153,9 @@ hc_status hc_synchronize(hc_ctx* ctx);
  * that drive several contexts from one thread. */
 int hc_busy(hc_ctx* ctx);
 int64_t hc_iteration(const hc_ctx* ctx);
+/* Replaces: the iteration read back by HemoCellFields::load (core/hemoCellFields.cpp:245);
+ * the material, velocity and repulsion timescales count from it. */
+hc_status hc_set_iteration(hc_ctx* ctx, int64_t iter);
 
 /* Single stages of iterate(), in its order (parity tests and partial drivers). */
 hc_status hc_stage_spread(hc_ctx* ctx);          /* spreadParticleForce  hemoCellParticleField.cpp:841-863 */
@@ -178,6 +181,14 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u /* 3*N, n*3+c */);
 hc_status hc_get_vertices(hc_ctx* ctx, double* x, double* v, double* f);
 hc_status hc_set_vertices(hc_ctx* ctx, const double* x, const double* v, const double* f);
 hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep);
+/* Replaces: the force_repulsion member restored with the particle field
+ * (HemoCellParticle::serializeValues_t, core/hemoCellParticle.h:45-63); it is kept between
+ * repulsion updates when the repulsion timescale is above one. */
+hc_status hc_set_repulsion_force(hc_ctx* ctx, const double* frep);
+/* cell type id and cell id of every cell, in cell order (vertex blocks follow the same
+ * order); either pointer may be NULL.  Replaces the celltype/cellId members of the
+ * particles (core/hemoCellParticle.h:52-58). */
+hc_status hc_get_cells(hc_ctx* ctx, int32_t* type_ids, int32_t* cell_ids);
 /* nearest-node bin of every vertex, node index z + nz*(y + ny*x) or -1 outside the domain.
  * Replaces: HemoCellParticleField::update_pg (core/hemoCellParticleField.cpp:137-168). */
 hc_status hc_get_vertex_bins(hc_ctx* ctx, int64_t* bins);

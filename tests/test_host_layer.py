@@ -48,7 +48,28 @@ def test_material_numbers_parse(tmp_path):
                           "MaterialModel/empty"]) == ["15.0", "3.91e-6", "1 2", ""]
 
 
-@pytest.mark.parametrize("driver", ["stretch_cell", "load_particles", "config_read"])
+def _tree(el):
+    return (el.tag, (el.text or "").strip(), [_tree(c) for c in el])
+
+
+@pytest.mark.parametrize("name", ["config_stretch_cell.xml", "stretch_RBC.xml"])
+def test_xml_writer_round_trips(name, tmp_path):
+    """The <hemocell> element as saveCheckPoint() writes it into checkpoint.xml
+    (core/hemoCellFields.cpp:270-276) parses back to the same tree, and a
+    <Checkpoint> file routes cfg[...] through its <hemocell> (config/config.cpp:52-75)."""
+    path = os.path.join(GOLDEN, name)
+    r = subprocess.run([os.path.join(BIN, "config_read"), path, "--xml"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert _tree(ET.fromstring(r.stdout)) == _tree(ET.parse(path).getroot())
+    chk = tmp_path / "checkpoint.xml"
+    chk.write_text('<?xml version="1.0" ?>\n<Checkpoint>\n%s<General><Iteration>40</Iteration>'
+                   "<OutDirectory>./tmp/</OutDirectory></General>\n</Checkpoint>\n" % r.stdout)
+    root = ET.parse(path).getroot()
+    keys = ["%s/%s" % (sec.tag, el.tag) for sec in root for el in sec if len(el) == 0]
+    assert _read(str(chk), keys) == _read(path, keys)
+
+
+@pytest.mark.parametrize("driver", ["stretch_cell", "load_particles", "checkpoint_run", "config_read"])
 def test_drivers_print_usage(driver):
     r = subprocess.run([os.path.join(BIN, driver)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
