@@ -395,8 +395,8 @@ hc_status do_spread(hc_ctx* ctx) {
   prof_begin(ctx, HC_PROF_SPREAD, &ev);
   if (ctx->n_extra > 0) HC_TRY(launch_add_extra_force(ctx->stream, vdev(ctx), ctx->d_extra_ids, ctx->d_extra_f, ctx->n_extra), "extra force");
   if (ctx->spread_per_cell) {
-    // LDS budget: room for the support box of a tumbling RBC in FP32, an aligned one in FP64
-    const int want = ctx->spread_lds > 0 ? ctx->spread_lds : (sizeof(R) == 4 ? 40960 : 65536);
+    // LDS budget (a box of doubles): room for the support box of an aligned RBC
+    const int want = ctx->spread_lds > 0 ? ctx->spread_lds : 65536;
     HC_TRY(launch_spread_cells<R>(ctx->stream, vdev(ctx), (R*)ctx->Fibm[ctx->cur], ctx->type, ctx->L, ctx->d_cell_base,
                                   ctx->d_cell_nv, (int)ctx->cell_base.size(), ctx->f_limit, ctx->repulsion ? 1 : 0,
                                   std::min(want, ctx->lds_limit)),

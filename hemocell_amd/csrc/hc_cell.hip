@@ -11,6 +11,8 @@
 //      any extra force, capped to f_limit in place) are spread from the same
 //      positions into the force buffer of the next step, through an LDS box
 //      covering the cell's support and one global atomic per non-zero node.
+//      The box holds doubles in both precisions: LDS double atomic adds run
+//      several times faster than float ones on this chip.
 // Step t+1's spread needs exactly x_t and F(x_t), which steps 1-2 produce, so
 // doing it here saves reloading the cell.  Positions stay in LDS for the whole
 // kernel; per-vertex bending forces and velocities too when they fit.
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
     by = scratch + vs + base;
     bz = scratch + 2 * vs + base;
   }
-  R* box = (R*)region;
+  double* box = region;
 
   // ---- 1. interpolation and advance ------------------------------------------
   for (int k = tid; k < nv; k += BS) {
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
   for (int xs = 0; xs < s0; xs += sx) {
     const int nxp = std::min(sx, s0 - xs);
     const int64_t total = (int64_t)nxp * plane;
-    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = R(0);
+    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = 0.0;
     __syncthreads();
     for (int k = tid; k < 64 * per; k += BS) {
       // lanes of one wavefront take vertices ~nv/64 apart (fewer LDS same-address adds)
@@ -269,15 +271,15 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
         const int lx = sp.lo[0] + (q >> 2) - o0 - xs;
         if (sp.w[q] == 0.0 || lx < 0 || lx >= nxp) continue;
         const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * lx);
-        lds_add(&box[e], (R)(f0 * sp.w[q]));
-        lds_add(&box[total + e], (R)(f1 * sp.w[q]));
-        lds_add(&box[2 * total + e], (R)(f2 * sp.w[q]));
+        lds_add(&box[e], f0 * sp.w[q]);
+        lds_add(&box[total + e], f1 * sp.w[q]);
+        lds_add(&box[2 * total + e], f2 * sp.w[q]);
       }
     }
     __syncthreads();
     for (int64_t e = tid; e < total; e += BS) {
-      const R a = box[e], b = box[total + e], cz = box[2 * total + e];
-      if (a == R(0) && b == R(0) && cz == R(0)) continue;
+      const double a = box[e], b = box[total + e], cz = box[2 * total + e];
+      if (a == 0.0 && b == 0.0 && cz == 0.0) continue;
       const int lz = (int)(e % s2);
       const int64_t r = e / s2;
       const int ly = (int)(r % s1), lx = (int)(r / s1);
@@ -285,9 +287,9 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
       int64_t wc[3];
       for (int q = 0; q < 3; ++q) wc[q] = (g[q] >= 0 && g[q] < dims[q]) ? g[q] : wrap_mod(g[q], dims[q]);
       const int64_t n = wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0]);
-      if (a != R(0)) atomic_add(&Fnext[n], a);
-      if (b != R(0)) atomic_add(&Fnext[S + n], b);
-      if (cz != R(0)) atomic_add(&Fnext[2 * S + n], cz);
+      if (a != 0.0) atomic_add(&Fnext[n], (R)a);
+      if (b != 0.0) atomic_add(&Fnext[S + n], (R)b);
+      if (cz != 0.0) atomic_add(&Fnext[2 * S + n], (R)cz);
     }
     __syncthreads();  // the box is zeroed again by the next slab
   }
@@ -334,7 +336,7 @@ static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& 
     region = std::min((size_t)box_bytes, dev > used ? dev - used : 0);
   }
   if (fixed + region + (staged ? st_staged : st_plain) + margin > dev) return hipErrorInvalidConfiguration;
-  const int box_nodes = (int)(region / (3 * sizeof(R)));
+  const int box_nodes = (int)(region / (3 * sizeof(double)));
   const size_t dyn = fixed + region;
   if (staged)
     hipLaunchKernelGGL((k_cell_step<R, M, B, true>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells, cell_base,

@@ -177,14 +177,15 @@ __device__ __forceinline__ void spread_force(const VertexDev& V, int64_t i, doub
 // in an LDS box covering its support (LDS atomics, many vertices share nodes),
 // then every non-zero node of the box is added to the force field with one
 // global atomic.  A cell whose box does not fit the LDS budget spreads with
-// global atomics directly.
+// global atomics directly.  The box holds doubles in both precisions (LDS
+// double atomic adds are several times faster than float ones here).
 template <typename R, int BS>
 __global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict__ Fibm, const uint8_t* __restrict__ type,
                                                      LatticeDev L, const int64_t* __restrict__ cell_base,
                                                      const int32_t* __restrict__ cell_nv, double f_limit, int use_rep,
                                                      int lds_nodes) {
   extern __shared__ __align__(16) unsigned char smem[];
-  R* box = (R*)smem;
+  double* box = (double*)smem;
   __shared__ int bmin[3], bmax[3], bad;
   const int cell = blockIdx.x, tid = threadIdx.x;
   const int64_t base = cell_base[cell];
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict_
   const int64_t total = (int64_t)s0 * s1 * s2;
   const bool direct = bad || total > lds_nodes || s0 <= 0;
   if (!direct)
-    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = R(0);
+    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = 0.0;
   __syncthreads();
   // lanes of one wavefront take vertices ~nv/64 apart on the mesh, so they rarely
   // add to the same LDS node at once (consecutive mesh vertices share support nodes)
@@ -248,9 +249,9 @@ __global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict_
         atomic_add(&Fibm[2 * S + n], (R)(t2 * sp.w[q]));
       } else {
         const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * (sp.lo[0] + (q >> 2) - o0));
-        lds_add(&box[e], (R)(t0 * sp.w[q]));
-        lds_add(&box[total + e], (R)(t1 * sp.w[q]));
-        lds_add(&box[2 * total + e], (R)(t2 * sp.w[q]));
+        lds_add(&box[e], t0 * sp.w[q]);
+        lds_add(&box[total + e], t1 * sp.w[q]);
+        lds_add(&box[2 * total + e], t2 * sp.w[q]);
       }
     }
   }
@@ -258,8 +259,8 @@ __global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict_
   __syncthreads();
   const int dims[3] = {L.nx, L.ny, L.nz};
   for (int64_t e = tid; e < total; e += BS) {
-    const R a = box[e], b = box[total + e], c = box[2 * total + e];
-    if (a == R(0) && b == R(0) && c == R(0)) continue;
+    const double a = box[e], b = box[total + e], c = box[2 * total + e];
+    if (a == 0.0 && b == 0.0 && c == 0.0) continue;
     const int lz = (int)(e % s2);
     const int64_t r = e / s2;
     const int ly = (int)(r % s1), lx = (int)(r / s1);
@@ -267,9 +268,9 @@ __global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict_
     int64_t wc[3];
     for (int q = 0; q < 3; ++q) wc[q] = (g[q] >= 0 && g[q] < dims[q]) ? g[q] : wrap_mod(g[q], dims[q]);
     const int64_t n = wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0]);
-    if (a != R(0)) atomic_add(&Fibm[n], a);
-    if (b != R(0)) atomic_add(&Fibm[S + n], b);
-    if (c != R(0)) atomic_add(&Fibm[2 * S + n], c);
+    if (a != 0.0) atomic_add(&Fibm[n], (R)a);
+    if (b != 0.0) atomic_add(&Fibm[S + n], (R)b);
+    if (c != 0.0) atomic_add(&Fibm[2 * S + n], (R)c);
   }
 }
 
@@ -300,7 +301,7 @@ hipError_t launch_spread_cells(hipStream_t s, const VertexDev& V, R* Fibm, const
                                const int64_t* cell_base, const int32_t* cell_nv, int ncells, double f_limit,
                                int use_rep, int lds_bytes) {
   if (ncells <= 0) return hipSuccess;
-  const int lds_nodes = lds_bytes / (3 * (int)sizeof(R));
+  const int lds_nodes = lds_bytes / (3 * (int)sizeof(double));
   hipLaunchKernelGGL((k_spread_cells<R, 256>), dim3((unsigned)ncells), dim3(256), (size_t)lds_bytes, s, V, Fibm, type,
                      L, cell_base, cell_nv, f_limit, use_rep, lds_nodes);
   return hipGetLastError();

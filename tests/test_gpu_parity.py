@@ -391,7 +391,10 @@ def test_graph_replays_match_eager_steps(monkeypatch):
     forces a new capture) in between.  The spread's float atomics commit in no fixed
     order, so two eager runs already differ at round-off: the graphed run must stay
     within the fused-step tolerances of the eager one, and as close to it as a second
-    eager run is, up to a factor."""
+    eager run is, up to a factor.  Two eager runs often commit in the same order and
+    agree to the last bit, while the graph's parallel branches commit in another; one
+    force sum differing in its last bit grows over the coupled steps to ~1e-10 in the
+    vertex velocities (8e-11 observed), hence a floor of 1e-9 on that ratio test."""
     u = cases.units()
     sims = []
     for graph in ("0", "0", "1"):
@@ -411,7 +414,7 @@ def test_graph_replays_match_eager_steps(monkeypatch):
         (xa, va, _), (x2, v2, _), (xb, vb, _) = eager.vertices(), eager2.vertices(), graphed.vertices()
         assert cases.rel_l2(xa, xb) < 1e-12, rnd
         assert cases.rel_l2(va, vb) < 1e-8, rnd
-        assert cases.rel_l2(va, vb) <= 100 * max(cases.rel_l2(va, v2), 1e-14), rnd
+        assert cases.rel_l2(va, vb) <= max(100 * cases.rel_l2(va, v2), 1e-9), rnd
         fa, fb = eager.populations(), graphed.populations()
         assert cases.rel_l2(fa, fb) < 1e-10, rnd
     assert eager.iteration == graphed.iteration == 33
