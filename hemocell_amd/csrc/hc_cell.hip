@@ -23,6 +23,8 @@ namespace hcg {
 namespace {
 
 enum { CELL_INTERP = 1, CELL_MECH = 2, CELL_SPREAD = 4, CELL_REPULSION = 8 };
+// x-planes of a cell's support box up to which the spread buckets its vertices
+constexpr int kMaxPlanes = 64;
 
 template <typename R, int MODEL, int BS, bool STAGED>
 __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
@@ -35,6 +37,7 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
   __shared__ double red[BS];
   __shared__ double vforce_sh;
   __shared__ int bmin[3], bmax[3], bad;
+  __shared__ int pcount[kMaxPlanes + 1], pstart[kMaxPlanes + 1];
   const int cell = cells[blockIdx.x];
   const int64_t base = cell_base[cell];
   const int tid = threadIdx.x;
@@ -66,6 +69,8 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
     bz = scratch + 2 * vs + base;
   }
   double* box = region;
+  // after the box: the spread's vertex list, bucketed by x-plane
+  uint16_t* vlist = reinterpret_cast<uint16_t*>(region + 3 * (int64_t)box_nodes);
 
   // ---- 1. interpolation and advance ------------------------------------------
   for (int k = tid; k < nv; k += BS) {
@@ -161,6 +166,9 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
   if (!(flags & CELL_SPREAD)) return;
 
   // ---- 3. spread of the next step ---------------------------------------------
+  // the force each vertex spreads: its force plus any extra force, capped to
+  // f_limit and written back in place when that changed it
+  // (hemoCellParticleField.cpp:846-852), plus the repulsion force when enabled
   if (tid == 0) {
     for (int a = 0; a < 3; ++a) {
       bmin[a] = 0x7fffffff;
@@ -183,29 +191,8 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
         lmin[a] = min(lmin[a], cc);
         lmax[a] = max(lmax[a], cc);
       }
-    }
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&bmin[a], lmin[a]);
-      atomicMax(&bmax[a], lmax[a]);
-    }
-    if (lbad) atomicOr(&bad, 1);
-  }
-  __syncthreads();
-  const int o0 = bmin[0], o1 = bmin[1], o2 = bmin[2];
-  const int s0 = bmax[0] - bmin[0] + 2, s1 = bmax[1] - bmin[1] + 2, s2 = bmax[2] - bmin[2] + 2;
-  const int64_t plane = (int64_t)s1 * s2;
-  // the box is spread in slabs of whole x-planes when it does not fit at once;
-  // only a cell whose single x-plane does not fit uses global atomics per vertex
-  const bool direct = bad || s0 <= 0 || plane > box_nodes;
-  const int64_t S = L.stride;
-  const int per = (nv + 63) / 64;
-  // the force each vertex spreads: its force plus any extra force, capped to
-  // f_limit and written back in place when that changed it (hemoCellParticleField.cpp:846-852)
-  auto spread_force = [&](int64_t i, double& f0, double& f1, double& f2, bool prepare) {
-    f0 = V.f[i];
-    f1 = V.f[vs + i];
-    f2 = V.f[2 * vs + i];
-    if (prepare) {
+      const int64_t i = base + k;
+      double f0 = V.f[i], f1 = V.f[vs + i], f2 = V.f[2 * vs + i];
       bool store = false;
       if (extra) {
         f0 += extra[i];
@@ -227,6 +214,24 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
         V.f[2 * vs + i] = f2;
       }
     }
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&bmin[a], lmin[a]);
+      atomicMax(&bmax[a], lmax[a]);
+    }
+    if (lbad) atomicOr(&bad, 1);
+  }
+  __syncthreads();
+  const int o0 = bmin[0], o1 = bmin[1], o2 = bmin[2];
+  const int s0 = bmax[0] - bmin[0] + 2, s1 = bmax[1] - bmin[1] + 2, s2 = bmax[2] - bmin[2] + 2;
+  const int64_t plane = (int64_t)s1 * s2;
+  // the box is spread in slabs of whole x-planes when it does not fit at once;
+  // only a cell whose single x-plane does not fit uses global atomics per vertex
+  const bool direct = bad || s0 <= 0 || plane > box_nodes;
+  const int64_t S = L.stride;
+  auto spread_force = [&](int64_t i, double& f0, double& f1, double& f2) {
+    f0 = V.f[i];
+    f1 = V.f[vs + i];
+    f2 = V.f[2 * vs + i];
     if (flags & CELL_REPULSION) {
       f0 = V.frep[i] + f0;
       f1 = V.frep[vs + i] + f1;
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
     for (int k = tid; k < nv; k += BS) {
       const int64_t i = base + k;
       double f0, f1, f2;
-      spread_force(i, f0, f1, f2, true);
+      spread_force(i, f0, f1, f2);
       Support sp;
       ibm_support(px[k], py[k], pz[k], L, type, sp);
 #pragma unroll
@@ -252,18 +257,45 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
     return;
   }
   const int sx = (int)std::min<int64_t>(s0, box_nodes / plane);  // x-planes per slab
+  // With several slabs, the vertices are bucketed by the x-plane of their
+  // support's low corner (a vertex adds to that plane and the next), so a slab
+  // visits only the vertices that reach it.
+  const bool bucketed = sx < s0 && s0 <= kMaxPlanes && nv <= 65535;
+  if (bucketed) {
+    for (int b = tid; b <= s0; b += BS) pcount[b] = 0;
+    __syncthreads();
+    for (int k = tid; k < nv; k += BS) atomicAdd(&pcount[(int)floor(px[k]) - o0], 1);
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int b = 0; b <= s0; ++b) {
+        const int c = pcount[b];
+        pstart[b] = acc;
+        pcount[b] = acc;  // becomes the insertion cursor
+        acc += c;
+      }
+    }
+    __syncthreads();
+    for (int k = tid; k < nv; k += BS) vlist[atomicAdd(&pcount[(int)floor(px[k]) - o0], 1)] = (uint16_t)k;
+    __syncthreads();
+  }
   const int dims[3] = {L.nx, L.ny, L.nz};
   for (int xs = 0; xs < s0; xs += sx) {
     const int nxp = std::min(sx, s0 - xs);
     const int64_t total = (int64_t)nxp * plane;
     for (int64_t e = tid; e < 3 * total; e += BS) box[e] = 0.0;
+    // the vertices to visit: a range of the bucketed list, or all of them
+    const int first = bucketed ? pstart[max(xs - 1, 0)] : 0;
+    const int count = bucketed ? pstart[min(xs + nxp, s0)] - first : nv;
     __syncthreads();
+    const int per = (count + 63) / 64;
     for (int k = tid; k < 64 * per; k += BS) {
-      // lanes of one wavefront take vertices ~nv/64 apart (fewer LDS same-address adds)
-      const int vtx = (k & 63) * per + (k >> 6);
-      if (vtx >= nv) continue;
+      // lanes of one wavefront take vertices ~count/64 apart (fewer LDS same-address adds)
+      const int j = (k & 63) * per + (k >> 6);
+      if (j >= count) continue;
+      const int vtx = bucketed ? (int)vlist[first + j] : j;
       double f0, f1, f2;
-      spread_force(base + vtx, f0, f1, f2, xs == 0);
+      spread_force(base + vtx, f0, f1, f2);
       Support sp;
       ibm_support(px[vtx], py[vtx], pz[vtx], L, type, sp);
 #pragma unroll
@@ -316,7 +348,8 @@ static size_t static_lds(K kernel) {
 
 // Lay out the dynamic LDS of one cell type for block size B and launch: staged
 // (positions, velocities and a region shared by bending forces and the box) when
-// it fits beside the kernel's static LDS, else positions plus as much box as fits.
+// it fits beside the kernel's static LDS, else positions plus as much box as
+// fits; the spread's vertex list (2 bytes per vertex) follows the region.
 template <typename R, int M, int B>
 static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                               const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L, R* Fnext,
@@ -326,18 +359,19 @@ static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& 
   static const size_t st_plain = static_lds(k_cell_step<R, M, B, false>);
   const size_t margin = 256, dev = (size_t)lds_device;
   const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * sizeof(double) : 0;
+  const size_t lists = (2 * (size_t)T.nv + 15) & ~(size_t)15;  // the spread's bucketed vertex list
   bool staged = true;
   size_t fixed = cell_lds_fixed(T, true);
   size_t region = std::max(bend, (size_t)box_bytes);
-  if (fixed + region + st_staged + margin > dev) {
+  if (fixed + region + lists + st_staged + margin > dev) {
     staged = false;
     fixed = cell_lds_fixed(T, false);
-    const size_t used = fixed + st_plain + margin;
+    const size_t used = fixed + lists + st_plain + margin;
     region = std::min((size_t)box_bytes, dev > used ? dev - used : 0);
   }
-  if (fixed + region + (staged ? st_staged : st_plain) + margin > dev) return hipErrorInvalidConfiguration;
+  if (fixed + region + lists + (staged ? st_staged : st_plain) + margin > dev) return hipErrorInvalidConfiguration;
   const int box_nodes = (int)(region / (3 * sizeof(double)));
-  const size_t dyn = fixed + region;
+  const size_t dyn = fixed + region + lists;
   if (staged)
     hipLaunchKernelGGL((k_cell_step<R, M, B, true>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells, cell_base,
                        uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
