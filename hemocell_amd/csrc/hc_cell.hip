@@ -73,59 +73,83 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
   uint16_t* vlist = reinterpret_cast<uint16_t*>(region + 3 * (int64_t)box_nodes);
 
   // ---- 1. interpolation and advance ------------------------------------------
-  for (int k = tid; k < nv; k += BS) {
-    const int64_t i = base + k;
-    double x0 = V.x[i], x1 = V.x[vs + i], x2 = V.x[2 * vs + i];
-    double v0, v1, v2;
-    if (flags & CELL_INTERP) {
-      Support sp;
-      ibm_support(x0, x1, x2, L, type, sp);
-      v0 = 0.0;
-      v1 = 0.0;
-      v2 = 0.0;
+  // A thread takes up to U of its vertices at once, so their position loads and
+  // support-node gathers are in flight together.
+  constexpr int U = 3;
+  for (int k0 = tid; k0 < nv; k0 += U * BS) {
+    double xa[U][3], va[U][3];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if (sp.w[q] == 0.0) continue;
-        const R* u = uf + 4 * (int64_t)sp.idx[q];
-        v0 += (double)u[0] * sp.w[q];
-        v1 += (double)u[1] * sp.w[q];
-        v2 += (double)u[2] * sp.w[q];
-      }
-      V.v[i] = v0;
-      V.v[vs + i] = v1;
-      V.v[2 * vs + i] = v2;
-    } else {
-      v0 = V.v[i];
-      v1 = V.v[vs + i];
-      v2 = V.v[2 * vs + i];
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * BS;
+      if (k >= nv) continue;
+      const int64_t i = base + k;
+      xa[u][0] = V.x[i];
+      xa[u][1] = V.x[vs + i];
+      xa[u][2] = V.x[2 * vs + i];
     }
-    x0 += v0;
-    x1 += v1;
-    x2 += v2;
-    V.x[i] = x0;
-    V.x[vs + i] = x1;
-    V.x[2 * vs + i] = x2;
-    px[k] = x0;
-    py[k] = x1;
-    pz[k] = x2;
-    if (STAGED && visc) {
-      vx[k] = v0;
-      vy[k] = v1;
-      vz[k] = v2;
-    }
-    if (type && isfinite(x0) && isfinite(x1) && isfinite(x2)) {
-      const double p[3] = {x0, x1, x2};
-      const int dims[3] = {L.nx, L.ny, L.nz};
-      int64_t wc[3];
-      bool out = false;
-      for (int a = 0; a < 3; ++a) {
-        const int64_t cc = (int64_t)floor(p[a] + 0.5);
-        if (cc >= 0 && cc < dims[a]) wc[a] = cc;
-        else if (L.periodic[a]) wc[a] = wrap_mod(cc, dims[a]);
-        else out = true;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * BS;
+      if (k >= nv) continue;
+      const int64_t i = base + k;
+      if (flags & CELL_INTERP) {
+        Support sp;
+        ibm_support(xa[u][0], xa[u][1], xa[u][2], L, type, sp);
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (sp.w[q] == 0.0) continue;
+          const R* un = uf + 4 * (int64_t)sp.idx[q];
+          v0 += (double)un[0] * sp.w[q];
+          v1 += (double)un[1] * sp.w[q];
+          v2 += (double)un[2] * sp.w[q];
+        }
+        va[u][0] = v0;
+        va[u][1] = v1;
+        va[u][2] = v2;
+      } else {
+        va[u][0] = V.v[i];
+        va[u][1] = V.v[vs + i];
+        va[u][2] = V.v[2 * vs + i];
       }
-      if (!out && type[wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0])] != HC_NODE_FLUID)
-        atomicAdd(wall_hits, 1ull);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * BS;
+      if (k >= nv) continue;
+      const int64_t i = base + k;
+      const double v0 = va[u][0], v1 = va[u][1], v2 = va[u][2];
+      if (flags & CELL_INTERP) {
+        V.v[i] = v0;
+        V.v[vs + i] = v1;
+        V.v[2 * vs + i] = v2;
+      }
+      const double x0 = xa[u][0] + v0, x1 = xa[u][1] + v1, x2 = xa[u][2] + v2;
+      V.x[i] = x0;
+      V.x[vs + i] = x1;
+      V.x[2 * vs + i] = x2;
+      px[k] = x0;
+      py[k] = x1;
+      pz[k] = x2;
+      if (STAGED && visc) {
+        vx[k] = v0;
+        vy[k] = v1;
+        vz[k] = v2;
+      }
+      if (type && isfinite(x0) && isfinite(x1) && isfinite(x2)) {
+        const double p[3] = {x0, x1, x2};
+        const int dims[3] = {L.nx, L.ny, L.nz};
+        int64_t wc[3];
+        bool out = false;
+        for (int a = 0; a < 3; ++a) {
+          const int64_t cc = (int64_t)floor(p[a] + 0.5);
+          if (cc >= 0 && cc < dims[a]) wc[a] = cc;
+          else if (L.periodic[a]) wc[a] = wrap_mod(cc, dims[a]);
+          else out = true;
+        }
+        if (!out && type[wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0])] != HC_NODE_FLUID)
+          atomicAdd(wall_hits, 1ull);
+      }
     }
   }
   __syncthreads();

@@ -394,7 +394,8 @@ def test_graph_replays_match_eager_steps(monkeypatch):
     eager run is, up to a factor.  Two eager runs often commit in the same order and
     agree to the last bit, while the graph's parallel branches commit in another; one
     force sum differing in its last bit grows over the coupled steps to ~1e-10 in the
-    vertex velocities (8e-11 observed), hence a floor of 1e-9 on that ratio test."""
+    vertex velocities and the populations (8e-11 and 1.1e-10 observed), hence 1e-9
+    for both; a graph replaying a stale argument would be off by far more."""
     u = cases.units()
     sims = []
     for graph in ("0", "0", "1"):
@@ -416,7 +417,7 @@ def test_graph_replays_match_eager_steps(monkeypatch):
         assert cases.rel_l2(va, vb) < 1e-8, rnd
         assert cases.rel_l2(va, vb) <= max(100 * cases.rel_l2(va, v2), 1e-9), rnd
         fa, fb = eager.populations(), graphed.populations()
-        assert cases.rel_l2(fa, fb) < 1e-10, rnd
+        assert cases.rel_l2(fa, fb) < 1e-9, rnd
     assert eager.iteration == graphed.iteration == 33
 
 
