@@ -79,6 +79,7 @@ struct hc_ctx {
   int lds_device = 65536;       // LDS per workgroup of the device (static + dynamic)
   bool spread_per_cell = true;  // LDS-staged spread (false: one thread per vertex, global atomics)
   int spread_lds = 0;           // LDS box budget in bytes (0: default per precision)
+  bool side_gpos = true;        // large-cell side types read positions from global memory
   int cell_block = 256;         // threads per RBC cell in the fused vertex kernel
   int wbc_block = 1024;         // threads per WBC cell (falls back when the LDS does not fit)
   bool cell_block_env = false;  // the RBC block was set explicitly
@@ -361,8 +362,10 @@ hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
     HC_TRY(hipEventRecord(ctx->fork_ev, ctx->stream), "hipEventRecord");
     for (int j = 0; j < nside; ++j) HC_TRY(hipStreamWaitEvent(ctx->side[j], ctx->fork_ev, 0), "hipStreamWaitEvent");
   }
-  // the side kernels are launched first: their few, long workgroups (the WBCs)
-  // should not wait for the many short ones of the main type to drain
+  // The side kernels are launched first.  A side type with large cells (the
+  // 2562-vertex WBCs) keeps its positions in global memory and runs 256 threads
+  // per cell: a workgroup staging them in LDS needs a whole compute unit and
+  // would wait until the main type's stream of small workgroups drains.
   for (size_t kk = 0; kk < act.size(); ++kk) {
     const size_t k = (kk + 1) % act.size();
     TypeBufs& tb = *act[k];
@@ -376,9 +379,11 @@ hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
     // few cells: their latency, not the occupancy, sets the time: the widest block
     int block = tb.host.model == HC_MODEL_WBC ? ctx->wbc_block : ctx->cell_block;
     if (tb.ncells < ctx->n_cu && !(ctx->cell_block_env && tb.host.model == HC_MODEL_RBC)) block = 1024;
+    const bool gpos = k != 0 && nside > 0 && ctx->side_gpos && 3 * (size_t)tb.host.nv * sizeof(double) > 32768;
+    if (gpos) block = 256;
     HC_TRY(launch_cell_step<R>(st, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base, (const R*)ctx->ufield,
                                ctx->type, ctx->L, (R*)ctx->Fibm[1 - ctx->cur], ctx->d_extra_dense, ctx->bend,
-                               ctx->f_limit, flags, box, ctx->lds_device, ctx->d_wall_hits, block),
+                               ctx->f_limit, flags, box, ctx->lds_device, ctx->d_wall_hits, block, gpos),
            "cell_step");
   }
   for (int j = 0; j < nside; ++j) {
@@ -545,6 +550,7 @@ std::vector<unsigned char> launch_signature(const hc_ctx* c) {
   put(v, c->wbc_block);
   put(v, c->n_cu);
   put(v, c->fuse_vel);
+  put(v, c->side_gpos);
   put(v, c->pv_timescale);
   for (const auto& tp : c->types) {
     put(v, tp->dev);
@@ -700,6 +706,8 @@ hc_status hc_create(int device, hc_ctx** out) {
   if (gr && std::string(gr) == "0") ctx->use_graphs = false;
   const char* sl = std::getenv("HC_SPREAD_LDS");  // tuning: LDS bytes of the per-cell box
   if (sl) ctx->spread_lds = std::atoi(sl);
+  const char* sg = std::getenv("HC_SIDE_GPOS");  // tuning: 0 stages every type's positions in LDS
+  if (sg) ctx->side_gpos = std::atoi(sg) != 0;
   *out = ctx;
   return HC_OK;
 }

@@ -26,7 +26,11 @@ enum { CELL_INTERP = 1, CELL_MECH = 2, CELL_SPREAD = 4, CELL_REPULSION = 8 };
 // x-planes of a cell's support box up to which the spread buckets its vertices
 constexpr int kMaxPlanes = 64;
 
-template <typename R, int MODEL, int BS, bool STAGED>
+// STAGED: positions, velocities and bending forces in LDS.  Otherwise positions
+// in LDS and the rest in global memory, or (GPOS) the positions too: then the
+// workgroup needs LDS only for the spread box and fits beside the many small
+// workgroups of another cell type running at the same time.
+template <typename R, int MODEL, int BS, bool STAGED, bool GPOS>
 __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
                                                   const int64_t* __restrict__ cell_base, const R* __restrict__ uf,
                                                   const uint8_t* __restrict__ type, LatticeDev L, R* __restrict__ Fnext,
@@ -46,9 +50,9 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
   const bool visc = (MODEL != HC_MODEL_RBC) || (T.eta_m != 0.0);
   // LDS: positions, then (STAGED) velocities, then a region shared by the bending
   // forces (phase 2) and the spread box (phase 3)
-  double* px = sm;
-  double* py = sm + nv;
-  double* pz = sm + 2 * nv;
+  double* px = GPOS ? V.x + base : sm;
+  double* py = GPOS ? V.x + vs + base : sm + nv;
+  double* pz = GPOS ? V.x + 2 * vs + base : sm + 2 * nv;
   double *vx, *vy, *vz, *bx, *by, *bz;
   double* region;
   if (STAGED) {
@@ -63,7 +67,7 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
     vx = V.v + base;
     vy = V.v + vs + base;
     vz = V.v + 2 * vs + base;
-    region = sm + 3 * nv;
+    region = GPOS ? sm : sm + 3 * nv;
     bx = scratch + base;
     by = scratch + vs + base;
     bz = scratch + 2 * vs + base;
@@ -128,9 +132,11 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
       V.x[i] = x0;
       V.x[vs + i] = x1;
       V.x[2 * vs + i] = x2;
-      px[k] = x0;
-      py[k] = x1;
-      pz[k] = x2;
+      if (!GPOS) {
+        px[k] = x0;
+        py[k] = x1;
+        pz[k] = x2;
+      }
       if (STAGED && visc) {
         vx[k] = v0;
         vy[k] = v1;
@@ -354,8 +360,9 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
 }  // namespace
 
 // Fixed LDS of the kernel (bytes) for a type: positions, plus velocities and bending
-// forces when staged.
-static size_t cell_lds_fixed(const TypeDev& T, bool staged) {
+// forces when staged; nothing with the positions in global memory.
+static size_t cell_lds_fixed(const TypeDev& T, bool staged, bool gpos) {
+  if (gpos) return 0;
   const bool visc = (T.model != HC_MODEL_RBC) || (T.eta_m != 0.0);
   const size_t nv = (size_t)T.nv;
   if (!staged) return 3 * nv * sizeof(double);
@@ -373,35 +380,41 @@ static size_t static_lds(K kernel) {
 // Lay out the dynamic LDS of one cell type for block size B and launch: staged
 // (positions, velocities and a region shared by bending forces and the box) when
 // it fits beside the kernel's static LDS, else positions plus as much box as
-// fits; the spread's vertex list (2 bytes per vertex) follows the region.
+// fits; with gpos only the box.  The spread's vertex list (2 bytes per vertex)
+// follows the region.
 template <typename R, int M, int B>
 static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                               const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L, R* Fnext,
                               const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
-                              int lds_device, unsigned long long* wall_hits) {
-  static const size_t st_staged = static_lds(k_cell_step<R, M, B, true>);
-  static const size_t st_plain = static_lds(k_cell_step<R, M, B, false>);
+                              int lds_device, unsigned long long* wall_hits, bool gpos) {
+  static const size_t st_staged = static_lds(k_cell_step<R, M, B, true, false>);
+  static const size_t st_plain = static_lds(k_cell_step<R, M, B, false, false>);
+  static const size_t st_gpos = static_lds(k_cell_step<R, M, B, false, true>);
   const size_t margin = 256, dev = (size_t)lds_device;
   const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * sizeof(double) : 0;
   const size_t lists = (2 * (size_t)T.nv + 15) & ~(size_t)15;  // the spread's bucketed vertex list
-  bool staged = true;
-  size_t fixed = cell_lds_fixed(T, true);
-  size_t region = std::max(bend, (size_t)box_bytes);
-  if (fixed + region + lists + st_staged + margin > dev) {
+  bool staged = !gpos;
+  size_t fixed = cell_lds_fixed(T, staged, gpos);
+  size_t region = staged ? std::max(bend, (size_t)box_bytes) : (size_t)box_bytes;
+  if (staged && fixed + region + lists + st_staged + margin > dev) {
     staged = false;
-    fixed = cell_lds_fixed(T, false);
+    fixed = cell_lds_fixed(T, false, false);
     const size_t used = fixed + lists + st_plain + margin;
     region = std::min((size_t)box_bytes, dev > used ? dev - used : 0);
   }
-  if (fixed + region + lists + (staged ? st_staged : st_plain) + margin > dev) return hipErrorInvalidConfiguration;
+  const size_t st = staged ? st_staged : (gpos ? st_gpos : st_plain);
+  if (fixed + region + lists + st + margin > dev) return hipErrorInvalidConfiguration;
   const int box_nodes = (int)(region / (3 * sizeof(double)));
   const size_t dyn = fixed + region + lists;
   if (staged)
-    hipLaunchKernelGGL((k_cell_step<R, M, B, true>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells, cell_base,
-                       uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
+    hipLaunchKernelGGL((k_cell_step<R, M, B, true, false>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells,
+                       cell_base, uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
+  else if (gpos)
+    hipLaunchKernelGGL((k_cell_step<R, M, B, false, true>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells,
+                       cell_base, uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
   else
-    hipLaunchKernelGGL((k_cell_step<R, M, B, false>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells, cell_base,
-                       uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
+    hipLaunchKernelGGL((k_cell_step<R, M, B, false, false>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells,
+                       cell_base, uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
   return hipGetLastError();
 }
 
@@ -409,7 +422,7 @@ template <typename R>
 hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                             const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L,
                             R* Fnext, const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
-                            int lds_device, unsigned long long* wall_hits, int block) {
+                            int lds_device, unsigned long long* wall_hits, int block, bool gpos) {
   if (ncells <= 0) return hipSuccess;
   // threads per cell for the RBC and WBC meshes: 256, 512 or 1024; a block whose
   // static LDS leaves too little room for the cell falls back to a smaller one
@@ -417,7 +430,7 @@ hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T,
 #define HC_CELL(M, B)                                                                                                \
   if (e == hipErrorInvalidConfiguration)                                                                             \
     e = launch_cell<R, M, B>(s, V, T, cells, ncells, cell_base, uf, type, L, Fnext, extra, scratch, f_limit, flags, \
-                             box_bytes, lds_device, wall_hits);
+                             box_bytes, lds_device, wall_hits, gpos);
   switch (T.model) {
     case HC_MODEL_RBC:
       if (block >= 1024) HC_CELL(HC_MODEL_RBC, 1024)
@@ -440,9 +453,11 @@ hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T,
 
 template hipError_t launch_cell_step<float>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
                                             const int64_t*, const float*, const uint8_t*, const LatticeDev&, float*,
-                                            const double*, double*, double, int, int, int, unsigned long long*, int);
+                                            const double*, double*, double, int, int, int, unsigned long long*, int,
+                                            bool);
 template hipError_t launch_cell_step<double>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
                                              const int64_t*, const double*, const uint8_t*, const LatticeDev&, double*,
-                                             const double*, double*, double, int, int, int, unsigned long long*, int);
+                                             const double*, double*, double, int, int, int, unsigned long long*, int,
+                                             bool);
 
 }  // namespace hcg

@@ -61,7 +61,7 @@ template <typename R>
 hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                             const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L,
                             R* Fnext, const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
-                            int lds_device, unsigned long long* wall_hits, int block);
+                            int lds_device, unsigned long long* wall_hits, int block, bool gpos = false);
 
 // binning and cell-cell repulsion (hc_repulsion.hip)
 struct RepulsionBufs {

@@ -421,6 +421,27 @@ def test_graph_replays_match_eager_steps(monkeypatch):
     assert eager.iteration == graphed.iteration == 33
 
 
+@pytest.mark.parametrize("precision", [hc.HC_FP64, hc.HC_FP32_SHIFTED])
+def test_wbc_beside_rbc_with_positions_in_global_memory(monkeypatch, precision):
+    """Beside the RBC kernel, the WBC kernel keeps its positions in global memory
+    (256 threads per cell) instead of LDS.  Only where the positions live differs,
+    so a step of each way agrees to the last bit; the bound absorbs only a different
+    commit order of the first spread's float atomics at nodes shared by three or
+    more cells (relative L2 1e-13 in FP64, 1e-6 in FP32)."""
+    u = cases.units()
+    out = []
+    for gpos in ("1", "0"):
+        monkeypatch.setenv("HC_SIDE_GPOS", gpos)
+        g = hc.Simulation(0)
+        g.lattice_init(40, 30, 36, u.tau, precision=precision)
+        _mixed([g])
+        g.step(3)
+        out.append(g.vertices())
+    tol = 1e-13 if precision == hc.HC_FP64 else 1e-6
+    for a, b in zip(out[0], out[1]):
+        assert cases.rel_l2(a, b) <= tol
+
+
 @pytest.mark.parametrize("block", ["256", "512"])
 def test_threads_per_cell_agree_with_the_oracle(monkeypatch, block):
     """The fused cell kernel at an explicit number of threads per RBC (the
