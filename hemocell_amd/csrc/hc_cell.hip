@@ -312,8 +312,9 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
   const int dims[3] = {L.nx, L.ny, L.nz};
   for (int xs = 0; xs < s0; xs += sx) {
     const int nxp = std::min(sx, s0 - xs);
-    const int64_t total = (int64_t)nxp * plane;
-    for (int64_t e = tid; e < 3 * total; e += BS) box[e] = 0.0;
+    // the slab fits the box (plane <= box_nodes), so 32-bit indices suffice
+    const int total = nxp * (int)plane;
+    for (int e = tid; e < 3 * total; e += BS) box[e] = 0.0;
     // the vertices to visit: a range of the bucketed list, or all of them
     const int first = bucketed ? pstart[max(xs - 1, 0)] : 0;
     const int count = bucketed ? pstart[min(xs + nxp, s0)] - first : nv;
@@ -339,12 +340,14 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
       }
     }
     __syncthreads();
-    for (int64_t e = tid; e < total; e += BS) {
+    const unsigned us1 = (unsigned)s1, us2 = (unsigned)s2;
+    for (int e = tid; e < total; e += BS) {
       const double a = box[e], b = box[total + e], cz = box[2 * total + e];
       if (a == 0.0 && b == 0.0 && cz == 0.0) continue;
-      const int lz = (int)(e % s2);
-      const int64_t r = e / s2;
-      const int ly = (int)(r % s1), lx = (int)(r / s1);
+      const unsigned r = (unsigned)e / us2;
+      const int lz = (int)((unsigned)e - r * us2);
+      const unsigned r2 = r / us1;
+      const int ly = (int)(r - r2 * us1), lx = (int)r2;
       const int64_t g[3] = {(int64_t)o0 + xs + lx, (int64_t)o1 + ly, (int64_t)o2 + lz};
       int64_t wc[3];
       for (int q = 0; q < 3; ++q) wc[q] = (g[q] >= 0 && g[q] < dims[q]) ? g[q] : wrap_mod(g[q], dims[q]);
