@@ -51,12 +51,33 @@ def profile_traffic(label):
     return sum(tot) / len(tot) if tot else None
 
 
-def build_case(hc, susp, shape, precision, seed=20261202):
+# the two one-GPU workloads: the Ht 30% periodic suspension (the headline) and the
+# largest domain, a 512x512x1024 channel between bounce-back planes y = 0 and
+# y = 511 at Ht 35% (RBC and PLT 1:10)
+WORKLOADS = {
+    "suspension": dict(shape="256,256,512", hematocrit=0.30, wbc_per_rbc=1.0 / 500, walls=False,
+                       text="periodic %dx%dx%d body-force suspension, Ht 30%%: RBC 642v, PLT 66v 1:10, "
+                            "WBC 2562v 1:500, seeded non-overlapping"),
+    "channel": dict(shape="512,512,1024", hematocrit=0.35, wbc_per_rbc=0.0, walls=True,
+                    text="%dx%dx%d channel, bounce-back walls y = 0 and y = ny-1, periodic x and z, body force "
+                         "along z, Ht 35%%: RBC 642v, PLT 66v 1:10, seeded non-overlapping"),
+}
+
+
+def build_case(hc, susp, shape, precision, seed=20261202, workload="suspension"):
+    w = WORKLOADS[workload]
     units = hc.lbm_base_parameters(5e-7, 1e-7)
     types = susp.cell_types(units)
-    cells = susp.pack(shape, types, seed=seed)
+    cells = susp.pack(shape, types, hematocrit=w["hematocrit"], wbc_per_rbc=w["wbc_per_rbc"], seed=seed,
+                      wall_axis=1 if w["walls"] else None)
     sim = hc.Simulation(0)
-    sim.lattice_init(*shape, units.tau, periodic=(1, 1, 1), precision=precision)
+    if w["walls"]:
+        mask = np.zeros(shape, dtype=np.uint8)
+        mask[:, 0, :] = mask[:, shape[1] - 1, :] = hc.NODE_BOUNCEBACK
+        sim.lattice_init(*shape, units.tau, periodic=(1, 0, 1), node_type=mask.reshape(-1), precision=precision)
+        del mask
+    else:
+        sim.lattice_init(*shape, units.tau, periodic=(1, 1, 1), precision=precision)
     sim.init_equilibrium(1.0, (0.0, 0.0, 0.0))
     sim.set_body_force((0.0, 0.0, 1e-6))
     sim.set_force_limit(units.f_limit)
@@ -69,11 +90,11 @@ def build_case(hc, susp, shape, precision, seed=20261202):
     return sim, units, types, cells
 
 
-def run_gpu(hc, susp, shape, precision, steps, warmup):
+def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension"):
     label = "fp32" if precision == hc.HC_FP32_SHIFTED else "fp64"
-    print("[bench] building %s case %s" % (label, shape), file=sys.stderr, flush=True)
+    print("[bench] building %s %s case %s" % (label, workload, shape), file=sys.stderr, flush=True)
     t0 = time.time()
-    sim, units, types, cells = build_case(hc, susp, shape, precision)
+    sim, units, types, cells = build_case(hc, susp, shape, precision, workload=workload)
     nv = sim.num_vertices
     x0 = sim.vertices()[0]
     print("[bench] %d cells, %d vertices, Ht %.3f, setup %.1fs" % (
@@ -98,12 +119,14 @@ def run_gpu(hc, susp, shape, precision, steps, warmup):
             collide_ms = ms / max(cnt, 1)
     sim.profile_enable(False)
     sim.check_finite()
+    wall_hits = sim.wall_hits()
     x1 = sim.vertices()[0]
     N = int(np.prod(shape))
     res = dict(label=label, ms_per_step=1e3 * dt / steps, mlups=N * steps / dt / 1e6,
                vertex_updates_per_s=nv * steps / dt, stage_ms=stages, vertices=nv, nodes=N,
                cells={k: int(len(v)) for k, v in cells.items()})
     res["collide_ms"] = collide_ms
+    res["wall_hits"] = int(wall_hits)
     res["x0"], res["x1"] = x0, x1
     res["collide_gbs"] = BYTES_PER_LUP[label] * N / (stages["collide_stream"] * 1e-3) / 1e9
     print("[bench] %s: %.3f ms/step, %.1f MLUPS, collide %.3f ms = %.0f GB/s" % (
@@ -180,7 +203,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="both", choices=["both", "fp32", "fp64"])
-    ap.add_argument("--shape", default="256,256,512")
+    ap.add_argument("--workload", default="suspension", choices=sorted(WORKLOADS),
+                    help="suspension: the Ht 30%% periodic box (the headline line); channel: the largest domain")
+    ap.add_argument("--shape", default=None, help="nx,ny,nz (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", type=int, default=32, metavar="N",
                     help="cases of the serving stream (0 skips the latency measurement)")
@@ -193,7 +218,8 @@ def main():
         sys.exit(2)
     import hemocell_amd as hc
     from hemocell_amd import suspension as susp
-    shape = tuple(int(s) for s in args.shape.split(","))
+    wl = WORKLOADS[args.workload]
+    shape = tuple(int(s) for s in (args.shape or wl["shape"]).split(","))
     if args.copy_calibration:
         import torch
         a = torch.ones(args.copy_calibration * 2 ** 18, dtype=torch.float32, device="cuda")
@@ -205,13 +231,13 @@ def main():
         del a, b
     res = {}
     if args.precision in ("both", "fp32"):
-        res["fp32"] = run_gpu(hc, susp, shape, hc.HC_FP32_SHIFTED, args.steps, args.warmup)
+        res["fp32"] = run_gpu(hc, susp, shape, hc.HC_FP32_SHIFTED, args.steps, args.warmup, args.workload)
     if args.precision in ("both", "fp64"):
-        res["fp64"] = run_gpu(hc, susp, shape, hc.HC_FP64, args.steps, args.warmup)
+        res["fp64"] = run_gpu(hc, susp, shape, hc.HC_FP64, args.steps, args.warmup, args.workload)
     head = res.get("fp32") or res["fp64"]
     traffic = profile_traffic(head["label"])
-    if traffic is not None and shape != (256, 256, 512):
-        traffic = None  # the committed summary is for the 256x256x512 workload only
+    if traffic is not None and (shape != (256, 256, 512) or args.workload != "suspension"):
+        traffic = None  # the committed summary is for the 256x256x512 suspension only
     line = {
         "metric": METRIC,
         "value": round(head["mlups"], 2),
@@ -225,9 +251,9 @@ def main():
         "vs_baseline": None,
         "dtype": head["label"],
         "data": "synthetic",
-        "config": {"workload": "periodic %dx%dx%d body-force suspension, Ht 30%%: RBC 642v, PLT 66v 1:10, "
-                               "WBC 2562v 1:500, seeded non-overlapping" % shape,
-                   "cells": head["cells"], "vertices": head["vertices"], "precision": head["label"]},
+        "config": {"workload": wl["text"] % shape,
+                   "cells": head["cells"], "vertices": head["vertices"], "precision": head["label"],
+                   "wall_hits": head["wall_hits"]},
         "roofline": {"bound": "hbm", "achieved": round(head["collide_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["collide_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_collide_stream", "mean_ms": round(head["collide_ms"], 4),
@@ -251,9 +277,10 @@ def main():
                         "collide_frac": round(f64["collide_gbs"] / HBM_PEAK_GBS, 4),
                         "stage_ms": {k: round(v, 4) for k, v in f64["stage_ms"].items()}}
         line["fp32_over_fp64"] = round(res["fp32"]["mlups"] / f64["mlups"], 3)
-    if args.serve > 0:
+    # the serving stream and the CPU baseline belong to the headline workload
+    if args.serve > 0 and args.workload == "suspension":
         line["serving"] = serve_stream(args.serve, args.serve_rate)
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "suspension":
         line["cpu_baseline"] = cpu_baseline(hc, susp)
     print(json.dumps(line), flush=True)
 

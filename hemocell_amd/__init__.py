@@ -297,11 +297,12 @@ class Simulation:
         self._chk(self.lib.hc_get_force_field(self.h, _dp(F)))
         return F.reshape(3, self.N)
 
-    def fluid(self):
+    def fluid(self, velocity=True):
+        """(rho, u) per node; u is None with velocity=False (only rho is read back)."""
         rho = np.zeros(self.N)
-        u = np.zeros(3 * self.N)
-        self._chk(self.lib.hc_get_fluid(self.h, _dp(rho), _dp(u)))
-        return rho, u.reshape(self.N, 3)
+        u = np.zeros(3 * self.N) if velocity else None
+        self._chk(self.lib.hc_get_fluid(self.h, _dp(rho), _dp(u) if velocity else None))
+        return rho, (u.reshape(self.N, 3) if velocity else None)
 
     def vertices(self):
         nv = self.num_vertices

@@ -1,4 +1,4 @@
-"""Seeded, non-overlapping cell suspensions for periodic boxes.
+"""Seeded, non-overlapping cell suspensions for periodic boxes and channels.
 
 Stands in for the reference's cell packing inputs (the RBC.pos / PLT.pos files
 written by tools/packCells, read by io/readPositionsBloodCells.cpp:186-285):
@@ -81,8 +81,13 @@ def cell_types(units, wbc_triangles=2000):
     return out
 
 
-def pack(shape, types, hematocrit=0.30, plt_per_rbc=0.1, wbc_per_rbc=1.0 / 500, seed=20261202, layer=5.6):
-    """Vertex positions per type: {name: array[ncells, nv, 3]} in lattice units."""
+def pack(shape, types, hematocrit=0.30, plt_per_rbc=0.1, wbc_per_rbc=1.0 / 500, seed=20261202, layer=5.6,
+         wall_axis=None):
+    """Vertex positions per type: {name: array[ncells, nv, 3]} in lattice units.
+
+    wall_axis 0 or 1: that axis is bounded by wall planes at 0 and n-1, and the
+    first and last columns along it stay empty, so no vertex's support reaches a
+    wall node."""
     nx, ny, nz = shape
     rng = np.random.default_rng(seed)
     rbc = types["RBC"]
@@ -98,8 +103,14 @@ def pack(shape, types, hematocrit=0.30, plt_per_rbc=0.1, wbc_per_rbc=1.0 / 500, 
         raise ValueError("RBC does not fit a 16 x 16 x %.2f slot" % h)
     wl = int(math.ceil((ext["WBC"][2] + 0.2) / h)) if n_wbc else 0
     free = np.ones((ci, cj, nl), dtype=bool)
+    if wall_axis == 0:
+        free[0, :, :] = free[ci - 1, :, :] = False
+    elif wall_axis == 1:
+        free[:, 0, :] = free[:, cj - 1, :] = False
+    elif wall_axis is not None:
+        raise ValueError("wall_axis must be 0, 1 or None")
     wbc_centres = []
-    cols = [(i, j) for i in range(0, ci, 2) for j in range(0, cj, 2)]
+    cols = [(i, j) for i in range(0, ci, 2) for j in range(0, cj, 2) if free[i, j, 0]]
     rng.shuffle(cols)
     if n_wbc > len(cols):
         raise ValueError("too many WBCs for this box")

@@ -1,0 +1,73 @@
+"""The largest one-GPU workload at its full size, checked through properties that
+do not depend on size (no oracle run is feasible at 268M nodes).
+
+512x512x1024 channel between bounce-back planes y = 0 and y = 511, periodic in x
+and z, body force along z, Ht 35%: 144780 RBCs and 14478 platelets (93.9M
+vertices), seeded non-overlapping placement clear of the walls.  After 20 steps:
+
+* FP64 total mass sum(rho) is conserved to 1e-12 relative: bounce-back planes
+  and periodic faces neither add nor remove mass, and the IBM force adds
+  momentum only;
+* every population and vertex coordinate is finite and no vertex reached a wall
+  node;
+* every cell's volume stays within 1% of its rest volume (volume constraint);
+* the FP32-shifted run's vertex displacements agree with the FP64 run's to 1e-4
+  relative L2, the bound stated for the FP32 form.
+"""
+import numpy as np
+import pytest
+
+import hemocell_amd as hc
+from hemocell_amd import suspension as susp
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (512, 512, 1024)
+STEPS = 20
+
+
+def _run(precision):
+    print("[largest] building", "fp64" if precision == hc.HC_FP64 else "fp32", flush=True)
+    units = hc.lbm_base_parameters(5e-7, 1e-7)
+    types = susp.cell_types(units)
+    cells = susp.pack(SHAPE, types, hematocrit=0.35, wbc_per_rbc=0.0, wall_axis=1)
+    mask = np.zeros(SHAPE, dtype=np.uint8)
+    mask[:, 0, :] = mask[:, SHAPE[1] - 1, :] = hc.NODE_BOUNCEBACK
+    sim = hc.Simulation(0)
+    sim.lattice_init(*SHAPE, units.tau, periodic=(1, 0, 1), node_type=mask.reshape(-1), precision=precision)
+    del mask
+    sim.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    sim.set_body_force((0.0, 0.0, 1e-6))
+    sim.set_force_limit(units.f_limit)
+    rest = []
+    for name in ("RBC", "PLT"):
+        t = types[name]
+        tid = sim.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner)
+        sim.add_cells(tid, cells[name])
+        rest += [susp._volume(t.verts, t.tris)] * len(cells[name])
+    del cells
+    rho0 = sim.fluid(velocity=False)[0].sum() if precision == hc.HC_FP64 else None
+    x0 = sim.vertices()[0]
+    print("[largest] stepping", flush=True)
+    sim.step(STEPS)
+    sim.synchronize()
+    sim.check_finite()
+    out = dict(x0=x0, x1=sim.vertices()[0], wall_hits=sim.wall_hits(), rest=np.array(rest),
+               volume=sim.cell_stats()["volume"])
+    if rho0 is not None:
+        out["rho0"], out["rho1"] = rho0, sim.fluid(velocity=False)[0].sum()
+    sim.close()
+    return out
+
+
+def test_largest_channel_conserves_mass_and_agrees_across_precisions():
+    r64 = _run(hc.HC_FP64)
+    assert r64["wall_hits"] == 0
+    assert abs(r64["rho1"] - r64["rho0"]) <= 1e-12 * r64["rho0"]
+    assert np.all(np.abs(r64["volume"] / r64["rest"] - 1.0) < 0.01)
+    r32 = _run(hc.HC_FP32_SHIFTED)
+    assert r32["wall_hits"] == 0
+    assert np.all(np.abs(r32["volume"] / r32["rest"] - 1.0) < 0.01)
+    d64 = r64["x1"] - r64["x0"]
+    d32 = r32["x1"] - r32["x0"]
+    assert np.linalg.norm(d32 - d64) <= 1e-4 * np.linalg.norm(d64)
