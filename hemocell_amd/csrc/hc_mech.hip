@@ -72,8 +72,9 @@ __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const 
 
   // ---- cell volume (sum of signed triple products, x 1/6) --------------------
   double part = 0.0;
+  const D3 r = c.pos(0);  // corners relative to vertex 0 (closed mesh: same volume, no cancellation)
   for (int t = tid; t < T.nt; t += BS) {
-    const D3 v0 = c.pos(T.tri[3 * t]), v1 = c.pos(T.tri[3 * t + 1]), v2 = c.pos(T.tri[3 * t + 2]);
+    const D3 v0 = c.pos(T.tri[3 * t]) - r, v1 = c.pos(T.tri[3 * t + 1]) - r, v2 = c.pos(T.tri[3 * t + 2]) - r;
     const double v210 = v2.x * v1.y * v0.z;
     const double v120 = v1.x * v2.y * v0.z;
     const double v201 = v2.x * v0.y * v1.z;
@@ -135,14 +136,16 @@ __global__ __launch_bounds__(BS) void k_cell_stats(VertexDev V, TypeDev T, const
   for (int k = 0; k < NQ; ++k) q[k] = 0.0;
   q[8] = q[10] = q[12] = INFINITY;
   q[9] = q[11] = q[13] = -INFINITY;
+  const D3 r = c.pos(0);  // volume corners relative to vertex 0, as the mechanics takes them
   for (int t = tid; t < T.nt; t += BS) {
     const D3 v0 = c.pos(T.tri[3 * t]), v1 = c.pos(T.tri[3 * t + 1]), v2 = c.pos(T.tri[3 * t + 2]);
-    const double v210 = v2.x * v1.y * v0.z;
-    const double v120 = v1.x * v2.y * v0.z;
-    const double v201 = v2.x * v0.y * v1.z;
-    const double v021 = v0.x * v2.y * v1.z;
-    const double v102 = v1.x * v0.y * v2.z;
-    const double v012 = v0.x * v1.y * v2.z;
+    const D3 w0 = v0 - r, w1 = v1 - r, w2 = v2 - r;
+    const double v210 = w2.x * w1.y * w0.z;
+    const double v120 = w1.x * w2.y * w0.z;
+    const double v201 = w2.x * w0.y * w1.z;
+    const double v021 = w0.x * w2.y * w1.z;
+    const double v102 = w1.x * w0.y * w2.z;
+    const double v012 = w0.x * w1.y * w2.z;
     q[0] += (1.0 / 6.0) * (-v210 + v120 + v201 - v021 - v102 + v012);
     q[1] += 0.5 * normp(crossp(v1 - v0, v2 - v0));
   }

@@ -305,8 +305,12 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
 template <int BS>
 __device__ __forceinline__ double volume_partial(const CellView& c, const TypeDev& T, int tid) {
   double part = 0.0;
+  // corners relative to the cell's vertex 0: the same volume for a closed mesh,
+  // without the cancellation of triple products of coordinates hundreds of
+  // spacings from the origin
+  const D3 r = c.pos(0);
   for (int t = tid; t < T.nt; t += BS) {
-    const D3 v0 = c.pos(T.tri[3 * t]), v1 = c.pos(T.tri[3 * t + 1]), v2 = c.pos(T.tri[3 * t + 2]);
+    const D3 v0 = c.pos(T.tri[3 * t]) - r, v1 = c.pos(T.tri[3 * t + 1]) - r, v2 = c.pos(T.tri[3 * t + 2]) - r;
     const double v210 = v2.x * v1.y * v0.z;
     const double v120 = v1.x * v2.y * v0.z;
     const double v201 = v2.x * v0.y * v1.z;

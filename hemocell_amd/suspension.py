@@ -142,6 +142,61 @@ def pack(shape, types, hematocrit=0.30, plt_per_rbc=0.1, wbc_per_rbc=1.0 / 500, 
     return out
 
 
+def tube_fluid(length, diameter):
+    """Fluid mask of a tube along x in a length x diameter x diameter box: fluid
+    iff (y - c)^2 + (z - c)^2 < r^2 with c = (diameter - 1) / 2, r = diameter / 2 - 0.5;
+    the rest is wall.  Indexed [x, y, z]."""
+    c = (diameter - 1) / 2.0
+    r = diameter / 2.0 - 0.5
+    y, z = np.meshgrid(np.arange(diameter), np.arange(diameter), indexing="ij")
+    inside = (y - c) ** 2 + (z - c) ** 2 < r * r
+    return np.broadcast_to(inside, (length, diameter, diameter))
+
+
+def pack_tube(length, diameter, types, hematocrit=0.20, plt_per_rbc=0.1, seed=20261202, layer=5.6, margin=1.6):
+    """Seeded, non-overlapping cells inside the tube of tube_fluid(length, diameter).
+
+    Slots are 16 (x) by 16 (y) by `layer` (z) boxes whose corners in the y-z
+    cross-section stay `margin` inside the wall radius; RBCs lie with their disc
+    axis along z, rotated by a random angle about it; platelets take a slot each
+    with a random orientation.  The hematocrit counts RBC volume over the tube's
+    fluid volume.  Returns {name: array[ncells, nv, 3]} in lattice units."""
+    rng = np.random.default_rng(seed)
+    c = (diameter - 1) / 2.0
+    r_in = diameter / 2.0 - 0.5 - margin
+    rbc = types["RBC"]
+    ext = np.ptp(rbc.verts, axis=0)
+    if ext[2] > layer or max(ext[0], ext[1]) > 16.0:
+        raise ValueError("RBC does not fit a 16 x 16 x %.2f slot" % layer)
+    # the cross-section's slot grid: the y offset, z offset and row stagger that
+    # fit the most slots inside the circle
+    def grid(yoff, zoff, stagger):
+        out = []
+        nrow = int(diameter // layer) + 2
+        for i in range(-nrow, nrow + 1):
+            zc = c + zoff + layer * i
+            for yc in np.arange(c - diameter, c + diameter + 0.1, 16.0) + yoff + (stagger if i % 2 else 0.0):
+                if all(np.hypot(yc + dy - c, zc + dz - c) < r_in for dy in (-8.0, 8.0)
+                       for dz in (-layer / 2, layer / 2)):
+                    out.append((yc, zc))
+        return out
+    cross = max((grid(yo, zo, st) for yo in np.arange(0.0, 8.01, 0.5) for zo in np.arange(0.0, layer, 0.4)
+                 for st in (0.0, 8.0)), key=len)
+    slots = [(8.0 + 16.0 * k, yc, zc) for k in range(length // 16) for (yc, zc) in cross]
+    fluid = float(tube_fluid(1, diameter).sum()) * length
+    n_rbc = int(round(hematocrit * fluid / _volume(rbc.verts, rbc.tris)))
+    n_plt = int(round(plt_per_rbc * n_rbc))
+    if n_rbc + n_plt > len(slots):
+        raise ValueError("hematocrit %.2f does not fit this tube (%d slots for %d cells)" %
+                         (hematocrit, len(slots), n_rbc + n_plt))
+    pick = rng.permutation(len(slots))[:n_rbc + n_plt]
+    centres = np.array(slots)[pick]
+    out = {"RBC": _instances(rbc.verts, _rot_z(rng.uniform(0, 2 * np.pi, n_rbc)), centres[:n_rbc])}
+    if n_plt:
+        out["PLT"] = _instances(types["PLT"].verts, _random_rotations(rng, n_plt), centres[n_rbc:])
+    return out
+
+
 def _instances(verts, rots, centres):
     v = verts - verts.mean(axis=0)
     return np.einsum("cij,vj->cvi", rots, v) + centres[:, None, :]

@@ -582,15 +582,23 @@ struct Oracle {
     double volume = 0.0;
     std::vector<double> tarea(ct.nt);
     std::vector<V3> tnorm(ct.nt);
+    // rbcHighOrderModel.cpp:48-100 sums the triple products of the absolute corner
+    // positions; here they are taken relative to the cell's vertex 0 (the same
+    // volume for a closed mesh), which removes the cancellation that absolute
+    // coordinates far from the origin cause
+    const double* r0 = P(0);
     for (int t = 0; t < ct.nt; ++t) {
       const int* T = &ct.tri[3 * t];
       const double *v0 = P(T[0]), *v1 = P(T[1]), *v2 = P(T[2]);
-      const double v210 = v2[0] * v1[1] * v0[2];
-      const double v120 = v1[0] * v2[1] * v0[2];
-      const double v201 = v2[0] * v0[1] * v1[2];
-      const double v021 = v0[0] * v2[1] * v1[2];
-      const double v102 = v1[0] * v0[1] * v2[2];
-      const double v012 = v0[0] * v1[1] * v2[2];
+      const double w0[3] = {v0[0] - r0[0], v0[1] - r0[1], v0[2] - r0[2]};
+      const double w1[3] = {v1[0] - r0[0], v1[1] - r0[1], v1[2] - r0[2]};
+      const double w2[3] = {v2[0] - r0[0], v2[1] - r0[1], v2[2] - r0[2]};
+      const double v210 = w2[0] * w1[1] * w0[2];
+      const double v120 = w1[0] * w2[1] * w0[2];
+      const double v201 = w2[0] * w0[1] * w1[2];
+      const double v021 = w0[0] * w2[1] * w1[2];
+      const double v102 = w1[0] * w0[1] * w2[2];
+      const double v012 = w0[0] * w1[1] * w2[2];
       volume += (-v210 + v120 + v201 - v021 - v102 + v012);
       double area;
       V3 n;

@@ -1,0 +1,187 @@
+"""The single-cell shear and pipe-flow configurations at their stated sizes.
+
+Single-cell shear: 50x50x50 lattice, periodic x and y, velocity walls z = 0 and
+z = 49 moving at -+(49) * gamma / 2 with gamma = 100 /s (1e-5 in lattice units at
+dt = 1e-7 s, so tau = 1.82 by the reference's rule), one 642-vertex RBC (radius
+3.91 um) placed as a .pos line "12.5 12.5 12.5 90 0 0" would place it, default RBC
+constants.
+  * FP64, 100 steps, against the FP64 oracle: vertex positions, vertex velocities
+    and the fluid velocity within 1e-10 relative L2;
+  * FP32-shifted, 1000 steps, against the FP64 oracle's 1000 steps: fluid velocity
+    within 1e-5 and vertex displacement within 1e-4 relative L2, and the
+    deformation index (L - B) / (L + B) of the cell's bounding box in the shear
+    plane within 1% (1000 steps at 100 /s are a strain of 0.01, so the shape has
+    barely changed: the displacement bound is the sharper check).
+
+Pipe flow: a tube of 64-node diameter along x in a 256x64x64 box (fluid iff
+(y - 31.5)^2 + (z - 31.5)^2 < 31.5^2, the rest bounce-back), periodic in x, body
+force g = 4 nu u_max / R^2 for u_max = 1 mm/s, Ht 20% (244 RBCs) plus 24
+platelets from the seeded tube packing.
+  * FP64, 100 steps, against the FP64 oracle: positions and fluid velocity within
+    1e-10 relative L2.  The cells are the packed shapes displaced by a seeded 3%
+    of a lattice spacing per vertex: at the exact rest shape every membrane force
+    is round-off (~1e-17), two correct FP64 evaluations differ at that level, and
+    this slow flow (u ~ 1e-6) turns it into 1e-9 relative velocity differences
+    (measured) that say nothing about parity;
+  * without cells, 6000 steps from rest (15 decay times of the slowest mode): the axial velocity over the fluid
+    nodes matches the Poiseuille parabola g (R^2 - r^2) / (4 nu) to 2% relative L2
+    (bounce-back puts the wall half a node outside the last fluid node, the
+    remaining staircase error), the mean velocity to 2%;
+  * with cells, FP32-shifted against FP64 over 1000 steps: the mean axial fluid
+    velocity (the apparent-viscosity measure: u_mean without cells over u_mean with
+    cells) agrees within 1%, and the vertex displacements within 1e-4.
+"""
+import numpy as np
+import pytest
+
+import cases
+import hemocell_amd as hc
+from hemocell_amd import suspension as susp
+from oracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+RBC_POS = [((12.5, 12.5, 12.5), (90.0, 0.0, 0.0))]
+
+
+def _shear(sim, u, n=50):
+    gamma = 100.0 * u.dt
+    vhalf = (n - 1) * gamma * 0.5
+    sim.set_wall_velocity(5, [-vhalf, 0.0, 0.0])
+    sim.set_wall_velocity(4, [vhalf, 0.0, 0.0])
+    sim.init_equilibrium(1.0, [0.0, 0.0, 0.0])
+    sim.set_force_limit(u.f_limit)
+
+
+def _shear_pair(precision):
+    u = cases.units()
+    n = 50
+    mask = cases.couette_mask(n, n, n)
+    g = hc.Simulation(0)
+    g.lattice_init(n, n, n, u.tau, periodic=(1, 1, 0), node_type=mask, precision=precision)
+    o = OracleSim(n, n, n, u.tau, periodic=(1, 1, 0), node_type=mask)
+    model, v, t, p, inner = cases.mesh("RBC", u, hc.build_mesh)
+    boundary = (mask != 0).reshape(n, n, n)
+    ids, placed = cases.pos_placement(v, RBC_POS, u.dx, boundary, 0.0, 0)
+    assert ids == [0]
+    for s in (g, o):
+        _shear(s, u, n)
+        tid = s.add_cell_type(model, v, t, p, inner)
+        s.add_cells(tid, placed[0][None])
+    return g, o, placed[0]
+
+
+def _deformation_index(x):
+    """(L - B) / (L + B) from the cell's extents in the shear plane (x, z)."""
+    lx = np.ptp(x[:, 0])
+    lz = np.ptp(x[:, 2])
+    L, B = max(lx, lz), min(lx, lz)
+    return (L - B) / (L + B)
+
+
+def test_single_cell_shear_fp64_matches_oracle_100_steps():
+    g, o, x0 = _shear_pair(hc.HC_FP64)
+    for _ in range(10):
+        g.step(10)
+        o.step(10)
+    xg, vg, _ = g.vertices()
+    xo, vo, _ = o.vertices()
+    assert cases.rel_l2(xg, xo) < 1e-10
+    assert cases.rel_l2(xg - x0, xo - x0) < 1e-8
+    assert cases.rel_l2(vg, vo) < 1e-10
+    assert cases.rel_l2(g.fluid()[1], o.fluid()[1]) < 1e-10
+    assert g.wall_hits() == o.wall_hits() == 0
+
+
+def test_single_cell_shear_fp32_1000_steps_against_fp64_oracle():
+    g, o, x0 = _shear_pair(hc.HC_FP32_SHIFTED)
+    for _ in range(10):
+        g.step(100)
+        o.step(100)
+    xg = g.vertices()[0]
+    xo = o.vertices()[0]
+    assert cases.rel_l2(g.fluid()[1], o.fluid()[1]) < 1e-5
+    assert cases.rel_l2(xg - x0, xo - x0) < 1e-4
+    dig, dio = _deformation_index(xg), _deformation_index(xo)
+    print("[shear] DI fp32 %.6f oracle %.6f rest %.6f" % (dig, dio, _deformation_index(x0)))
+    assert abs(dig - dio) <= 0.01 * dio
+
+
+def _pipe(precision, cells=True, oracle=False, perturb=0.0):
+    u = cases.units()
+    L, D = 256, 64
+    fluid = susp.tube_fluid(L, D)
+    mask = np.where(fluid, 0, hc.NODE_BOUNCEBACK).astype(np.uint8).reshape(-1)
+    R = D / 2.0 - 0.5
+    u_max = 1e-3 * u.dt / u.dx
+    gx = 4.0 * u.nu_lbm * u_max / (R * R)
+    sims = []
+    g = hc.Simulation(0)
+    g.lattice_init(L, D, D, u.tau, periodic=(1, 0, 0), node_type=mask, precision=precision)
+    sims.append(g)
+    if oracle:
+        sims.append(OracleSim(L, D, D, u.tau, periodic=(1, 0, 0), node_type=mask))
+    for s in sims:
+        s.init_equilibrium(1.0, [0.0, 0.0, 0.0])
+        s.set_body_force([gx, 0.0, 0.0])
+        s.set_force_limit(u.f_limit)
+    if cells:
+        types = susp.cell_types(hc.lbm_base_parameters(u.dx, u.dt))
+        packed = susp.pack_tube(L, D, types, hematocrit=0.20)
+        assert len(packed["RBC"]) == 244 and len(packed["PLT"]) == 24
+        if perturb:
+            rng = np.random.default_rng(11)
+            packed = {k: v + rng.uniform(-perturb, perturb, v.shape) for k, v in packed.items()}
+        for name in ("RBC", "PLT"):
+            t = types[name]
+            for s in sims:
+                tid = s.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner)
+                s.add_cells(tid, packed[name])
+    return sims, fluid, gx, R, u
+
+
+def test_pipe_fp64_matches_oracle_100_steps():
+    (g, o), fluid, _, _, _ = _pipe(hc.HC_FP64, oracle=True, perturb=0.03)
+    x0 = g.vertices()[0]
+    for _ in range(10):
+        g.step(10)
+        o.step(10)
+    xg, xo = g.vertices()[0], o.vertices()[0]
+    assert cases.rel_l2(xg, xo) < 1e-10
+    assert cases.rel_l2(xg - x0, xo - x0) < 1e-8
+    ug, uo = g.fluid()[1], o.fluid()[1]
+    f = fluid.reshape(-1)
+    assert cases.rel_l2(ug[f], uo[f]) < 1e-10
+    assert g.wall_hits() == o.wall_hits() == 0
+
+
+def test_pipe_poiseuille_profile():
+    (g,), fluid, gx, R, u = _pipe(hc.HC_FP64, cells=False)
+    g.step(6000)
+    ux = g.fluid()[1][:, 0].reshape(fluid.shape)[0]  # one cross-section (x-invariant)
+    f = fluid[0]
+    y, z = np.meshgrid(np.arange(f.shape[0]), np.arange(f.shape[1]), indexing="ij")
+    r2 = (y - 31.5) ** 2 + (z - 31.5) ** 2
+    exact = gx * (R * R - r2) / (4.0 * u.nu_lbm)
+    err, merr = cases.rel_l2(ux[f], exact[f]), ux[f].mean() / exact[f].mean() - 1.0
+    print("[pipe] Poiseuille relative L2 %.4f, mean velocity %+.4f" % (err, merr))
+    assert err < 0.02
+    assert abs(merr) < 0.02
+
+
+def test_pipe_fp32_and_fp64_long_run_statistics_agree():
+    out = {}
+    for prec in (hc.HC_FP64, hc.HC_FP32_SHIFTED):
+        (g,), fluid, _, _, _ = _pipe(prec)
+        x0 = g.vertices()[0]
+        g.step(1000)
+        g.check_finite()
+        f = fluid.reshape(-1)
+        out[prec] = dict(umean=g.fluid()[1][f, 0].mean(), d=g.vertices()[0] - x0, hits=g.wall_hits())
+        g.close()
+    a, b = out[hc.HC_FP64], out[hc.HC_FP32_SHIFTED]
+    dd = np.linalg.norm(b["d"] - a["d"]) / np.linalg.norm(a["d"])
+    print("[pipe] FP32/FP64 mean velocity %.6f, displacement relative L2 %.2e" % (b["umean"] / a["umean"], dd))
+    assert a["hits"] == b["hits"] == 0
+    assert abs(b["umean"] / a["umean"] - 1.0) < 0.01
+    assert dd <= 1e-4

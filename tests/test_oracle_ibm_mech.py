@@ -128,13 +128,20 @@ def mech_forces(o, x, vel=None):
 
 
 @pytest.mark.parametrize("kind", ["RBC", "PLT", "WBC"])
-def test_rest_shape_is_force_free(kind):
-    # evaluated where the rest tables were built: the reference's volume is a sum of
-    # triple products of absolute coordinates, which loses ~|x|^3/V digits away from 0
+@pytest.mark.parametrize("shift", [0.0, 300.0])
+def test_rest_shape_is_force_free(kind, shift):
+    # also 300 spacings from the origin: the volume's triple products are taken
+    # relative to the cell's vertex 0, so they do not lose ~|x|^3/V digits to
+    # cancellation as absolute coordinates (the reference's form) would; there
+    # the volume term alone would leave forces ~1e-6 of the stiffness.  The bound
+    # is 1e-11 there because storing the shifted positions rounds the shape by
+    # ulp(300) ~ 6e-14 spacings.
     o, tid, v, t, p = rbc_sim(40, kind)
-    o.add_cells(tid, v[None])
-    F = mech_forces(o, v)
-    assert np.abs(F).max() < 1e-12 * max(p.k_link, p.k_bend, p.k_area, p.k_volume)
+    x = v + shift
+    o.add_cells(tid, x[None])
+    F = mech_forces(o, x)
+    bound = 1e-12 if shift == 0.0 else 1e-11
+    assert np.abs(F).max() < bound * max(p.k_link, p.k_bend, p.k_area, p.k_volume)
 
 
 @pytest.mark.parametrize("kind", ["RBC", "PLT", "WBC"])
