@@ -102,6 +102,7 @@ SIGNATURES = {
     "hc_get_vertex_bins": (_ST, [_P, _I64]),
     "hc_get_cell_volume_area": (_ST, [_P, _D, _D]),
     "hc_get_cell_stats": (_ST, [_P, _D, _D, _D, _D, _D, _D]),
+    "hc_get_fluid_stats": (_ST, [_P, _INT, _D]),
     "hc_wall_hits": (ctypes.c_int64, [_P]),
     "hc_check_finite": (_ST, [_P]),
     "hc_profile_enable": (_ST, [_P, _INT]),
@@ -350,6 +351,13 @@ class Simulation:
         self._chk(self.lib.hc_get_cell_stats(self.h, *[_dp(out[k]) for k in
                                                        ("volume", "area", "centroid", "velocity", "bbox", "stretch")]))
         return out
+
+    def fluid_stats(self, what="velocity"):
+        """FluidInfo statistics: dict(min, max, avg, ncells) of |u| ("velocity") or of the
+        spread vertex forces |F| ("force") over the nodes that are not bounce-back."""
+        out = np.zeros(4)
+        self._chk(self.lib.hc_get_fluid_stats(self.h, {"velocity": 0, "force": 1}[what], _dp(out)))
+        return dict(min=out[0], max=out[1], avg=out[2], ncells=int(out[3]))
 
     def wall_hits(self):
         return self.lib.hc_wall_hits(self.h)

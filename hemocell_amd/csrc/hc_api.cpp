@@ -649,6 +649,54 @@ hc_status ready_for_step(hc_ctx* ctx) {
 
 }  // namespace
 
+template <typename R>
+static hc_status fluid_stats(hc_ctx* ctx, int what, double* out) {
+  const int nb = (int)std::min<int64_t>(4096, (ctx->L.N + 255) / 256);
+  double* part = nullptr;
+  R* Fs = nullptr;
+  hc_status s = dalloc(ctx, (void**)&part, sizeof(double) * 4 * nb, "statistics partials");
+  if (s != HC_OK) return s;
+  CollideArgs a;
+  a.omega = 1.0 / ctx->tau;
+  for (int c = 0; c < 3; ++c) a.g[c] = ctx->g[c];
+  std::memcpy(a.wall_u, ctx->wall_u, sizeof(a.wall_u));
+  const R* Fc = ctx->spread_ready ? nullptr : (const R*)ctx->Fibm[ctx->cur];  // reset field after a fused step
+  hipError_t e = hipSuccess;
+  if (what == 1) {
+    // calculateForceStatistics spreads the vertex forces into a reset field first
+    // (helper/fluidInfo.cpp:117-120): here into a field of its own
+    if ((s = dalloc(ctx, (void**)&Fs, sizeof(R) * 3 * ctx->L.stride, "statistics force field")) != HC_OK) {
+      hipFree(part);
+      return s;
+    }
+    e = launch_fill_zero<R>(ctx->stream, Fs, 3 * ctx->L.stride);
+    if (e == hipSuccess && ctx->nv > 0 && sync_cells(ctx) == HC_OK)
+      e = launch_spread_cells<R>(ctx->stream, vdev(ctx), Fs, ctx->type, ctx->L, ctx->d_cell_base, ctx->d_cell_nv,
+                                 (int)ctx->cell_base.size(), ctx->f_limit, ctx->repulsion ? 1 : 0,
+                                 std::min(65536, ctx->lds_limit));
+  }
+  if (e == hipSuccess)
+    e = launch_fluid_stats<R>(ctx->stream, (const R*)ctx->f, Fc, ctx->type, ctx->L, ctx->phase, a, Fs, what, part, nb);
+  std::vector<double> h(4 * (size_t)nb);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(part);
+  if (Fs) hipFree(Fs);
+  if (e != hipSuccess) return hipfail(ctx, e, "fluid statistics");
+  double mn = INFINITY, mx = -INFINITY, sum = 0.0, cnt = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    mn = std::min(mn, h[4 * b]);
+    mx = std::max(mx, h[4 * b + 1]);
+    sum += h[4 * b + 2];
+    cnt += h[4 * b + 3];
+  }
+  out[0] = cnt > 0 ? mn : 0.0;
+  out[1] = cnt > 0 ? mx : 0.0;
+  out[2] = cnt > 0 ? sum / cnt : 0.0;
+  out[3] = cnt;
+  return HC_OK;
+}
+
 extern "C" {
 
 hc_status hc_create(int device, hc_ctx** out) {
@@ -1167,6 +1215,13 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u) {
   if (du) hipFree(du);
   if (e != hipSuccess) return hipfail(ctx, e, "get_fluid");
   return HC_OK;
+}
+
+hc_status hc_get_fluid_stats(hc_ctx* ctx, int what, double* out) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!out || (what != 0 && what != 1)) return fail(ctx, HC_INVALID_ARG, "fluid statistics: what is 0 or 1");
+  return fp32(ctx) ? fluid_stats<float>(ctx, what, out) : fluid_stats<double>(ctx, what, out);
 }
 
 static hc_status get_planes(hc_ctx* ctx, const double* d, double* out) {

@@ -26,6 +26,12 @@ hipError_t launch_store_populations(hipStream_t s, R* f, const double* in, const
 template <typename R>
 hipError_t launch_fluid_moments(hipStream_t s, const R* f, const R* Fcur, const uint8_t* type, const LatticeDev& L,
                                 int phase, const CollideArgs& a, double* rho, double* u);
+// per-block {min, max, sum, count} of |u| (what 0) or |Fstat| (what 1) over the
+// non-bounce-back nodes; nblocks * 4 doubles in partials
+template <typename R>
+hipError_t launch_fluid_stats(hipStream_t s, const R* f, const R* Fcur, const uint8_t* type, const LatticeDev& L,
+                              int phase, const CollideArgs& a, const R* Fstat, int what, double* partials,
+                              int nblocks);
 template <typename R>
 hipError_t launch_fill_zero(hipStream_t s, R* p, int64_t n);
 template <typename R>

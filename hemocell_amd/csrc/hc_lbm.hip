@@ -284,14 +284,12 @@ __global__ void k_store(R* f, const double* in, LatticeDev L) {
   for (int i = 0; i < 19; ++i) f[i * L.stride + n] = (R)in[i * L.N + n];
 }
 
-// Density and velocity of every node for output; a velocity-boundary node
-// reports its imposed velocity and the density of the mass balance over its
-// known populations.
+// Density and velocity of node n for output; a velocity-boundary node reports
+// its imposed velocity and the density of the mass balance over its known
+// populations.
 template <typename R>
-__global__ void k_moments(const R* f, const R* Fcur, const uint8_t* type, LatticeDev L, int phase, double g0,
-                          double g1, double g2, CollideArgs a, double* rho, double* u) {
-  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (n >= L.N) return;
+__device__ void node_moments(const R* f, const R* Fcur, const uint8_t* type, const LatticeDev& L, int phase,
+                             const CollideArgs& a, int64_t n, double& rho, double& u0, double& u1, double& u2) {
   int x, y, z;
   coords(n, L, x, y, z);
   R fl[19];
@@ -301,9 +299,9 @@ __global__ void k_moments(const R* f, const R* Fcur, const uint8_t* type, Lattic
   const R invRho = R(1) / (rhoBar + R(1));
   const int64_t S = L.stride;
   const R F0 = Fcur ? Fcur[n] : R(0), F1 = Fcur ? Fcur[S + n] : R(0), F2 = Fcur ? Fcur[2 * S + n] : R(0);
-  double u0 = (double)(j0 * invRho + (R(g0) + F0) / R(2));
-  double u1 = (double)(j1 * invRho + (R(g1) + F1) / R(2));
-  double u2 = (double)(j2 * invRho + (R(g2) + F2) / R(2));
+  u0 = (double)(j0 * invRho + (R(a.g[0]) + F0) / R(2));
+  u1 = (double)(j1 * invRho + (R(a.g[1]) + F1) / R(2));
+  u2 = (double)(j2 * invRho + (R(a.g[2]) + F2) / R(2));
   double rb = (double)rhoBar;
   if (type && type[n] >= HC_NODE_VELOCITY) {
     const int face = type[n] - HC_NODE_VELOCITY, axis = face >> 1, orient = (face & 1) ? 1 : -1;
@@ -319,11 +317,72 @@ __global__ void k_moments(const R* f, const R* Fcur, const uint8_t* type, Lattic
     u1 = a.wall_u[face][1];
     u2 = a.wall_u[face][2];
   }
-  if (rho) rho[n] = rb + 1.0;
+  rho = rb + 1.0;
+}
+
+template <typename R>
+__global__ void k_moments(const R* f, const R* Fcur, const uint8_t* type, LatticeDev L, int phase, CollideArgs a,
+                          double* rho, double* u) {
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= L.N) return;
+  double r, u0, u1, u2;
+  node_moments<R>(f, Fcur, type, L, phase, a, n, r, u0, u1, u2);
+  if (rho) rho[n] = r;
   if (u) {
     u[3 * n + 0] = u0;
     u[3 * n + 1] = u1;
     u[3 * n + 2] = u2;
+  }
+}
+
+// FluidInfo statistics (helper/fluidInfo.cpp:27-102): minimum, maximum and sum of
+// a per-node magnitude over the nodes that are not bounce-back, and their count,
+// one partial per block (reduced on the host).  what 0: |u| as k_moments gives
+// it; what 1: |F| of the force field Fstat.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_fluid_stats(const R* f, const R* Fcur, const uint8_t* type, LatticeDev L,
+                                                       int phase, CollideArgs a, const R* Fstat, int what,
+                                                       double* partials) {
+  __shared__ double smin[BLOCK], smax[BLOCK], ssum[BLOCK], scnt[BLOCK];
+  double mn = INFINITY, mx = -INFINITY, sum = 0.0, cnt = 0.0;
+  const int64_t S = L.stride;
+  for (int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x; n < L.N; n += (int64_t)gridDim.x * BLOCK) {
+    if (type && type[n] == HC_NODE_BOUNCEBACK) continue;
+    double m;
+    if (what == 0) {
+      double r, u0, u1, u2;
+      node_moments<R>(f, Fcur, type, L, phase, a, n, r, u0, u1, u2);
+      m = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    } else {
+      const double F0 = (double)Fstat[n], F1 = (double)Fstat[S + n], F2 = (double)Fstat[2 * S + n];
+      m = sqrt(F0 * F0 + F1 * F1 + F2 * F2);
+    }
+    mn = fmin(mn, m);
+    mx = fmax(mx, m);
+    sum += m;
+    cnt += 1.0;
+  }
+  const int t = threadIdx.x;
+  smin[t] = mn;
+  smax[t] = mx;
+  ssum[t] = sum;
+  scnt[t] = cnt;
+  __syncthreads();
+  for (int h = BLOCK / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      smin[t] = fmin(smin[t], smin[t + h]);
+      smax[t] = fmax(smax[t], smax[t + h]);
+      ssum[t] += ssum[t + h];
+      scnt[t] += scnt[t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* p = partials + 4 * blockIdx.x;
+    p[0] = smin[0];
+    p[1] = smax[0];
+    p[2] = ssum[0];
+    p[3] = scnt[0];
   }
 }
 
@@ -399,8 +458,16 @@ hipError_t launch_store_populations(hipStream_t s, R* f, const double* in, const
 template <typename R>
 hipError_t launch_fluid_moments(hipStream_t s, const R* f, const R* Fcur, const uint8_t* type, const LatticeDev& L,
                                 int phase, const CollideArgs& a, double* rho, double* u) {
-  hipLaunchKernelGGL((k_moments<R>), grid_for(L.N), dim3(BLOCK), 0, s, f, Fcur, type, L, phase, a.g[0], a.g[1], a.g[2],
-                     a, rho, u);
+  hipLaunchKernelGGL((k_moments<R>), grid_for(L.N), dim3(BLOCK), 0, s, f, Fcur, type, L, phase, a, rho, u);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_fluid_stats(hipStream_t s, const R* f, const R* Fcur, const uint8_t* type, const LatticeDev& L,
+                              int phase, const CollideArgs& a, const R* Fstat, int what, double* partials,
+                              int nblocks) {
+  hipLaunchKernelGGL((k_fluid_stats<R>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, f, Fcur, type, L, phase, a, Fstat,
+                     what, partials);
   return hipGetLastError();
 }
 
@@ -425,6 +492,8 @@ hipError_t launch_to_double(hipStream_t s, const R* in, double* out, int64_t n) 
   template hipError_t launch_store_populations<R>(hipStream_t, R*, const double*, const LatticeDev&);           \
   template hipError_t launch_fluid_moments<R>(hipStream_t, const R*, const R*, const uint8_t*, const LatticeDev&,  \
                                               int, const CollideArgs&, double*, double*);                     \
+  template hipError_t launch_fluid_stats<R>(hipStream_t, const R*, const R*, const uint8_t*, const LatticeDev&,    \
+                                            int, const CollideArgs&, const R*, int, double*, int);             \
   template hipError_t launch_fill_zero<R>(hipStream_t, R*, int64_t);                                            \
   template hipError_t launch_to_double<R>(hipStream_t, const R*, double*, int64_t);
 HC_INSTANTIATE(float)

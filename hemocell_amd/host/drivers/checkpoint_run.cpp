@@ -99,8 +99,11 @@ static int run(int argc, char* argv[]) {
   std::fwrite(f.data(), 8, f.size(), o);
   std::fwrite(pop.data(), 8, pop.size(), o);
   std::fclose(o);
-  std::printf("{\"start\": %u, \"iteration\": %lld, \"cells\": %lld, \"vertices\": %lld, \"out\": \"%s\"}\n", start,
-              (long long)it, (long long)nc, (long long)nv, hemocell.outputDirectory().c_str());
+  const hemo::FluidStatistics fs = hemo::FluidInfo::calculateVelocityStatistics(&hemocell);
+  std::printf("{\"start\": %u, \"iteration\": %lld, \"cells\": %lld, \"vertices\": %lld, \"out\": \"%s\", "
+              "\"u_max\": %.17g, \"u_mean\": %.17g, \"fluid_nodes\": %lu}\n",
+              start, (long long)it, (long long)nc, (long long)nv, hemocell.outputDirectory().c_str(), fs.max, fs.avg,
+              (unsigned long)fs.ncells);
   return 0;
 }
 

@@ -753,6 +753,26 @@ void HemoCell::iterate() {
   ++iter;
 }
 
+// ============================================================= fluidInfo ====
+// helper/fluidInfo.cpp:102-141 (the reference's force gatherer never counts its
+// nodes, so its mean divides by zero; here both means divide by the node count)
+namespace {
+FluidStatistics fluid_statistics(HemoCell* h, int what) {
+  h->bindLattice();
+  double out[4];
+  h->check(hc_get_fluid_stats(h->context(), what, out), "FluidInfo");
+  FluidStatistics r;
+  r.min = out[0];
+  r.max = out[1];
+  r.avg = out[2];
+  r.ncells = (pluint)out[3];
+  return r;
+}
+}  // namespace
+
+FluidStatistics FluidInfo::calculateVelocityStatistics(HemoCell* hemocell) { return fluid_statistics(hemocell, 0); }
+FluidStatistics FluidInfo::calculateForceStatistics(HemoCell* hemocell) { return fluid_statistics(hemocell, 1); }
+
 // ============================================================== cellInfo ====
 std::map<int, CellInformation> CellInformationFunctionals::info_per_cell;
 

@@ -371,6 +371,21 @@ class HemoCell {
   std::string outDir;
 };
 
+// ------------------------------------------------------------- fluidInfo -----
+// helper/fluidInfo.h: minimum, maximum and mean of the fluid velocity magnitude,
+// or of the spread IBM force magnitude, over the nodes that are not bounce-back
+// (lattice units).  ncells is the number of nodes counted.
+struct FluidStatistics {
+  double min = 0, max = 0, avg = 0;
+  pluint ncells = 0;
+};
+class FluidInfo {
+ public:
+  static FluidStatistics calculateVelocityStatistics(HemoCell* hemocell);
+  // spreads the vertex forces into a reset field first (helper/fluidInfo.cpp:117-120)
+  static FluidStatistics calculateForceStatistics(HemoCell* hemocell);
+};
+
 // -------------------------------------------------------------- cellInfo -----
 // Per-cell statistics (helper/cellInfo.h): lattice units, keyed by cell id.
 struct CellInformation {

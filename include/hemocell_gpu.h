@@ -200,6 +200,11 @@ hc_status hc_get_cell_volume_area(hc_ctx* ctx, double* volume, double* area);
  * distance ("stretch"); lattice units; any pointer may be NULL. */
 hc_status hc_get_cell_stats(hc_ctx* ctx, double* volume, double* area, double* centroid,
                             double* velocity, double* bbox, double* stretch);
+/* Replaces: FluidInfo::calculateVelocityStatistics / calculateForceStatistics
+ * (helper/fluidInfo.cpp:102-141).  out = {min, max, mean, count} over the nodes that are
+ * not bounce-back of |u| (what 0, u as hc_get_fluid reports it) or of |F| (what 1, the
+ * vertex forces spread into a reset field, capped in place as the spread does). */
+hc_status hc_get_fluid_stats(hc_ctx* ctx, int what, double out[4]);
 /* vertices whose nearest node became a wall node during advance (the reference deletes
  * their cells, hemoCellParticleField.cpp:570-584) */
 int64_t hc_wall_hits(hc_ctx* ctx);

@@ -176,7 +176,8 @@ def test_host_layer_checkpoint_files_and_restart(tmp_path):
         for name in ("RBC.xml", "PLT.xml", "RBC.pos", "PLT.pos"):
             (d / name).write_text((tmp_path / name).read_text())
         (d / "config.xml").write_text(cases.config_xml())
-        _run([drv, "config.xml"] + common + ["14", "0", "out.bin"] + specs, cwd=d, env=env)
+        r = _run([drv, "config.xml"] + common + ["14", "0", "out.bin"] + specs, cwd=d, env=env)
+        (d / "stats.json").write_text(json.dumps(r))
     # a run that checkpoints at 4 and 8 and stops at 8
     split = tmp_path / "split"
     split.mkdir()
@@ -199,7 +200,11 @@ def test_host_layer_checkpoint_files_and_restart(tmp_path):
     res = _run([drv, "tmp/checkpoint/checkpoint.xml"] + common + ["14", "0", "resumed.bin"] + specs, cwd=split,
                env=env)
     assert res["start"] == 8 and res["iteration"] == 14 and res["out"] == "./tmp/"
+    assert res["fluid_nodes"] == nx * ny * nz  # velocity walls are not bounce-back: every node counts
     assert not (split / "tmp_0").exists()
+    whole_res = json.loads((whole / "stats.json").read_text())
+    assert abs(res["u_max"] - whole_res["u_max"]) <= 1e-10 * whole_res["u_max"]
+    assert abs(res["u_mean"] - whole_res["u_mean"]) <= 1e-10 * whole_res["u_mean"]
     a, a2, b = _read_out(whole / "out.bin"), _read_out(whole2 / "out.bin"), _read_out(split / "resumed.bin")
     assert a["it"] == b["it"] == 14
     assert np.array_equal(a["ids"], b["ids"]) and len(a["ids"]) == 3

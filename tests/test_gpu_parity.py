@@ -384,6 +384,42 @@ def test_cell_stats_match_reference_formulas():
     assert c == len(st["volume"])
 
 
+@pytest.mark.parametrize("precision", [hc.HC_FP64, hc.HC_FP32_SHIFTED])
+def test_fluid_statistics(precision):
+    """FluidInfo statistics (helper/fluidInfo.cpp:27-141) over the non-bounce-back
+    nodes: |u| against numpy on the device's own velocity field, |F| of the
+    vertex forces spread into a reset field against the FP64 oracle's spread of
+    the same positions and forces."""
+    u = cases.units()
+    nx, ny, nz = 40, 30, 36
+    mask = cases.channel_mask(nx, ny, nz, axis=1)
+    g = hc.Simulation(0)
+    g.lattice_init(nx, ny, nz, u.tau, periodic=(1, 0, 1), node_type=mask, precision=precision)
+    g.init_equilibrium(1.0, [0, 0, 0])
+    _mixed([g])
+    g.step(7)
+    fluid = mask == 0
+    mag = np.linalg.norm(g.fluid()[1][fluid], axis=1)
+    st = g.fluid_stats("velocity")
+    assert st["ncells"] == int(fluid.sum())
+    assert st["min"] == pytest.approx(mag.min(), rel=1e-14, abs=1e-300)
+    assert st["max"] == pytest.approx(mag.max(), rel=1e-14)
+    assert st["avg"] == pytest.approx(mag.mean(), rel=1e-12)
+    x, _, f = g.vertices()
+    sf = g.fluid_stats("force")
+    o = OracleSim(nx, ny, nz, u.tau, periodic=(1, 0, 1), node_type=mask)
+    o.set_force_limit(u.f_limit)
+    _mixed([o])
+    o.set_vertices(x, None, f)
+    o.set_force_field(np.zeros((3, nx * ny * nz)))
+    o.stage("spread")
+    fm = np.linalg.norm(o.force_field()[:, fluid], axis=0)
+    tol = 1e-12 if precision == hc.HC_FP64 else 1e-6
+    assert sf["ncells"] == int(fluid.sum())
+    assert sf["max"] == pytest.approx(fm.max(), rel=tol)
+    assert sf["avg"] == pytest.approx(fm.mean(), rel=tol)
+
+
 # ------------------------------------------------------------- step graphs --
 def test_graph_replays_match_eager_steps(monkeypatch):
     """Steady fused steps replayed from a captured two-step graph against the eager
