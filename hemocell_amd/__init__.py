@@ -103,6 +103,7 @@ SIGNATURES = {
     "hc_get_cell_volume_area": (_ST, [_P, _D, _D]),
     "hc_get_cell_stats": (_ST, [_P, _D, _D, _D, _D, _D, _D]),
     "hc_get_fluid_stats": (_ST, [_P, _INT, _D]),
+    "hc_get_force_terms": (_ST, [_P, _D, _D, _D, _D, _D, _D]),
     "hc_wall_hits": (ctypes.c_int64, [_P]),
     "hc_check_finite": (_ST, [_P]),
     "hc_profile_enable": (_ST, [_P, _INT]),
@@ -350,6 +351,14 @@ class Simulation:
                    bbox=np.zeros((nc, 6)), stretch=np.zeros(nc))
         self._chk(self.lib.hc_get_cell_stats(self.h, *[_dp(out[k]) for k in
                                                        ("volume", "area", "centroid", "velocity", "bbox", "stretch")]))
+        return out
+
+    def force_terms(self):
+        """Per-vertex force terms at the current state: dict of (nv, 3) arrays
+        area, volume, bending, link, visc, inner_link."""
+        names = ("area", "volume", "bending", "link", "visc", "inner_link")
+        out = {k: np.zeros((self.num_vertices, 3)) for k in names}
+        self._chk(self.lib.hc_get_force_terms(self.h, *[_dp(out[k]) for k in names]))
         return out
 
     def fluid_stats(self, what="velocity"):

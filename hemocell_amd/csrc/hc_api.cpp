@@ -1217,6 +1217,36 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u) {
   return HC_OK;
 }
 
+static hc_status get_planes(hc_ctx* ctx, const double* d, double* out);
+
+hc_status hc_get_force_terms(hc_ctx* ctx, double* area, double* volume, double* bending, double* link,
+                             double* visc, double* inner_link) {
+  if (!ctx) return HC_INVALID_ARG;
+  hc_status s = sync_cells(ctx);
+  if (s != HC_OK) return s;
+  if (ctx->nv == 0) return HC_OK;
+  double* outs[6] = {area, volume, bending, link, visc, inner_link};
+  const int terms[6] = {1, 2, 4, 8, 16, 32};  // TERM_AREA ... TERM_INNER of the mechanics kernels
+  double* d = nullptr;
+  if ((s = dalloc(ctx, (void**)&d, sizeof(double) * 3 * ctx->vcap, "force terms")) != HC_OK) return s;
+  for (int t = 0; t < 6 && s == HC_OK; ++t) {
+    if (!outs[t]) continue;
+    hipError_t e = launch_fill_zero<double>(ctx->stream, d, 3 * ctx->vcap);
+    for (auto& tp : ctx->types) {
+      if (e != hipSuccess) break;
+      TypeBufs& tb = *tp;
+      if (tb.ncells == 0) continue;
+      e = launch_mechanics(ctx->stream, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base, ctx->bend,
+                           ctx->lds_limit, terms[t], d);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) s = hipfail(ctx, e, "force terms");
+    else s = get_planes(ctx, d, outs[t]);
+  }
+  hipFree(d);
+  return s;
+}
+
 hc_status hc_get_fluid_stats(hc_ctx* ctx, int what, double* out) {
   hc_status s = need_lattice(ctx);
   if (s != HC_OK) return s;

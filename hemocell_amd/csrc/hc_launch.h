@@ -85,7 +85,7 @@ hipError_t launch_vertex_bins(hipStream_t s, const VertexDev& V, const LatticeDe
 
 // mechanics (hc_mech.hip)
 hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
-                            const int64_t* cell_base, double* bend_scratch, int lds_limit);
+                            const int64_t* cell_base, double* bend_scratch, int lds_limit, int term = 63, double* out = nullptr);
 hipError_t launch_cell_stats(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                              const int64_t* cell_base, double* out);
 hipError_t launch_check_finite(hipStream_t s, const double* p, int64_t n, unsigned int* bad);

@@ -26,9 +26,12 @@ namespace {
 
 // STAGED: the cell's positions, bending forces and (with membrane viscosity)
 // velocities live in LDS for the whole kernel (9 x 8 bytes per vertex at most).
-template <int MODEL, int BS, bool STAGED>
+// TERMS: the force terms computed (all for the step; one at a time, into `out`
+// planes of stride V.vs, for output).
+template <int MODEL, int BS, bool STAGED, int TERMS = TERM_ALL>
 __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
-                                                  const int64_t* __restrict__ cell_base, double* __restrict__ bend) {
+                                                  const int64_t* __restrict__ cell_base, double* __restrict__ bend,
+                                                  double* __restrict__ out) {
   extern __shared__ double sm[];
   __shared__ double red[BS];
   __shared__ double vforce_sh;
@@ -108,12 +111,13 @@ __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const 
 
   // ---- gather every term of vertex k in the reference's order ----------------
   for (int k = tid; k < T.nv; k += BS) {
-    const D3 F = vertex_force<MODEL>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
+    const D3 F = vertex_force<MODEL, TERMS>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
     const double F0 = F.x, F1 = F.y, F2 = F.z;
     const int64_t q = base + k;
-    V.f[q] = F0;
-    V.f[vs + q] = F1;
-    V.f[2 * vs + q] = F2;
+    double* dst = out ? out : V.f;
+    dst[q] = F0;
+    dst[vs + q] = F1;
+    dst[2 * vs + q] = F2;
   }
 }
 
@@ -207,16 +211,27 @@ __global__ void k_check_finite_f(const float* p, int64_t n, unsigned int* bad) {
 }  // namespace
 
 hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
-                            const int64_t* cell_base, double* bend, int lds_limit) {
+                            const int64_t* cell_base, double* bend, int lds_limit, int term, double* out) {
   if (ncells <= 0) return hipSuccess;
   dim3 g((unsigned)ncells);
   const bool visc = (T.model != HC_MODEL_RBC) || (T.eta_m != 0.0);
   const size_t lds = sizeof(double) * (size_t)T.nv * (visc ? 9 : 6);
   const bool staged = lds <= (size_t)std::min(lds_limit, 65536);
   const size_t dyn = staged ? lds : 0;
-#define HC_MECH(M, B)                                                                                           \
-  if (staged) hipLaunchKernelGGL((k_mechanics<M, B, true>), g, dim3(B), dyn, s, V, T, cells, cell_base, bend);   \
-  else hipLaunchKernelGGL((k_mechanics<M, B, false>), g, dim3(B), 0, s, V, T, cells, cell_base, bend);
+#define HC_MECH_T(M, B, TM)                                                                                      \
+  if (staged) hipLaunchKernelGGL((k_mechanics<M, B, true, TM>), g, dim3(B), dyn, s, V, T, cells, cell_base, bend,  \
+                                 out);                                                                            \
+  else hipLaunchKernelGGL((k_mechanics<M, B, false, TM>), g, dim3(B), 0, s, V, T, cells, cell_base, bend, out);
+#define HC_MECH(M, B)                                                      \
+  switch (term) {                                                          \
+    case TERM_AREA: HC_MECH_T(M, B, TERM_AREA) break;                      \
+    case TERM_VOLUME: HC_MECH_T(M, B, TERM_VOLUME) break;                  \
+    case TERM_BENDING: HC_MECH_T(M, B, TERM_BENDING) break;                \
+    case TERM_LINK: HC_MECH_T(M, B, TERM_LINK) break;                      \
+    case TERM_VISC: HC_MECH_T(M, B, TERM_VISC) break;                      \
+    case TERM_INNER: HC_MECH_T(M, B, TERM_INNER) break;                    \
+    default: HC_MECH_T(M, B, TERM_ALL) break;                              \
+  }
   switch (T.model) {
     case HC_MODEL_RBC:
       HC_MECH(HC_MODEL_RBC, 256)
@@ -231,6 +246,7 @@ hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T,
       return hipErrorInvalidValue;
   }
 #undef HC_MECH
+#undef HC_MECH_T
   return hipGetLastError();
 }
 

@@ -120,10 +120,17 @@ __device__ D3 bending_force(const CellView& c, const TypeDev& T, int i) {
   return {pn.x * s, pn.y * s, pn.z * s};
 }
 
+// Force terms, as the reference's models add them to force_area, force_volume,
+// force_bending, force_link, force_visc and force_inner_link
+// (rbcHighOrderModel.cpp, pltSimpleModel.cpp, wbcHighOrderModel.cpp).
+enum { TERM_AREA = 1, TERM_VOLUME = 2, TERM_BENDING = 4, TERM_LINK = 8, TERM_VISC = 16, TERM_INNER = 32,
+       TERM_ALL = 63 };
+
 // Force on vertex k of one cell: every term gathered in the reference's order
-// (area, volume, bending, edges, inner links).  bx/by/bz: per-vertex bending
-// forces of the cell; vx/vy/vz: vertex velocities (membrane viscosity).
-template <int MODEL>
+// (area, volume, bending, edges, inner links), or only the terms in TERMS.
+// bx/by/bz: per-vertex bending forces of the cell; vx/vy/vz: vertex velocities
+// (membrane viscosity).
+template <int MODEL, int TERMS = TERM_ALL>
 __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, int k, double vforce, const double* bx,
                                            const double* by, const double* bz, const double* vx, const double* vy,
                                            const double* vz, bool visc) {
@@ -176,12 +183,22 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
     } else {
       for (int e = T.vtri_off[k]; e < T.vtri_off[k + 1]; ++e) tri_term(T.vtri4[e], T.tri_area_eq[T.vtri4[e].x]);
     }
-    F0 = A0 + B0;
-    F1 = A1 + B1;
-    F2 = A2 + B2;
+    if ((TERMS & TERM_AREA) && (TERMS & TERM_VOLUME)) {
+      F0 = A0 + B0;
+      F1 = A1 + B1;
+      F2 = A2 + B2;
+    } else if (TERMS & TERM_AREA) {
+      F0 = A0;
+      F1 = A1;
+      F2 = A2;
+    } else if (TERMS & TERM_VOLUME) {
+      F0 = B0;
+      F1 = B1;
+      F2 = B2;
+    }
   }
   // bending: own force and -F_i/n_i of every vertex i whose 1-ring holds k, ascending i
-  if (MODEL != HC_MODEL_PLT) {
+  if (MODEL != HC_MODEL_PLT && (TERMS & TERM_BENDING)) {
     auto bend_term = [&](const int i, const double dn) {
       const double b0 = bx[i], b1 = by[i], b2 = bz[i];
       if (i == k) {
@@ -222,12 +239,14 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
     const D3 uv = {ev.x * il, ev.y * il, ev.z * il};
     if (kind != 2) {
       const double sgn = kind == 0 ? 1.0 : -1.0;
-      const double ef = (len - l0) / l0;
-      const double fs = T.k_link * (ef + ef / fabs(MaxCellPersistenceLength - ef * ef));
-      const D3 fl = {uv.x * fs, uv.y * fs, uv.z * fs};
-      if (sgn > 0) { F0 += fl.x; F1 += fl.y; F2 += fl.z; }
-      else { F0 -= fl.x; F1 -= fl.y; F2 -= fl.z; }
-      if (visc) {
+      if (TERMS & TERM_LINK) {
+        const double ef = (len - l0) / l0;
+        const double fs = T.k_link * (ef + ef / fabs(MaxCellPersistenceLength - ef * ef));
+        const D3 fl = {uv.x * fs, uv.y * fs, uv.z * fs};
+        if (sgn > 0) { F0 += fl.x; F1 += fl.y; F2 += fl.z; }
+        else { F0 -= fl.x; F1 -= fl.y; F2 -= fl.z; }
+      }
+      if ((TERMS & TERM_VISC) && visc) {
         const D3 rv = {vx[b] - vx[a], vy[b] - vy[a], vz[b] - vz[a]};
         const double pr = dotp(rv, uv);
         D3 fv = {(pr * uv.x) * T.eta_m, (pr * uv.y) * T.eta_m, (pr * uv.z) * T.eta_m};
@@ -242,7 +261,7 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
         else { F0 -= fv.x; F1 -= fv.y; F2 -= fv.z; }
       }
     }
-    if (MODEL == HC_MODEL_PLT) {
+    if (MODEL == HC_MODEL_PLT && (TERMS & TERM_BENDING)) {
       const int32_t* T0 = T.tri + 3 * T.edge_tris[2 * ed];
       const int32_t* T1 = T.tri + 3 * T.edge_tris[2 * ed + 1];
       const D3 n1 = unit_normal(c.pos(T0[0]), c.pos(T0[1]), c.pos(T0[2]));
@@ -270,7 +289,7 @@ __device__ __forceinline__ D3 vertex_force(const CellView& c, const TypeDev& T, 
     for (int e = T.vedge_off[k]; e < T.vedge_off[k + 1]; ++e) edge_term(T.vedge4[e], T.edge_len_eq[T.vedge4[e].x >> 2]);
   }
   // inner links, inner-edge order
-  for (int e = T.vinner_off[k]; e < T.vinner_off[k + 1]; ++e) {
+  for (int e = T.vinner_off[k]; (TERMS & TERM_INNER) && e < T.vinner_off[k + 1]; ++e) {
     const int ie = T.vinner[e] >> 1, end = T.vinner[e] & 1;
     const D3 pa = c.pos(T.inner[2 * ie]), pb = c.pos(T.inner[2 * ie + 1]);
     const D3 ev = pb - pa;

@@ -73,7 +73,10 @@ static int run(int argc, char* argv[]) {
     // step, then write the HDF5 / CSV output of the reference layout under ./tmp
     for (auto& fp : hemocell.cellfields->fields)
       hemocell.setOutputs(fp->name, {hemo::OUTPUT_POSITION, hemo::OUTPUT_VELOCITY, hemo::OUTPUT_FORCE,
-                                     hemo::OUTPUT_TRIANGLES, hemo::OUTPUT_CELL_ID, hemo::OUTPUT_VERTEX_ID});
+                                     hemo::OUTPUT_TRIANGLES, hemo::OUTPUT_CELL_ID, hemo::OUTPUT_VERTEX_ID,
+                                     hemo::OUTPUT_FORCE_AREA, hemo::OUTPUT_FORCE_VOLUME, hemo::OUTPUT_FORCE_BENDING,
+                                     hemo::OUTPUT_FORCE_LINK, hemo::OUTPUT_FORCE_VISC,
+                                     hemo::OUTPUT_FORCE_INNER_LINK});
     hemocell.setFluidOutputs({hemo::OUTPUT_VELOCITY, hemo::OUTPUT_DENSITY, hemo::OUTPUT_BOUNDARY});
     const int n = std::atoi(steps);
     for (int k = 0; k < n; ++k) hemocell.iterate();

@@ -15,9 +15,10 @@
 //     (fluid, bounce-back, velocity wall) set with defineBounceBack() and the
 //     boundary-condition helpers below, not with Palabos dynamics objects.
 //   * writeOutput() writes the HDF5 and CSV files of the reference's layout
-//     (positions, velocities, total and repulsion forces, ids, triangles; fluid
-//     velocity, density and boundary); per-term force splits, checkpoints and
-//     load balancing are not produced (those calls are accepted and do nothing).
+//     (positions, velocities, total, repulsion and per-term forces, ids,
+//     triangles; fluid velocity, density and boundary); checkpoints are this
+//     build's own binary format behind the reference's checkpoint.xml; load
+//     balancing does not exist (one device holds the domain).
 //   * Errors that the reference reports with a log line and exit(1) do the same
 //     here; errors from the device return as hemo::HemoCellError.
 #pragma once

@@ -130,6 +130,7 @@ void HemoCell::writeOutput() {
   // ---- cells ----
   const int64_t nv = hc_num_vertices(ctx);
   std::vector<double> x((size_t)nv * 3), v((size_t)nv * 3), f((size_t)nv * 3), fr;
+  std::vector<std::vector<double>> terms;  // area, volume, bending, link, visc, inner link (on first use)
   if (nv) check(hc_get_vertices(ctx, x.data(), v.data(), f.data()), "writeOutput");
   if (cellfields) {
     // vertex base of every device cell
@@ -165,13 +166,36 @@ void HemoCell::writeOutput() {
         } else if (o == OUTPUT_VELOCITY) {
           vec3(v, si ? dx / dt : 1.0, "Velocity");
         } else if (o == OUTPUT_FORCE) {
-          vec3(f, si ? df : 1.0, "Total force");
+          // force_total = force + force_repulsion (hemoCellParticleField.cpp:596)
+          if (fr.empty()) {
+            fr.assign((size_t)nv * 3, 0.0);
+            if (repulsionEnabled && nv) check(hc_get_repulsion_force(ctx, fr.data()), "writeOutput");
+          }
+          std::vector<double> tot(f);
+          for (size_t k = 0; k < tot.size(); ++k) tot[k] += fr[k];
+          vec3(tot, si ? df : 1.0, "Total force");
         } else if (o == OUTPUT_FORCE_REPULSION) {
           if (fr.empty()) {
             fr.assign((size_t)nv * 3, 0.0);
             if (repulsionEnabled && nv) check(hc_get_repulsion_force(ctx, fr.data()), "writeOutput");
           }
           vec3(fr, si ? df : 1.0, "Repulsion force");
+        } else if (o == OUTPUT_FORCE_AREA || o == OUTPUT_FORCE_VOLUME || o == OUTPUT_FORCE_BENDING ||
+                   o == OUTPUT_FORCE_LINK || o == OUTPUT_FORCE_VISC || o == OUTPUT_FORCE_INNER_LINK) {
+          // separateForceVectors + applyConstitutiveModel(true): every term on its own
+          // (hemoCellParticleField.cpp:590-614, hemoCellParticleFieldOutputFunctions.cpp:113-288)
+          if (terms.empty()) {
+            terms.assign(6, std::vector<double>((size_t)nv * 3, 0.0));
+            if (nv)
+              check(hc_get_force_terms(ctx, terms[0].data(), terms[1].data(), terms[2].data(), terms[3].data(),
+                                       terms[4].data(), terms[5].data()),
+                    "writeOutput");
+          }
+          const int k = o == OUTPUT_FORCE_AREA ? 0 : o == OUTPUT_FORCE_VOLUME ? 1 : o == OUTPUT_FORCE_BENDING ? 2
+                        : o == OUTPUT_FORCE_LINK ? 3 : o == OUTPUT_FORCE_VISC ? 4 : 5;
+          static const char* names[6] = {"Area force", "Volume force", "Bending force", "Link force",
+                                         "Viscous force", "Inner link force"};
+          vec3(terms[(size_t)k], si ? df : 1.0, names[k]);
         } else if (o == OUTPUT_CELL_ID || o == OUTPUT_VERTEX_ID) {
           std::vector<float> ids;
           for (size_t c : cells)
@@ -190,8 +214,6 @@ void HemoCell::writeOutput() {
           long n = (long)nt;
           H5LTset_attribute_long(file, "/", "numberOfTriangles", &n, 1);
         }
-        // the per-term forces (OUTPUT_FORCE_AREA, _BENDING, _LINK, _VOLUME, _VISC,
-        // _INNER_LINK) are not separated by the device mechanics and are not written
       }
       H5Fclose(file);
     }

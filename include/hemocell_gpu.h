@@ -200,6 +200,13 @@ hc_status hc_get_cell_volume_area(hc_ctx* ctx, double* volume, double* area);
  * distance ("stretch"); lattice units; any pointer may be NULL. */
 hc_status hc_get_cell_stats(hc_ctx* ctx, double* volume, double* area, double* centroid,
                             double* velocity, double* bbox, double* stretch);
+/* Replaces: the per-term force vectors of HemoCellParticleField::separateForceVectors
+ * (core/hemoCellParticleField.cpp:590-614) that writeOutput recomputes with
+ * applyConstitutiveModel(true): the area, volume, bending, link, membrane-viscosity and
+ * inner-link forces of every vertex at the current positions and velocities, n*3+c;
+ * any pointer may be NULL.  Their sum is the mechanics force to round-off. */
+hc_status hc_get_force_terms(hc_ctx* ctx, double* area, double* volume, double* bending, double* link,
+                             double* visc, double* inner_link);
 /* Replaces: FluidInfo::calculateVelocityStatistics / calculateForceStatistics
  * (helper/fluidInfo.cpp:102-141).  out = {min, max, mean, count} over the nodes that are
  * not bounce-back of |u| (what 0, u as hc_get_fluid reports it) or of |F| (what 1, the

@@ -100,7 +100,8 @@ def _h5(path, what):
 def test_write_output_in_the_reference_layout(tmp_path):
     """writeOutput(): per-type and fluid HDF5 files and per-type CSV files with the
     reference's names, datasets, attributes and SI scaling
-    (io/ParticleHdf5IO.cpp, io/FluidHdf5IO.hh, io/writeCellInfoCSV.cpp)."""
+    (io/ParticleHdf5IO.cpp, io/FluidHdf5IO.hh, io/writeCellInfoCSV.cpp); the
+    per-term force datasets (stored as float32) add up to "Total force"."""
     u = cases.units()
     nx, ny, nz = 40, 30, 36
     (tmp_path / "config.xml").write_text(cases.config_xml())
@@ -132,6 +133,13 @@ def test_write_output_in_the_reference_layout(tmp_path):
     assert _h5(rbc, "Cell Id")["data"] == [0.0] * 642 + [1.0] * 642
     assert _h5(rbc, "Vertex Id")["data"][:642] == [float(k) for k in range(642)]
     assert _h5(d / ("PLT.%s.p.0.h5" % it), "Position")["dims"] == [66, 3]
+    # per-term forces (separateForceVectors): the six terms add up to the total
+    total = np.array(_h5(rbc, "Total force")["data"]).reshape(-1, 3)
+    parts = sum(np.array(_h5(rbc, n)["data"]).reshape(-1, 3)
+                for n in ("Area force", "Volume force", "Bending force", "Link force", "Viscous force",
+                          "Inner link force"))
+    assert np.abs(total).max() > 0
+    assert np.abs(parts - total).max() <= 1e-5 * np.abs(total).max()
     fl = d / ("Fluid.%s.p.0.h5" % it)
     fa = _h5(fl, "--attrs")
     assert fa["subdomainSize"] == [nz + 2, ny + 2, nx + 2] and fa["numberOfCells"] == [(nx + 2) * (ny + 2) * (nz + 2)]

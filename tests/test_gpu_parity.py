@@ -156,6 +156,58 @@ def test_mechanics_stage_fp64(kind):
     assert cases.rel_l2(fg, fo) < 1e-11
 
 
+@pytest.mark.parametrize("kind", ["RBC", "PLT", "WBC"])
+def test_force_terms_match_single_stiffness_oracle(kind):
+    """hc_get_force_terms (the per-term vectors writeOutput separates): each term
+    equals the oracle's whole mechanics with every other stiffness set to zero
+    (area: k_area; volume: k_volume; bending: k_bend; link and, for platelets, the
+    inner links: k_link; viscosity: eta_m; WBC inner links: k_inner_rigid and
+    k_cytoskeleton), and the six terms add up to the device's full mechanics."""
+    rng = np.random.default_rng(31)
+    u = cases.units()
+    model, v, t, p, inner = cases.mesh(kind, u, hc.build_mesh)
+    x = cases.perturb(cases.place(v, (12.3, 11.7, 12.9), cases.rotation(rng)), rng, 0.05)
+    vel = 1e-4 * rng.uniform(-1, 1, x.shape)
+    g = hc.Simulation(0)
+    g.lattice_init(26, 24, 26, u.tau)
+    tid = g.add_cell_type(model, v, t, p, inner)
+    g.add_cells(tid, x[None])
+    g.set_vertices(x, vel, None)
+    terms = g.force_terms()
+    g.stage("mechanics", 1)
+    full = g.vertices()[2]
+    total = sum(terms.values())
+    assert cases.rel_l2(total, full) < 1e-13
+    fields = [f[0] for f in type(p)._fields_]
+
+    def oracle_with(keep):
+        q = type(p).from_buffer_copy(p)
+        for name in ("k_link", "k_bend", "k_volume", "k_area", "eta_m", "k_inner_rigid", "k_cytoskeleton"):
+            if name in fields and name not in keep:
+                setattr(q, name, 0.0)
+        o = OracleSim(26, 24, 26, u.tau)
+        ot = o.add_cell_type(model, v, t, q, inner)
+        o.add_cells(ot, x[None])
+        o.set_vertices(x, vel, None)
+        o.stage("mechanics", 1)
+        return o.vertices()[2]
+
+    checks = [("area", ("k_area",)), ("volume", ("k_volume",)), ("bending", ("k_bend",))]
+    if kind == "PLT":
+        checks.append(("link+inner_link", ("k_link",)))
+    else:
+        checks.append(("link", ("k_link",)))
+    if p.eta_m != 0.0:  # the test materials give only the WBC membrane viscosity
+        checks.append(("visc", ("eta_m",)))
+    if kind == "WBC":
+        checks.append(("inner_link", ("k_inner_rigid", "k_cytoskeleton")))
+    for name, keep in checks:
+        dev = sum(terms[n] for n in name.split("+"))
+        ref = oracle_with(keep)
+        assert np.abs(ref).max() > 0, name
+        assert cases.rel_l2(dev, ref) < 1e-11, name
+
+
 def test_mechanics_rest_shape_is_force_free():
     u = cases.units()
     g = hc.Simulation(0)
