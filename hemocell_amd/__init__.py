@@ -97,6 +97,9 @@ SIGNATURES = {
     "hc_get_vertices": (_ST, [_P, _D, _D, _D]),
     "hc_set_vertices": (_ST, [_P, _D, _D, _D]),
     "hc_get_repulsion_force": (_ST, [_P, _D]),
+    "hc_get_boundary_repulsion_force": (_ST, [_P, _D]),
+    "hc_set_boundary_repulsion": (_ST, [_P, ctypes.c_double, ctypes.c_double, _INT]),
+    "hc_stage_boundary_repulsion": (_ST, [_P]),
     "hc_set_repulsion_force": (_ST, [_P, _D]),
     "hc_get_cells": (_ST, [_P, _I32, _I32]),
     "hc_get_vertex_bins": (_ST, [_P, _I64]),
@@ -251,6 +254,14 @@ class Simulation:
 
     def set_repulsion(self, k_rep, cutoff_lu, timescale=1):
         self._chk(self.lib.hc_set_repulsion(self.h, k_rep, cutoff_lu, timescale))
+
+    def set_boundary_repulsion(self, k_rep, cutoff_lu, timescale=1):
+        self._chk(self.lib.hc_set_boundary_repulsion(self.h, k_rep, cutoff_lu, timescale))
+
+    def boundary_repulsion_force(self):
+        fr = np.zeros((self.num_vertices, 3))
+        self._chk(self.lib.hc_get_boundary_repulsion_force(self.h, _dp(fr)))
+        return fr
 
     def set_extra_vertex_force(self, ids, forces):
         ids = np.ascontiguousarray(ids, dtype=np.int64)

@@ -71,7 +71,7 @@ struct hc_ctx {
   std::vector<int32_t> cell_type, cell_id;
   std::vector<int64_t> cell_base;
   int64_t nv = 0, vcap = 0;
-  double *x = nullptr, *v = nullptr, *fv = nullptr, *frep = nullptr, *bend = nullptr;
+  double *x = nullptr, *v = nullptr, *fv = nullptr, *frep = nullptr, *bend = nullptr, *fbr = nullptr;
   int64_t* d_cell_base = nullptr;
   int32_t* d_cell_nv = nullptr;
   bool cells_dirty = true;
@@ -108,6 +108,11 @@ struct hc_ctx {
   bool repulsion = false;
   double rep_k = 0, rep_cutoff = 0;
   int rep_timescale = 1;
+  // wall repulsion from boundary particles (flags per node, built on first use)
+  bool brep = false;
+  double brep_k = 0, brep_cutoff = 0;
+  int brep_timescale = 1;
+  uint8_t* d_bflag = nullptr;
   // binning and repulsion buffers, built on first use and whenever the vertex count changed
   RepulsionBufs rb{};
   int64_t rb_nv = -1;
@@ -145,7 +150,7 @@ hc_status dalloc(hc_ctx* ctx, void** p, size_t bytes, const char* what) {
   return HC_OK;
 }
 
-VertexDev vdev(hc_ctx* c) { return VertexDev{c->x, c->v, c->fv, c->frep, c->nv, c->vcap}; }
+VertexDev vdev(hc_ctx* c) { return VertexDev{c->x, c->v, c->fv, c->frep, c->fbr, c->nv, c->vcap}; }
 
 void prof_begin(hc_ctx* c, int stage, hipEvent_t* start) {
   *start = nullptr;
@@ -193,9 +198,9 @@ hc_status grow_vertices(hc_ctx* ctx, int64_t need) {
   if (need <= ctx->vcap) return HC_OK;
   int64_t cap = std::max<int64_t>(need, ctx->vcap * 2);
   cap = (cap + 63) / 64 * 64;
-  double* arrs[5] = {ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend};
-  double* nw[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  for (int a = 0; a < 5; ++a) {
+  double* arrs[6] = {ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend, ctx->fbr};
+  double* nw[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int a = 0; a < 6; ++a) {
     hc_status s = dalloc(ctx, (void**)&nw[a], sizeof(double) * 3 * cap, "vertex state");
     if (s != HC_OK) {
       for (int b = 0; b < a; ++b) hipFree(nw[b]);
@@ -208,13 +213,14 @@ hc_status grow_vertices(hc_ctx* ctx, int64_t need) {
                               hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
   }
   HC_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  for (int a = 0; a < 5; ++a)
+  for (int a = 0; a < 6; ++a)
     if (arrs[a]) hipFree(arrs[a]);
   ctx->x = nw[0];
   ctx->v = nw[1];
   ctx->fv = nw[2];
   ctx->frep = nw[3];
   ctx->bend = nw[4];
+  ctx->fbr = nw[5];
   ctx->vcap = cap;
   return HC_OK;
 }
@@ -346,6 +352,24 @@ hc_status do_repulsion(hc_ctx* ctx) {
   return HC_OK;
 }
 
+hc_status do_boundary_repulsion(hc_ctx* ctx) {
+  hc_status s;
+  if (!ctx->d_bflag) {
+    if ((s = dalloc(ctx, (void**)&ctx->d_bflag, std::max<int64_t>(ctx->L.N, 1), "boundary particles")) != HC_OK)
+      return s;
+    HC_TRY(launch_boundary_flags(ctx->stream, ctx->type, ctx->L, ctx->d_bflag), "boundary particles");
+  }
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_REPULSION, &ev);
+  HC_TRY(launch_boundary_repulsion(ctx->stream, vdev(ctx), ctx->L, ctx->d_bflag, ctx->brep_k, ctx->brep_cutoff),
+         "boundary repulsion");
+  prof_end(ctx, HC_PROF_REPULSION, ev);
+  return HC_OK;
+}
+
+// the repulsion terms the spread adds: 1 cell-cell, 2 wall
+int rep_mask(const hc_ctx* c) { return (c->repulsion ? 1 : 0) | (c->brep ? 2 : 0); }
+
 template <typename R>
 hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
   hipEvent_t ev;
@@ -403,12 +427,12 @@ hc_status do_spread(hc_ctx* ctx) {
     // LDS budget (a box of doubles): room for the support box of an aligned RBC
     const int want = ctx->spread_lds > 0 ? ctx->spread_lds : 65536;
     HC_TRY(launch_spread_cells<R>(ctx->stream, vdev(ctx), (R*)ctx->Fibm[ctx->cur], ctx->type, ctx->L, ctx->d_cell_base,
-                                  ctx->d_cell_nv, (int)ctx->cell_base.size(), ctx->f_limit, ctx->repulsion ? 1 : 0,
+                                  ctx->d_cell_nv, (int)ctx->cell_base.size(), ctx->f_limit, rep_mask(ctx),
                                   std::min(want, ctx->lds_limit)),
            "spread");
   } else {
     HC_TRY(launch_spread<R>(ctx->stream, vdev(ctx), (R*)ctx->Fibm[ctx->cur], ctx->type, ctx->L, ctx->f_limit,
-                            ctx->repulsion ? 1 : 0), "spread");
+                            rep_mask(ctx)), "spread");
   }
   prof_end(ctx, HC_PROF_SPREAD, ev);
   return HC_OK;
@@ -480,8 +504,10 @@ hc_status one_step(hc_ctx* ctx) {
   hc_status s;
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_STEP, &ev);
-  const bool fused = ctx->fuse && !ctx->repulsion && ctx->nv > 0;
+  const bool fused = ctx->fuse && !ctx->repulsion && !ctx->brep && ctx->nv > 0;
   if (ctx->repulsion && ctx->nv > 0 && ctx->iter % ctx->rep_timescale == 0 && (s = do_repulsion(ctx)) != HC_OK)
+    return s;
+  if (ctx->brep && ctx->nv > 0 && ctx->iter % ctx->brep_timescale == 0 && (s = do_boundary_repulsion(ctx)) != HC_OK)
     return s;
   if (ctx->nv > 0 && !ctx->spread_ready && (s = do_spread<R>(ctx)) != HC_OK) return s;
   if ((s = do_collide<R>(ctx)) != HC_OK) return s;
@@ -563,7 +589,8 @@ std::vector<unsigned char> launch_signature(const hc_ctx* c) {
 
 // Two steps repeat exactly when the step is fused and every timescale is 1.
 bool graph_eligible(const hc_ctx* c) {
-  if (!c->use_graphs || c->prof.on || !c->fuse || c->repulsion || c->nv == 0 || !c->spread_ready) return false;
+  if (!c->use_graphs || c->prof.on || !c->fuse || c->repulsion || c->brep || c->nv == 0 || !c->spread_ready)
+    return false;
   if (c->pv_timescale != 1) return false;
   for (const auto& tp : c->types)
     if (tp->ncells && tp->host.timescale != 1) return false;
@@ -672,7 +699,7 @@ static hc_status fluid_stats(hc_ctx* ctx, int what, double* out) {
     e = launch_fill_zero<R>(ctx->stream, Fs, 3 * ctx->L.stride);
     if (e == hipSuccess && ctx->nv > 0 && sync_cells(ctx) == HC_OK)
       e = launch_spread_cells<R>(ctx->stream, vdev(ctx), Fs, ctx->type, ctx->L, ctx->d_cell_base, ctx->d_cell_nv,
-                                 (int)ctx->cell_base.size(), ctx->f_limit, ctx->repulsion ? 1 : 0,
+                                 (int)ctx->cell_base.size(), ctx->f_limit, rep_mask(ctx),
                                  std::min(65536, ctx->lds_limit));
   }
   if (e == hipSuccess)
@@ -765,7 +792,7 @@ void hc_destroy(hc_ctx* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->d_extra_dense, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
-                ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag};
+                ctx->fbr, ctx->d_bflag, ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag};
   for (void* p : ps)
     if (p) hipFree(p);
   for (auto& tp : ctx->types) {
@@ -1029,6 +1056,21 @@ hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int time
   return HC_OK;
 }
 
+hc_status hc_set_boundary_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (!std::isfinite(k_rep) || !(cutoff_lu > 0.0) || !std::isfinite(cutoff_lu) || timescale < 1)
+    return fail(ctx, HC_INVALID_ARG, "boundary repulsion needs a finite constant, a cut-off > 0 and a timescale >= 1");
+  if (ctx->lattice_ready) {
+    hc_status s = invalidate_spread(ctx);
+    if (s != HC_OK) return s;
+  }
+  ctx->brep = true;
+  ctx->brep_k = k_rep;
+  ctx->brep_cutoff = cutoff_lu;
+  ctx->brep_timescale = timescale;
+  return HC_OK;
+}
+
 hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* ids, const double* forces) {
   if (!ctx || n < 0 || (n > 0 && (!ids || !forces))) return fail(ctx, HC_INVALID_ARG, "bad extra-force arguments");
   if (ctx->lattice_ready) {
@@ -1125,6 +1167,13 @@ hc_status hc_stage_repulsion(hc_ctx* ctx) {
   if (!ctx->repulsion) return fail(ctx, HC_STATE, "repulsion not enabled (hc_set_repulsion)");
   if ((s = invalidate_spread(ctx)) != HC_OK) return s;
   return ctx->nv > 0 ? do_repulsion(ctx) : HC_OK;
+}
+hc_status hc_stage_boundary_repulsion(hc_ctx* ctx) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  if (!ctx->brep) return fail(ctx, HC_STATE, "boundary repulsion not enabled (hc_set_boundary_repulsion)");
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
+  return ctx->nv > 0 ? do_boundary_repulsion(ctx) : HC_OK;
 }
 hc_status hc_stage_end(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
@@ -1296,6 +1345,11 @@ hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep) {
   if (!ctx || !frep) return HC_INVALID_ARG;
   HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
   return get_planes(ctx, ctx->frep, frep);
+}
+hc_status hc_get_boundary_repulsion_force(hc_ctx* ctx, double* fbr) {
+  if (!ctx || !fbr) return HC_INVALID_ARG;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  return get_planes(ctx, ctx->fbr, fbr);
 }
 hc_status hc_set_repulsion_force(hc_ctx* ctx, const double* frep) {
   if (!ctx) return HC_INVALID_ARG;

@@ -27,6 +27,26 @@ namespace {
 
 constexpr int BLOCK = 256;
 
+// The repulsion force a vertex spreads (force_repulsion of the reference): bit 1
+// of use_rep adds the cell-cell term, bit 2 the wall term, cell-cell first.
+__device__ __forceinline__ void repulsion_sum(const VertexDev& V, int64_t i, int use_rep, double& r0, double& r1,
+                                              double& r2) {
+  const int64_t vs = V.vs;
+  r0 = 0.0;
+  r1 = 0.0;
+  r2 = 0.0;
+  if (use_rep & 1) {
+    r0 = V.frep[i];
+    r1 = V.frep[vs + i];
+    r2 = V.frep[2 * vs + i];
+  }
+  if (use_rep & 2) {
+    r0 = r0 + V.fbr[i];
+    r1 = r1 + V.fbr[vs + i];
+    r2 = r2 + V.fbr[2 * vs + i];
+  }
+}
+
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_spread(VertexDev V, R* __restrict__ Fibm, const uint8_t* __restrict__ type,
                                                   LatticeDev L, double f_limit, int use_rep) {
@@ -48,9 +68,11 @@ __global__ __launch_bounds__(BLOCK) void k_spread(VertexDev V, R* __restrict__ F
   }
   double t0 = f0, t1 = f1, t2 = f2;
   if (use_rep) {
-    t0 = V.frep[i] + f0;
-    t1 = V.frep[vs + i] + f1;
-    t2 = V.frep[2 * vs + i] + f2;
+    double r0, r1, r2;
+    repulsion_sum(V, i, use_rep, r0, r1, r2);
+    t0 = r0 + f0;
+    t1 = r1 + f1;
+    t2 = r2 + f2;
   }
   const int64_t S = L.stride;
 #pragma unroll
@@ -167,9 +189,11 @@ __device__ __forceinline__ void spread_force(const VertexDev& V, int64_t i, doub
   t1 = f1;
   t2 = f2;
   if (use_rep) {
-    t0 = V.frep[i] + f0;
-    t1 = V.frep[vs + i] + f1;
-    t2 = V.frep[2 * vs + i] + f2;
+    double r0, r1, r2;
+    repulsion_sum(V, i, use_rep, r0, r1, r2);
+    t0 = r0 + f0;
+    t1 = r1 + f1;
+    t2 = r2 + f2;
   }
 }
 

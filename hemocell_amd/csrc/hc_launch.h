@@ -42,7 +42,8 @@ struct VertexDev {
   double* x;      // 3 components, stride vs
   double* v;
   double* f;
-  double* frep;
+  double* frep;   // cell-cell repulsion force
+  double* fbr;    // wall (boundary particle) repulsion force
   int64_t n;      // vertices
   int64_t vs;     // component stride
 };
@@ -81,6 +82,11 @@ struct RepulsionBufs {
 size_t repulsion_temp_bytes(int64_t nv, int64_t N);
 hipError_t launch_repulsion(hipStream_t s, const VertexDev& V, const LatticeDev& L, const RepulsionBufs& B,
                             double k_rep, double r_c);
+// boundary particles (bounce-back nodes next to a non-bounce-back node) and the
+// repulsion they exert on nearby vertices (hc_repulsion.hip)
+hipError_t launch_boundary_flags(hipStream_t s, const uint8_t* type, const LatticeDev& L, uint8_t* flags);
+hipError_t launch_boundary_repulsion(hipStream_t s, const VertexDev& V, const LatticeDev& L, const uint8_t* flags,
+                                     double k, double r_c);
 hipError_t launch_vertex_bins(hipStream_t s, const VertexDev& V, const LatticeDev& L, int64_t* out);
 
 // mechanics (hc_mech.hip)

@@ -3,6 +3,7 @@
 // Replaces (reference paths relative to the HemoCell tree):
 //   HemoCellParticleField::update_pg           core/hemoCellParticleField.cpp:137-168
 //   HemoCellParticleField::applyRepulsionForce core/hemoCellParticleField.cpp:677-743
+//   populateBoundaryParticles / applyBoundaryRepulsionForce  :865-918
 //
 // Binning: every vertex gets the key of its nearest node, floor(x + 0.5) per
 // axis (wrapped on periodic axes; vertices outside a non-periodic axis get the
@@ -88,6 +89,75 @@ __global__ void k_bin_clear(const uint32_t* __restrict__ keys, int64_t n, uint32
   if (p >= n) return;
   const uint32_t k = keys[p];
   if (k < N) bins[k] = make_int2(0, 0);
+}
+
+// Boundary particles (populateBoundaryParticles, hemoCellParticleField.cpp:865-887):
+// bounce-back nodes with a node that is not bounce-back among their 26
+// neighbours; neighbours wrap on periodic axes and end at non-periodic faces.
+__global__ void k_boundary_flags(const uint8_t* __restrict__ type, LatticeDev L, uint8_t* __restrict__ flags) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= L.N) return;
+  uint8_t flag = 0;
+  if (type && type[n] == HC_NODE_BOUNCEBACK) {
+    const int z = (int)(n % L.nz);
+    const int64_t t = n / L.nz;
+    const int y = (int)(t % L.ny), x = (int)(t / L.ny);
+    for (int dx = -1; dx <= 1 && !flag; ++dx) {
+      const int xx = wrap_axis(x + dx, L.nx, L.periodic[0]);
+      if (xx < 0) continue;
+      for (int dy = -1; dy <= 1 && !flag; ++dy) {
+        const int yy = wrap_axis(y + dy, L.ny, L.periodic[1]);
+        if (yy < 0) continue;
+        for (int dz = -1; dz <= 1 && !flag; ++dz) {
+          const int zz = wrap_axis(z + dz, L.nz, L.periodic[2]);
+          if (zz < 0) continue;
+          if (type[node_index(xx, yy, zz, L)] != HC_NODE_BOUNCEBACK) flag = 1;
+        }
+      }
+    }
+  }
+  flags[n] = flag;
+}
+
+// Wall repulsion (applyBoundaryRepulsionForce, hemoCellParticleField.cpp:889-918):
+// every boundary particle b adds k (r_c / r) r_hat, r = x - b, to each vertex of
+// the bins in its 3x3x3 neighbourhood closer than r_c.  Gathered per vertex: the
+// boundary particles in the 3x3x3 neighbourhood of the vertex's own bin, in
+// ascending x, y, z (the reference's order of boundary particles), the separation
+// taken to the unwrapped node, i.e. the minimum image.  Overwrites fbr.
+__global__ void k_boundary_repulsion(VertexDev V, LatticeDev L, const uint8_t* __restrict__ flags, double k_rep,
+                                     double r_c) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V.n) return;
+  const int64_t vs = V.vs;
+  const double p[3] = {V.x[i], V.x[vs + i], V.x[2 * vs + i]};
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+  if (fabs(p[0]) < 1.0e9 && fabs(p[1]) < 1.0e9 && fabs(p[2]) < 1.0e9) {
+    const int64_t c0 = (int64_t)floor(p[0] + 0.5), c1 = (int64_t)floor(p[1] + 0.5), c2 = (int64_t)floor(p[2] + 0.5);
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int xx = wrap_axis(c0 + dx, L.nx, L.periodic[0]);
+      if (xx < 0) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = wrap_axis(c1 + dy, L.ny, L.periodic[1]);
+        if (yy < 0) continue;
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int zz = wrap_axis(c2 + dz, L.nz, L.periodic[2]);
+          if (zz < 0 || !flags[node_index(xx, yy, zz, L)]) continue;
+          const double d0 = p[0] - (double)(c0 + dx), d1 = p[1] - (double)(c1 + dy), d2 = p[2] - (double)(c2 + dz);
+          const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+          if (dist < r_c) {
+            const double s = k_rep * (1 / (dist / r_c));
+            F0 = F0 + s * (d0 / dist);
+            F1 = F1 + s * (d1 / dist);
+            F2 = F2 + s * (d2 / dist);
+          }
+        }
+      }
+    }
+  }
+  V.fbr[i] = F0;
+  V.fbr[vs + i] = F1;
+  V.fbr[2 * vs + i] = F2;
 }
 
 __global__ void k_vertex_bins(const VertexDev V, LatticeDev L, int64_t* __restrict__ out) {
@@ -226,6 +296,19 @@ hipError_t launch_repulsion(hipStream_t s, const VertexDev& V, const LatticeDev&
   hipLaunchKernelGGL(k_bin_bounds, dim3(grid), dim3(256), 0, s, B.keys_out, V.n, (uint32_t)L.N, B.bins);
   hipLaunchKernelGGL(k_repulsion, dim3(grid), dim3(256), 0, s, V, L, B.vcell, B.bins, B.ids_out, k_rep, r_c);
   hipLaunchKernelGGL(k_bin_clear, dim3(grid), dim3(256), 0, s, B.keys_out, V.n, (uint32_t)L.N, B.bins);
+  return hipGetLastError();
+}
+
+hipError_t launch_boundary_flags(hipStream_t s, const uint8_t* type, const LatticeDev& L, uint8_t* flags) {
+  if (L.N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_boundary_flags, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0, s, type, L, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_boundary_repulsion(hipStream_t s, const VertexDev& V, const LatticeDev& L, const uint8_t* flags,
+                                     double k, double r_c) {
+  if (V.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_boundary_repulsion, dim3((unsigned)((V.n + 255) / 256)), dim3(256), 0, s, V, L, flags, k, r_c);
   return hipGetLastError();
 }
 

@@ -574,6 +574,10 @@ void HemoCell::bindLattice() {
     }
   check(hc_set_force_limit(ctx, param::f_limit), "force limit");
   if (repulsionEnabled) check(hc_set_repulsion(ctx, repulsionConstant, repulsionCutoff, (int)repulsionTimescale), "repulsion");
+  if (boundaryRepulsionEnabled)
+    check(hc_set_boundary_repulsion(ctx, boundaryRepulsionConstant, boundaryRepulsionCutoff,
+                                    (int)boundaryRepulsionTimescale),
+          "boundary repulsion");
   bound = true;
 }
 
@@ -593,6 +597,19 @@ void HemoCell::setRepulsion(T k, T cutoff_um) {
   repulsionCutoff = cutoff_um * (1e-6 / param::dx);
   repulsionEnabled = true;
   if (bound) check(hc_set_repulsion(ctx, repulsionConstant, repulsionCutoff, (int)repulsionTimescale), "setRepulsion");
+}
+
+void HemoCell::enableBoundaryParticles(T k, T cutoff_um, unsigned int timestep) {
+  hlog << "(HemoCell) (Repulsion) Setting boundary repulsion constant to " << k << ". boundary repulsionCutoff to"
+       << cutoff_um << " µm" << std::endl;
+  boundaryRepulsionConstant = k;
+  boundaryRepulsionCutoff = cutoff_um * (1e-6 / param::dx);
+  boundaryRepulsionTimescale = std::max(1u, timestep);
+  boundaryRepulsionEnabled = true;
+  if (bound)
+    check(hc_set_boundary_repulsion(ctx, boundaryRepulsionConstant, boundaryRepulsionCutoff,
+                                    (int)boundaryRepulsionTimescale),
+          "enableBoundaryParticles");
 }
 
 void HemoCell::setRepulsionTimeScaleSeperation(unsigned int separation) {

@@ -316,6 +316,9 @@ class HemoCell {
   void setOutputs(std::string name, std::vector<int> outputs) { (*cellfields)[name]->setOutputVariables(outputs); }
   void setFluidOutputs(std::vector<int> outputs) { fluidOutputs = outputs; }
   void setRepulsion(T repulsionConstant, T repulsionCutoff);
+  // core/hemoCell.cpp:428-436: wall repulsion from the boundary particles (cut-off in
+  // micrometres), applied every `timestep` steps
+  void enableBoundaryParticles(T boundaryRepulsionConstant, T boundaryRepulsionCutoff, unsigned int timestep = 1);
   void setRepulsionTimeScaleSeperation(unsigned int separation);
   void setMaterialTimeScaleSeparation(std::string name, unsigned int separation);
   void setParticleVelocityUpdateTimeScaleSeparation(unsigned int separation);
@@ -349,6 +352,7 @@ class HemoCell {
 
   bool outputInSiUnits = true;
   bool repulsionEnabled = false;
+  bool boundaryRepulsionEnabled = false;
   Lattice* lattice = nullptr;
   Config* cfg = nullptr;
   HemoCellFields* cellfields = nullptr;
@@ -365,6 +369,8 @@ class HemoCell {
   bool loadParticlesIsCalled = false;
   unsigned int repulsionTimescale = 1;
   T repulsionConstant = 0, repulsionCutoff = 0;
+  T boundaryRepulsionConstant = 0, boundaryRepulsionCutoff = 0;
+  unsigned int boundaryRepulsionTimescale = 1;
   std::vector<int64_t> extra_ids, pending_ids;
   std::vector<T> extra_f, pending_f;
   bool extra_set = false;

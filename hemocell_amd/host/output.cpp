@@ -122,6 +122,7 @@ void HemoCell::writeOutput() {
             << std::endl;
   check(hc_stage_mechanics(ctx, 1), "writeOutput");
   if (repulsionEnabled) check(hc_stage_repulsion(ctx), "writeOutput");
+  if (boundaryRepulsionEnabled) check(hc_stage_boundary_repulsion(ctx), "writeOutput");
   mkpath(out + "hdf5/" + zero_pad(it) + "/");
   mkpath(out + "csv/");
   const double dx = param::dx, dt = param::dt, df = param::df;
@@ -131,6 +132,17 @@ void HemoCell::writeOutput() {
   const int64_t nv = hc_num_vertices(ctx);
   std::vector<double> x((size_t)nv * 3), v((size_t)nv * 3), f((size_t)nv * 3), fr;
   std::vector<std::vector<double>> terms;  // area, volume, bending, link, visc, inner link (on first use)
+  // force_repulsion: the cell-cell and the wall terms (hemoCellParticleField.cpp:690, :911)
+  auto repulsion_force = [&](int64_t n) {
+    std::vector<double> r((size_t)n * 3, 0.0);
+    if (n && repulsionEnabled) check(hc_get_repulsion_force(ctx, r.data()), "writeOutput");
+    if (n && boundaryRepulsionEnabled) {
+      std::vector<double> b((size_t)n * 3);
+      check(hc_get_boundary_repulsion_force(ctx, b.data()), "writeOutput");
+      for (size_t k = 0; k < r.size(); ++k) r[k] = r[k] + b[k];
+    }
+    return r;
+  };
   if (nv) check(hc_get_vertices(ctx, x.data(), v.data(), f.data()), "writeOutput");
   if (cellfields) {
     // vertex base of every device cell
@@ -167,18 +179,12 @@ void HemoCell::writeOutput() {
           vec3(v, si ? dx / dt : 1.0, "Velocity");
         } else if (o == OUTPUT_FORCE) {
           // force_total = force + force_repulsion (hemoCellParticleField.cpp:596)
-          if (fr.empty()) {
-            fr.assign((size_t)nv * 3, 0.0);
-            if (repulsionEnabled && nv) check(hc_get_repulsion_force(ctx, fr.data()), "writeOutput");
-          }
+          if (fr.empty()) fr = repulsion_force(nv);
           std::vector<double> tot(f);
           for (size_t k = 0; k < tot.size(); ++k) tot[k] += fr[k];
           vec3(tot, si ? df : 1.0, "Total force");
         } else if (o == OUTPUT_FORCE_REPULSION) {
-          if (fr.empty()) {
-            fr.assign((size_t)nv * 3, 0.0);
-            if (repulsionEnabled && nv) check(hc_get_repulsion_force(ctx, fr.data()), "writeOutput");
-          }
+          if (fr.empty()) fr = repulsion_force(nv);
           vec3(fr, si ? df : 1.0, "Repulsion force");
         } else if (o == OUTPUT_FORCE_AREA || o == OUTPUT_FORCE_VOLUME || o == OUTPUT_FORCE_BENDING ||
                    o == OUTPUT_FORCE_LINK || o == OUTPUT_FORCE_VISC || o == OUTPUT_FORCE_INNER_LINK) {

@@ -138,6 +138,13 @@ hc_status hc_set_velocity_timescale(hc_ctx* ctx, int timescale);
 hc_status hc_set_material_timescale(hc_ctx* ctx, int type_id, int timescale);
 hc_status hc_set_force_limit(hc_ctx* ctx, double f_limit);
 hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale);
+/* Replaces: HemoCell::enableBoundaryParticles (core/hemoCell.cpp:428-436) with
+ * populateBoundaryParticles / applyBoundaryRepulsionForce
+ * (core/hemoCellParticleField.cpp:865-918): bounce-back nodes next to a non-bounce-back
+ * node repel vertices of the 3x3x3 node neighbourhood of their bin closer than the
+ * cut-off (lattice units) with k (r_c / r) r_hat, every `timescale` steps, before the
+ * spread; the force is recomputed at each application and kept between. */
+hc_status hc_set_boundary_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale);
 /* Replaces: HemoCellStretch::applyForce (helper/hemoCellStretch.cpp:63-110): an extra
  * force added to the listed vertices (global vertex ids) before every step's spread. */
 hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* vertex_ids,
@@ -164,6 +171,7 @@ hc_status hc_stage_interpolate(hc_ctx* ctx);     /* interpolateFluidVelocity  he
 hc_status hc_stage_advance(hc_ctx* ctx);         /* advanceParticles  hemoCellParticleField.cpp:566-588 */
 hc_status hc_stage_mechanics(hc_ctx* ctx, int forced); /* applyConstitutiveModel  :633-675 */
 hc_status hc_stage_repulsion(hc_ctx* ctx);       /* applyRepulsionForce  :677-743 */
+hc_status hc_stage_boundary_repulsion(hc_ctx* ctx); /* applyBoundaryRepulsionForce  :889-918 */
 hc_status hc_stage_end(hc_ctx* ctx);             /* setExternalVector reset + iter++  hemoCell.cpp:369-374 */
 
 /* ---- state access (synchronous) ------------------------------------------ */
@@ -181,6 +189,8 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u /* 3*N, n*3+c */);
 hc_status hc_get_vertices(hc_ctx* ctx, double* x, double* v, double* f);
 hc_status hc_set_vertices(hc_ctx* ctx, const double* x, const double* v, const double* f);
 hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep);
+/* the wall repulsion force of every vertex, n*3+c */
+hc_status hc_get_boundary_repulsion_force(hc_ctx* ctx, double* fbr);
 /* Replaces: the force_repulsion member restored with the particle field
  * (HemoCellParticle::serializeValues_t, core/hemoCellParticle.h:45-63); it is kept between
  * repulsion updates when the repulsion timescale is above one. */

@@ -19,6 +19,7 @@
 //   * rest-geometry tables                               mechanics/commonCellConstants.cpp:70-409
 //   * rest volume                                        helper/meshMetrics.h:127-184
 //   * cell-cell repulsion                                core/hemoCellParticleField.cpp:677-743
+//   * wall (boundary particle) repulsion                 core/hemoCellParticleField.cpp:865-918
 //   * Palabos D3Q19 Guo-forced BGK, bounce-back, local regularized velocity
 //     boundary (Palabos v2.3.0 is not vendored; restated from its published
 //     scheme; call sites core/hemoCell.cpp:317,455-469, helper/hemocellInit.hh:58-92).
@@ -423,6 +424,12 @@ struct Oracle {
   bool repulsion = false;
   double rep_const = 0, rep_cutoff = 0;
   long wall_hits = 0;
+  // wall repulsion: boundary particles and the force they exert (kept per vertex)
+  bool brep = false;
+  double brep_const = 0, brep_cutoff = 0;
+  int brep_timescale = 1;
+  std::vector<long> bparticles;  // node indices, ascending x, y, z
+  std::vector<double> fbr;       // 3*NV
 
   long idx(int X, int Y, int Z) const { return (long)Z + (long)nz * ((long)Y + (long)ny * X); }
   int dim(int a) const { return a == 0 ? nx : (a == 1 ? ny : nz); }
@@ -535,8 +542,11 @@ struct Oracle {
         double s = f_limit / fm;
         for (int c = 0; c < 3; ++c) fv[c] *= s;
       }
+      double r[3];
+      for (int c = 0; c < 3; ++c) r[c] = brep && (long)fbr.size() == 3 * NV() ? frep[3 * i + c] + fbr[3 * i + c]
+                                                                              : frep[3 * i + c];
       for (int j = 0; j < k.n; ++j)
-        for (int c = 0; c < 3; ++c) Fibm[(long)c * N + k.idx[j]] += (frep[3 * i + c] + fv[c]) * k.w[j];
+        for (int c = 0; c < 3; ++c) Fibm[(long)c * N + k.idx[j]] += (r[c] + fv[c]) * k.w[j];
     }
   }
 
@@ -798,10 +808,78 @@ struct Oracle {
         }
   }
 
+  // populateBoundaryParticles (hemoCellParticleField.cpp:865-887): bounce-back nodes
+  // with a node that is not bounce-back among their 26 neighbours, in x, y, z order
+  // (periodic axes wrap; the reference's scan stops one node short of the far
+  // faces of its block, which here would drop a wall plane, so every node counts)
+  void populate_boundary() {
+    bparticles.clear();
+    for (int X = 0; X < nx; ++X)
+      for (int Y = 0; Y < ny; ++Y)
+        for (int Z = 0; Z < nz; ++Z) {
+          if (type[idx(X, Y, Z)] != BOUNCEBACK) continue;
+          bool found = false;
+          for (int a = -1; a <= 1 && !found; ++a)
+            for (int b = -1; b <= 1 && !found; ++b)
+              for (int c = -1; c <= 1 && !found; ++c) {
+                int xx = wrap(0, X + a), yy = wrap(1, Y + b), zz = wrap(2, Z + c);
+                if (xx < 0 || yy < 0 || zz < 0) continue;
+                if (type[idx(xx, yy, zz)] != BOUNCEBACK) found = true;
+              }
+          if (found) bparticles.push_back(idx(X, Y, Z));
+        }
+  }
+
+  // applyBoundaryRepulsionForce (hemoCellParticleField.cpp:889-918): every boundary
+  // particle, in order, adds k (r_c / r) r_hat to each vertex binned in its 3x3x3
+  // neighbourhood closer than r_c; r from the node to the vertex, minimum image on
+  // periodic axes.  The force is recomputed here at each application.
+  void boundary_repulsion_force() {
+    long nv = NV();
+    fbr.assign(3 * nv, 0.0);
+    std::vector<std::vector<long>> bin(N);
+    for (long i = 0; i < nv; ++i) {
+      int w3[3];
+      bool out = false;
+      for (int a = 0; a < 3; ++a) {
+        w3[a] = wrap(a, (long)std::floor(x[3 * i + a] + 0.5));
+        if (w3[a] < 0) out = true;
+      }
+      if (!out) bin[idx(w3[0], w3[1], w3[2])].push_back(i);
+    }
+    for (long b : bparticles) {
+      const int BZ = (int)(b % nz), BY = (int)((b / nz) % ny), BX = (int)(b / ((long)ny * nz));
+      const int P[3] = {BX, BY, BZ};
+      for (int a = -1; a <= 1; ++a)
+        for (int bb = -1; bb <= 1; ++bb)
+          for (int c = -1; c <= 1; ++c) {
+            int xx = wrap(0, BX + a), yy = wrap(1, BY + bb), zz = wrap(2, BZ + c);
+            if (xx < 0 || yy < 0 || zz < 0) continue;
+            for (long i : bin[idx(xx, yy, zz)]) {
+              double dv[3];
+              for (int q = 0; q < 3; ++q) {
+                double d = x[3 * i + q] - P[q];
+                if (periodic[q]) {
+                  int n = dim(q);
+                  d -= n * std::floor(d / n + 0.5);
+                }
+                dv[q] = d;
+              }
+              double dist = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+              if (dist < brep_cutoff) {
+                double s = brep_const * (1 / (dist / brep_cutoff));
+                for (int q = 0; q < 3; ++q) fbr[3 * i + q] = fbr[3 * i + q] + s * (dv[q] / dist);
+              }
+            }
+          }
+    }
+  }
+
   void reset_force_field() { std::fill(Fibm.begin(), Fibm.end(), 0.0); }
 
   void step() {
     if (repulsion && iter % rep_timescale == 0) repulsion_force();
+    if (brep && iter % brep_timescale == 0) boundary_repulsion_force();
     spread();
     collide_and_stream();
     if (iter % pv_timescale == 0) interpolate();
@@ -885,6 +963,7 @@ int ho_add_cells(void* h, int t, int ncells, const double* xyz) {
       }
       o->vcell.push_back(cell);
     }
+    if (o->brep) o->fbr.resize(3 * o->NV(), 0.0);
   }
   return 0;
 }
@@ -979,6 +1058,20 @@ void ho_vertex_bins(void* h, long* out) {
     out[i] = out_of ? -1 : o->idx(w3[0], w3[1], w3[2]);
   }
 }
+void ho_set_boundary_repulsion(void* h, double k, double cutoff, int timescale) {
+  Oracle* o = (Oracle*)h;
+  o->brep = true;
+  o->brep_const = k;
+  o->brep_cutoff = cutoff;
+  o->brep_timescale = timescale;
+  o->populate_boundary();
+  o->fbr.assign(3 * o->NV(), 0.0);
+}
+void ho_get_boundary_repulsion(void* h, double* fr) {
+  Oracle* o = (Oracle*)h;
+  for (long i = 0; i < 3 * o->NV(); ++i) fr[i] = i < (long)o->fbr.size() ? o->fbr[i] : 0.0;
+}
+void ho_stage_boundary_repulsion(void* h) { ((Oracle*)h)->boundary_repulsion_force(); }
 void ho_get_repulsion(void* h, double* fr) { Oracle* o = (Oracle*)h; std::copy(o->frep.begin(), o->frep.end(), fr); }
 
 // Individual stages, in iterate() order, for stage-level parity tests.

@@ -44,6 +44,9 @@ _SIG = {
     "ho_set_vertices": (None, [_P, _D, _D, _D]),
     "ho_get_repulsion": (None, [_P, _D]),
     "ho_stage_repulsion": (None, [_P]),
+    "ho_set_boundary_repulsion": (None, [_P, ctypes.c_double, ctypes.c_double, _INT]),
+    "ho_get_boundary_repulsion": (None, [_P, _D]),
+    "ho_stage_boundary_repulsion": (None, [_P]),
     "ho_vertex_bins": (None, [_P, ctypes.POINTER(ctypes.c_long)]),
     "ho_stage_spread": (None, [_P]),
     "ho_stage_collide_stream": (None, [_P]),
@@ -121,6 +124,14 @@ class OracleSim:
 
     def set_repulsion(self, k_rep, cutoff_lu, timescale=1):
         self.lib.ho_set_repulsion(self.h, k_rep, cutoff_lu, timescale)
+
+    def set_boundary_repulsion(self, k_rep, cutoff_lu, timescale=1):
+        self.lib.ho_set_boundary_repulsion(self.h, k_rep, cutoff_lu, timescale)
+
+    def boundary_repulsion_force(self):
+        fr = np.zeros((self.num_vertices, 3))
+        self.lib.ho_get_boundary_repulsion(self.h, _dp(fr))
+        return fr
 
     def repulsion_force(self):
         fr = np.zeros((self.num_vertices, 3))

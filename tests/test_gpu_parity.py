@@ -409,6 +409,43 @@ def test_full_step_with_repulsion_fp64():
     assert cases.rel_l2(g.repulsion_force(), o.repulsion_force()) < 1e-9
 
 
+def _near_walls_pair(shape, seed):
+    """Platelets and one RBC in a channel (bounce-back planes y = 0 and y = ny-1),
+    several platelets within a lattice spacing of the walls."""
+    rng = np.random.default_rng(seed)
+    u = cases.units()
+    mask = cases.channel_mask(*shape, axis=1)
+    g, o = pair(*shape, tau=u.tau, periodic=(1, 0, 1), mask=mask)
+    for kind, centers in (("PLT", [(5.0, 3.1, 6.0), (13.0, shape[1] - 4.2, 9.0), (8.0, 3.3, 15.0)]),
+                          ("RBC", [(10.0, shape[1] / 2.0, 10.0)])):
+        model, v, t, p, inner = cases.mesh(kind, u, hc.build_mesh)
+        xs = np.stack([cases.place(v, c, cases.rotation(rng)) for c in centers])
+        for s in (g, o):
+            tid = s.add_cell_type(model, v, t, p, inner)
+            s.add_cells(tid, xs)
+    return g, o, u
+
+
+def test_wall_repulsion_stage_and_steps_fp64():
+    """Wall (boundary particle) repulsion, hemoCellParticleField.cpp:865-918: the
+    per-vertex force against the oracle after one application, then whole steps
+    with it on every other step (positions within 1e-10, the force within 1e-12)."""
+    g, o, u = _near_walls_pair((18, 24, 20), seed=17)
+    both(g, o, "set_force_limit", u.f_limit)
+    both(g, o, "set_boundary_repulsion", 2e-4, 1.3, 1)
+    both(g, o, "stage", "boundary_repulsion")
+    fo = o.boundary_repulsion_force()
+    assert np.abs(fo).max() > 0 and np.count_nonzero(np.abs(fo).sum(axis=1)) >= 2
+    assert cases.rel_l2(g.boundary_repulsion_force(), fo) < 1e-14
+    both(g, o, "set_boundary_repulsion", 2e-4, 1.3, 2)
+    both(g, o, "set_body_force", [0, 0, 1e-6])
+    for _ in range(3):
+        both(g, o, "step", 5)
+    assert cases.rel_l2(g.vertices()[0], o.vertices()[0]) < 1e-10
+    assert cases.rel_l2(g.boundary_repulsion_force(), o.boundary_repulsion_force()) < 1e-12
+    assert g.wall_hits() == o.wall_hits() == 0
+
+
 # ---------------------------------------------------------- cell statistics --
 def test_cell_stats_match_reference_formulas():
     u = cases.units()
