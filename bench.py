@@ -90,7 +90,7 @@ def build_case(hc, susp, shape, precision, seed=20261202, workload="suspension")
     return sim, units, types, cells
 
 
-def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension"):
+def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension", autotune=True):
     label = "fp32" if precision == hc.HC_FP32_SHIFTED else "fp64"
     print("[bench] building %s %s case %s" % (label, workload, shape), file=sys.stderr, flush=True)
     t0 = time.time()
@@ -102,6 +102,11 @@ def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension"):
         file=sys.stderr, flush=True)
     sim.step(warmup)
     sim.synchronize()
+    tuning = sim.autotune(2) if autotune else None
+    if tuning:
+        print("[bench] autotune: %s" % json.dumps({k: v["chosen"] for k, v in tuning.items()}), file=sys.stderr,
+              flush=True)
+    done_before = sim.iteration
     sim.profile_enable(True)
     sim.synchronize()
     t1 = time.perf_counter()
@@ -126,6 +131,8 @@ def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension"):
                vertex_updates_per_s=nv * steps / dt, stage_ms=stages, vertices=nv, nodes=N,
                cells={k: int(len(v)) for k, v in cells.items()})
     res["collide_ms"] = collide_ms
+    res["tuning"] = {k: v["chosen"] for k, v in tuning.items()} if tuning else None
+    res["steps_before"] = done_before
     res["wall_hits"] = int(wall_hits)
     res["x0"], res["x1"] = x0, x1
     res["collide_gbs"] = BYTES_PER_LUP[label] * N / (stages["collide_stream"] * 1e-3) / 1e9
@@ -210,6 +217,7 @@ def main():
     ap.add_argument("--serve", type=int, default=32, metavar="N",
                     help="cases of the serving stream (0 skips the latency measurement)")
     ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate, cases/s")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a device-to-device copy of MIB MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -231,9 +239,11 @@ def main():
         del a, b
     res = {}
     if args.precision in ("both", "fp32"):
-        res["fp32"] = run_gpu(hc, susp, shape, hc.HC_FP32_SHIFTED, args.steps, args.warmup, args.workload)
+        res["fp32"] = run_gpu(hc, susp, shape, hc.HC_FP32_SHIFTED, args.steps, args.warmup, args.workload,
+                              not args.no_autotune)
     if args.precision in ("both", "fp64"):
-        res["fp64"] = run_gpu(hc, susp, shape, hc.HC_FP64, args.steps, args.warmup, args.workload)
+        res["fp64"] = run_gpu(hc, susp, shape, hc.HC_FP64, args.steps, args.warmup, args.workload,
+                              not args.no_autotune)
     head = res.get("fp32") or res["fp64"]
     traffic = profile_traffic(head["label"])
     if traffic is not None and (shape != (256, 256, 512) or args.workload != "suspension"):
@@ -262,6 +272,7 @@ def main():
         "whole_step": whole_step(head),
         "vertex_updates_per_s": round(head["vertex_updates_per_s"], 1),
         "stage_ms": {k: round(v, 4) for k, v in head["stage_ms"].items()},
+        "tuning": head["tuning"],
     }
     if "fp64" in res and "fp32" in res:
         f64 = res["fp64"]
@@ -271,7 +282,7 @@ def main():
         line["fp32_accuracy"] = {
             "vertex_position_rel_l2": float(np.linalg.norm(res["fp32"]["x1"] - f64["x1"]) / np.linalg.norm(f64["x1"])),
             "vertex_displacement_rel_l2": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
-            "steps": args.steps + args.warmup, "reference": "the FP64 form of this build"}
+            "steps": res["fp32"]["steps_before"] + args.steps, "reference": "the FP64 form of this build"}
         line["fp64"] = {"value": round(f64["mlups"], 2), "ms_per_step": round(f64["ms_per_step"], 4),
                         "collide_gbs": round(f64["collide_gbs"], 1),
                         "collide_frac": round(f64["collide_gbs"] / HBM_PEAK_GBS, 4),

@@ -79,6 +79,7 @@ SIGNATURES = {
     "hc_synchronize": (_ST, [_P]),
     "hc_iteration": (ctypes.c_int64, [_P]),
     "hc_set_iteration": (_ST, [_P, ctypes.c_int64]),
+    "hc_autotune": (_ST, [_P, _INT, ctypes.c_char_p, _INT]),
     "hc_stage_spread": (_ST, [_P]),
     "hc_stage_collide_stream": (_ST, [_P]),
     "hc_stage_interpolate": (_ST, [_P]),
@@ -287,6 +288,14 @@ class Simulation:
     @property
     def iteration(self):
         return self.lib.hc_iteration(self.h)
+
+    def autotune(self, steps_per_trial=3):
+        """Time the per-cell kernels' launch shapes on real steps and keep the fastest;
+        returns the timings (ms per step) and the choices."""
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        self._chk(self.lib.hc_autotune(self.h, int(steps_per_trial), buf, len(buf)))
+        return json.loads(buf.value.decode() or "{}")
 
     def set_iteration(self, it):
         self._chk(self.lib.hc_set_iteration(self.h, int(it)))

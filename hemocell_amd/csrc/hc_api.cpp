@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
+#include <sstream>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -399,7 +401,7 @@ hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
     if ((ctx->iter % tb.host.timescale) == 0) flags |= CELL_FLAG_MECH;
     if (ctx->repulsion) flags |= CELL_FLAG_REPULSION;
     // spread box in LDS; larger supports are spread in slabs of x-planes
-    const int box = ctx->spread_lds > 0 ? ctx->spread_lds : 16384;
+    const int box = ctx->spread_lds > 0 ? ctx->spread_lds : 12288;
     // few cells: their latency, not the occupancy, sets the time: the widest block
     int block = tb.host.model == HC_MODEL_WBC ? ctx->wbc_block : ctx->cell_block;
     if (tb.ncells < ctx->n_cu && !(ctx->cell_block_env && tb.host.model == HC_MODEL_RBC)) block = 1024;
@@ -664,6 +666,83 @@ hc_status step_impl(hc_ctx* ctx, int nsteps) {
     ++k;
   }
   return HC_OK;
+}
+
+// Autotuner: times real steps of this context (HIP events on its stream, steps
+// launched eagerly) for each value of one parameter at a time -- the spread
+// box, the threads per RBC cell, the side types' layout, the threads per WBC
+// cell -- and keeps the fastest.  The steps advance the simulation as any other.
+template <typename R>
+hc_status autotune_impl(hc_ctx* ctx, int steps, std::string& report) {
+  const bool graphs = ctx->use_graphs;
+  ctx->use_graphs = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HC_TRY(hipEventCreate(&e0), "hipEventCreate");
+  HC_TRY(hipEventCreate(&e1), "hipEventCreate");
+  hc_status s = HC_OK;
+  auto trial = [&](double& ms) -> hc_status {
+    hc_status t;
+    if ((t = one_step<R>(ctx)) != HC_OK) return t;  // first launch of a new shape: not timed
+    HC_TRY(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+    for (int k = 0; k < steps; ++k)
+      if ((t = one_step<R>(ctx)) != HC_OK) return t;
+    HC_TRY(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+    HC_TRY(hipEventSynchronize(e1), "hipEventSynchronize");
+    float f = 0.0f;
+    HC_TRY(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+    ms = (double)f / steps;
+    return HC_OK;
+  };
+  bool has_rbc = false, has_wbc = false, big_side = false;
+  int ntypes = 0;
+  for (auto& tp : ctx->types) {
+    if (!tp->ncells) continue;
+    ++ntypes;
+    has_rbc |= tp->host.model == HC_MODEL_RBC;
+    has_wbc |= tp->host.model == HC_MODEL_WBC;
+    big_side |= 3 * (size_t)tp->host.nv * sizeof(double) > 32768;
+  }
+  struct Param {
+    const char* name;
+    std::vector<int> values;
+    std::function<void(int)> set;
+    bool on;
+  };
+  std::vector<Param> params = {
+      {"spread_box_bytes", {8192, 12288, 16384, 24576}, [&](int v) { ctx->spread_lds = v; }, true},
+      {"rbc_threads", {256, 512, 1024}, [&](int v) { ctx->cell_block = v; ctx->cell_block_env = true; }, has_rbc},
+      {"side_positions_global", {1, 0}, [&](int v) { ctx->side_gpos = v != 0; }, ntypes > 1 && big_side},
+      {"wbc_threads", {256, 512, 1024}, [&](int v) { ctx->wbc_block = v; }, has_wbc},
+  };
+  std::ostringstream out;
+  out << "{";
+  bool first_param = true;
+  for (auto& p : params) {
+    if (!p.on || s != HC_OK) continue;
+    int best = p.values[0];
+    double best_ms = 1e300;
+    out << (first_param ? "" : ", ") << "\"" << p.name << "\": {";
+    first_param = false;
+    for (size_t k = 0; k < p.values.size(); ++k) {
+      p.set(p.values[k]);
+      double ms = 0;
+      if ((s = trial(ms)) != HC_OK) break;
+      out << (k ? ", " : "") << "\"" << p.values[k] << "\": " << ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = p.values[k];
+      }
+    }
+    p.set(best);
+    out << ", \"chosen\": " << best << "}";
+  }
+  out << "}";
+  report = out.str();
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  ctx->use_graphs = graphs;
+  drop_graph(ctx);
+  return s;
 }
 
 bool fp32(const hc_ctx* c) { return c->precision == HC_FP32_SHIFTED; }
@@ -1103,6 +1182,21 @@ hc_status hc_step(hc_ctx* ctx, int nsteps) {
   if (s != HC_OK) return s;
   if (nsteps < 0) return fail(ctx, HC_INVALID_ARG, "nsteps must be >= 0");
   return fp32(ctx) ? step_impl<float>(ctx, nsteps) : step_impl<double>(ctx, nsteps);
+}
+
+hc_status hc_autotune(hc_ctx* ctx, int steps_per_trial, char* report, int report_len) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  if (steps_per_trial < 1) return fail(ctx, HC_INVALID_ARG, "steps_per_trial must be >= 1");
+  std::string rep = "{}";
+  if (ctx->nv > 0 && ctx->fuse && !ctx->repulsion && !ctx->brep)
+    s = fp32(ctx) ? autotune_impl<float>(ctx, steps_per_trial, rep) : autotune_impl<double>(ctx, steps_per_trial, rep);
+  if (report && report_len > 0) {
+    const size_t n = std::min<size_t>(rep.size(), (size_t)report_len - 1);
+    std::memcpy(report, rep.data(), n);
+    report[n] = '\0';
+  }
+  return s;
 }
 
 int hc_busy(hc_ctx* ctx) {

@@ -160,6 +160,12 @@ hc_status hc_synchronize(hc_ctx* ctx);
  * that drive several contexts from one thread. */
 int hc_busy(hc_ctx* ctx);
 int64_t hc_iteration(const hc_ctx* ctx);
+/* Picks the launch shapes of the per-cell kernels for this context's mesh mix (spread box
+ * bytes, threads per RBC cell, side types' position layout, threads per WBC cell) by
+ * timing steps_per_trial real steps per candidate, one parameter at a time; the steps
+ * advance the simulation.  report (may be NULL) receives the timings and choices as JSON.
+ * No counterpart in the reference (its kernels have no launch shape to choose). */
+hc_status hc_autotune(hc_ctx* ctx, int steps_per_trial, char* report, int report_len);
 /* Replaces: the iteration read back by HemoCellFields::load (core/hemoCellFields.cpp:245);
  * the material, velocity and repulsion timescales count from it. */
 hc_status hc_set_iteration(hc_ctx* ctx, int64_t iter);

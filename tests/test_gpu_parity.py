@@ -567,6 +567,32 @@ def test_wbc_beside_rbc_with_positions_in_global_memory(monkeypatch, precision):
         assert cases.rel_l2(a, b) <= tol
 
 
+def test_autotune_steps_like_plain_steps():
+    """hc_autotune times real steps under each launch shape and keeps the fastest:
+    the run it leaves behind is the same simulation as plain steps (the shapes
+    change only the commit order of float atomics, hence the round-off bounds of
+    the graph test), and it reports a choice for every parameter it tried."""
+    u = cases.units()
+    sims = []
+    for _ in range(2):
+        g = hc.Simulation(0)
+        g.lattice_init(40, 30, 36, u.tau)
+        _mixed([g])
+        sims.append(g)
+    tuned, plain = sims
+    rep = tuned.autotune(1)
+    assert set(rep) >= {"spread_box_bytes", "rbc_threads", "wbc_threads"}
+    assert all("chosen" in v for v in rep.values())
+    n = tuned.iteration
+    plain.step(n)
+    tuned.step(5)
+    plain.step(5)
+    assert tuned.iteration == plain.iteration == n + 5
+    (xa, va, _), (xb, vb, _) = tuned.vertices(), plain.vertices()
+    assert cases.rel_l2(xa, xb) < 1e-12
+    assert cases.rel_l2(va, vb) < 1e-8
+
+
 @pytest.mark.parametrize("block", ["256", "512"])
 def test_threads_per_cell_agree_with_the_oracle(monkeypatch, block):
     """The fused cell kernel at an explicit number of threads per RBC (the
