@@ -102,6 +102,11 @@ SIGNATURES = {
     "hc_set_boundary_repulsion": (_ST, [_P, ctypes.c_double, ctypes.c_double, _INT]),
     "hc_stage_boundary_repulsion": (_ST, [_P]),
     "hc_set_repulsion_force": (_ST, [_P, _D]),
+    "hc_set_interior_viscosity": (_ST, [_P, _INT, ctypes.c_double]),
+    "hc_set_interior_viscosity_timescales": (_ST, [_P, _INT, _INT]),
+    "hc_get_interior_viscosity": (_ST, [_P, _D]),
+    "hc_set_interior_viscosity_field": (_ST, [_P, _D]),
+    "hc_stage_interior_viscosity": (_ST, [_P, _INT]),
     "hc_get_cells": (_ST, [_P, _I32, _I32]),
     "hc_get_vertex_bins": (_ST, [_P, _I64]),
     "hc_get_cell_volume_area": (_ST, [_P, _D, _D]),
@@ -263,6 +268,25 @@ class Simulation:
         fr = np.zeros((self.num_vertices, 3))
         self._chk(self.lib.hc_get_boundary_repulsion_force(self.h, _dp(fr)))
         return fr
+
+    def set_interior_viscosity(self, type_id, tau_inner):
+        """Fluid nodes inside the membranes of type_id relax with tau_inner (<= 0: off)."""
+        self._chk(self.lib.hc_set_interior_viscosity(self.h, type_id, float(tau_inner)))
+
+    def set_interior_viscosity_timescales(self, membrane, entire_grid):
+        self._chk(self.lib.hc_set_interior_viscosity_timescales(self.h, int(membrane), int(entire_grid)))
+
+    def interior_viscosity(self):
+        """Relaxation time per node, shape (nx, ny, nz): 0 where the background applies."""
+        t = np.zeros(self.shape)
+        self._chk(self.lib.hc_get_interior_viscosity(self.h, _dp(t)))
+        return t
+
+    def set_interior_viscosity_field(self, tau):
+        t = np.ascontiguousarray(tau, dtype=np.float64)
+        if t.size != int(np.prod(self.shape)):
+            raise ValueError("one value per node expected")
+        self._chk(self.lib.hc_set_interior_viscosity_field(self.h, _dp(t)))
 
     def set_extra_vertex_force(self, ids, forces):
         ids = np.ascontiguousarray(ids, dtype=np.int64)

@@ -115,6 +115,15 @@ struct hc_ctx {
   double brep_k = 0, brep_cutoff = 0;
   int brep_timescale = 1;
   uint8_t* d_bflag = nullptr;
+  // interior viscosity: tau inside the membranes of each type (0 = off) and the
+  // two refresh timescales; node classes, last-writer stamps of the membrane
+  // pass (zero between passes) and the positions before the advance
+  std::vector<double> iv_tau;
+  int iv_timescale = 1, iv_grid_timescale = 1;
+  uint8_t* d_visc = nullptr;
+  uint32_t* d_stamp = nullptr;
+  double* d_xold = nullptr;
+  int64_t xold_cap = 0;
   // binning and repulsion buffers, built on first use and whenever the vertex count changed
   RepulsionBufs rb{};
   int64_t rb_nv = -1;
@@ -372,6 +381,84 @@ hc_status do_boundary_repulsion(hc_ctx* ctx) {
 // the repulsion terms the spread adds: 1 cell-cell, 2 wall
 int rep_mask(const hc_ctx* c) { return (c->repulsion ? 1 : 0) | (c->brep ? 2 : 0); }
 
+double iv_tau_of(const hc_ctx* c, size_t t) { return t < c->iv_tau.size() ? c->iv_tau[t] : 0.0; }
+bool iv_active(const hc_ctx* c) {
+  for (double t : c->iv_tau)
+    if (t > 0.0) return true;
+  return false;
+}
+
+// node classes and stamps (zeroed once), positions-before-advance buffer
+hc_status ensure_visc(hc_ctx* ctx) {
+  hc_status s;
+  const int64_t N = std::max<int64_t>(ctx->L.N, 1);
+  if (!ctx->d_visc) {
+    if ((s = dalloc(ctx, (void**)&ctx->d_visc, N, "interior viscosity classes")) != HC_OK) return s;
+    HC_TRY(launch_visc_clear(ctx->stream, ctx->d_visc, ctx->L.N), "interior viscosity classes");
+  }
+  if (!ctx->d_stamp) {
+    if ((s = dalloc(ctx, (void**)&ctx->d_stamp, sizeof(uint32_t) * N, "interior viscosity stamps")) != HC_OK) return s;
+    HC_TRY(hipMemsetAsync(ctx->d_stamp, 0, sizeof(uint32_t) * N, ctx->stream), "hipMemsetAsync");
+  }
+  if (ctx->xold_cap != ctx->vcap) {
+    HC_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ctx->d_xold) hipFree(ctx->d_xold);
+    ctx->d_xold = nullptr;
+    ctx->xold_cap = 0;
+    if ((s = dalloc(ctx, (void**)&ctx->d_xold, sizeof(double) * 3 * std::max<int64_t>(ctx->vcap, 1),
+                    "positions before the advance")) != HC_OK)
+      return s;
+    ctx->xold_cap = ctx->vcap;
+  }
+  return HC_OK;
+}
+
+// the collide's per-class relaxation rates (classes of disabled types relax
+// with the background rate)
+void visc_args(const hc_ctx* ctx, CollideArgs& a) {
+  if (!ctx->d_visc || !iv_active(ctx)) return;
+  a.visc = ctx->d_visc;
+  for (int k = 0; k < 16; ++k) a.omega_cls[k] = a.omega;
+  for (size_t t = 0; t < ctx->iv_tau.size() && t < 15; ++t)
+    if (ctx->iv_tau[t] > 0.0) a.omega_cls[t + 1] = 1.0 / ctx->iv_tau[t];
+}
+
+// entire-grid pass: every class back to 0, then the inner nodes of each cell
+hc_status do_visc_grid(hc_ctx* ctx) {
+  hc_status s = ensure_visc(ctx);
+  if (s != HC_OK) return s;
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_MECH, &ev);
+  HC_TRY(launch_visc_clear(ctx->stream, ctx->d_visc, ctx->L.N), "interior viscosity classes");
+  for (size_t t = 0; t < ctx->types.size(); ++t) {
+    TypeBufs& tb = *ctx->types[t];
+    if (!(iv_tau_of(ctx, t) > 0.0) || tb.ncells == 0) continue;
+    HC_TRY(launch_visc_grid(ctx->stream, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base, ctx->L,
+                            ctx->type, (int)t + 1, ctx->d_visc),
+           "interior viscosity (entire grid)");
+  }
+  prof_end(ctx, HC_PROF_MECH, ev);
+  return HC_OK;
+}
+
+// membrane pass with the support of the positions xold (3 planes, stride vcap)
+hc_status do_visc_membrane(hc_ctx* ctx, const double* xold) {
+  hc_status s = ensure_visc(ctx);
+  if (s != HC_OK) return s;
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_MECH, &ev);
+  for (int apply = 0; apply < 2; ++apply)
+    for (size_t t = 0; t < ctx->types.size(); ++t) {
+      TypeBufs& tb = *ctx->types[t];
+      if (!(iv_tau_of(ctx, t) > 0.0) || tb.ncells == 0) continue;
+      HC_TRY(launch_visc_membrane(ctx->stream, vdev(ctx), tb.dev, tb.d_cells, tb.ncells, ctx->d_cell_base, xold,
+                                  ctx->L, ctx->type, (int)t + 1, ctx->d_stamp, ctx->d_visc, apply != 0),
+             "interior viscosity (membrane)");
+    }
+  prof_end(ctx, HC_PROF_MECH, ev);
+  return HC_OK;
+}
+
 template <typename R>
 hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
   hipEvent_t ev;
@@ -442,10 +529,11 @@ hc_status do_spread(hc_ctx* ctx) {
 
 template <typename R>
 hc_status do_collide(hc_ctx* ctx) {
-  CollideArgs a;
+  CollideArgs a{};
   a.omega = 1.0 / ctx->tau;
   for (int c = 0; c < 3; ++c) a.g[c] = ctx->g[c];
   std::memcpy(a.wall_u, ctx->wall_u, sizeof(a.wall_u));
+  visc_args(ctx, a);
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_COLLIDE, &ev);
   HC_TRY(launch_collide_stream<R>(ctx->stream, (R*)ctx->f, (const R*)ctx->Fibm[ctx->cur], (R*)ctx->Fibm[1 - ctx->cur],
@@ -506,7 +594,8 @@ hc_status one_step(hc_ctx* ctx) {
   hc_status s;
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_STEP, &ev);
-  const bool fused = ctx->fuse && !ctx->repulsion && !ctx->brep && ctx->nv > 0;
+  const bool iv = iv_active(ctx) && ctx->nv > 0;
+  const bool fused = ctx->fuse && !ctx->repulsion && !ctx->brep && !iv && ctx->nv > 0;
   if (ctx->repulsion && ctx->nv > 0 && ctx->iter % ctx->rep_timescale == 0 && (s = do_repulsion(ctx)) != HC_OK)
     return s;
   if (ctx->brep && ctx->nv > 0 && ctx->iter % ctx->brep_timescale == 0 && (s = do_boundary_repulsion(ctx)) != HC_OK)
@@ -525,8 +614,17 @@ hc_status one_step(hc_ctx* ctx) {
     }
     if ((s = do_cell_step<R>(ctx, do_interp)) != HC_OK) return s;
   } else {
+    // interior viscosity (hemoCell.cpp:347-357): after the constitutive model,
+    // the membrane pass with the support of the positions before the advance
+    const bool memb = iv && ctx->iter % ctx->iv_timescale == 0;
+    if (iv && (s = ensure_visc(ctx)) != HC_OK) return s;
+    if (memb)
+      HC_TRY(hipMemcpyAsync(ctx->d_xold, ctx->x, sizeof(double) * 3 * ctx->vcap, hipMemcpyDeviceToDevice, ctx->stream),
+             "hipMemcpyAsync");
     if ((s = do_interp_advance<R>(ctx, do_interp, 1)) != HC_OK) return s;
     if ((s = do_mechanics(ctx, false)) != HC_OK) return s;
+    if (iv && ctx->iter % ctx->iv_grid_timescale == 0 && (s = do_visc_grid(ctx)) != HC_OK) return s;
+    if (memb && (s = do_visc_membrane(ctx, ctx->d_xold)) != HC_OK) return s;
   }
   if ((s = do_end<R>(ctx)) != HC_OK) return s;
   ctx->spread_ready = fused;
@@ -591,7 +689,8 @@ std::vector<unsigned char> launch_signature(const hc_ctx* c) {
 
 // Two steps repeat exactly when the step is fused and every timescale is 1.
 bool graph_eligible(const hc_ctx* c) {
-  if (!c->use_graphs || c->prof.on || !c->fuse || c->repulsion || c->brep || c->nv == 0 || !c->spread_ready)
+  if (!c->use_graphs || c->prof.on || !c->fuse || c->repulsion || c->brep || iv_active(c) || c->nv == 0 ||
+      !c->spread_ready)
     return false;
   if (c->pv_timescale != 1) return false;
   for (const auto& tp : c->types)
@@ -762,7 +861,7 @@ static hc_status fluid_stats(hc_ctx* ctx, int what, double* out) {
   R* Fs = nullptr;
   hc_status s = dalloc(ctx, (void**)&part, sizeof(double) * 4 * nb, "statistics partials");
   if (s != HC_OK) return s;
-  CollideArgs a;
+  CollideArgs a{};
   a.omega = 1.0 / ctx->tau;
   for (int c = 0; c < 3; ++c) a.g[c] = ctx->g[c];
   std::memcpy(a.wall_u, ctx->wall_u, sizeof(a.wall_u));
@@ -871,7 +970,8 @@ void hc_destroy(hc_ctx* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->d_extra_dense, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
-                ctx->fbr, ctx->d_bflag, ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag};
+                ctx->fbr, ctx->d_bflag, ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag,
+                ctx->d_visc, ctx->d_stamp, ctx->d_xold};
   for (void* p : ps)
     if (p) hipFree(p);
   for (auto& tp : ctx->types) {
@@ -1150,6 +1250,73 @@ hc_status hc_set_boundary_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu,
   return HC_OK;
 }
 
+hc_status hc_set_interior_viscosity(hc_ctx* ctx, int type_id, double tau_inner) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (type_id < 0 || type_id >= (int)ctx->types.size()) return fail(ctx, HC_INVALID_ARG, "bad type id");
+  if (type_id >= 15) return fail(ctx, HC_INVALID_ARG, "interior viscosity is available for type ids 0..14");
+  if (!std::isfinite(tau_inner) || (tau_inner > 0.0 && !(tau_inner > 0.5)))
+    return fail(ctx, HC_INVALID_ARG, "tau_inner must be > 0.5 (or <= 0 to turn interior viscosity off)");
+  if (ctx->lattice_ready) {
+    hc_status s = invalidate_spread(ctx);
+    if (s != HC_OK) return s;
+  }
+  if (ctx->iv_tau.size() < ctx->types.size()) ctx->iv_tau.resize(ctx->types.size(), 0.0);
+  ctx->iv_tau[type_id] = tau_inner > 0.0 ? tau_inner : 0.0;
+  return HC_OK;
+}
+
+hc_status hc_set_interior_viscosity_timescales(hc_ctx* ctx, int membrane, int entire_grid) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (membrane < 1 || entire_grid < 1) return fail(ctx, HC_INVALID_ARG, "timescales must be >= 1");
+  ctx->iv_timescale = membrane;
+  ctx->iv_grid_timescale = entire_grid;
+  return HC_OK;
+}
+
+hc_status hc_get_interior_viscosity(hc_ctx* ctx, double* tau) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!tau) return fail(ctx, HC_INVALID_ARG, "null output");
+  std::vector<uint8_t> cls((size_t)ctx->L.N, 0);
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  if (ctx->d_visc) HC_TRY(hipMemcpy(cls.data(), ctx->d_visc, (size_t)ctx->L.N, hipMemcpyDeviceToHost), "hipMemcpy");
+  for (int64_t n = 0; n < ctx->L.N; ++n) tau[n] = cls[n] ? iv_tau_of(ctx, cls[n] - 1) : 0.0;
+  return HC_OK;
+}
+
+hc_status hc_set_interior_viscosity_field(hc_ctx* ctx, const double* tau) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!tau) return fail(ctx, HC_INVALID_ARG, "null input");
+  std::vector<uint8_t> cls((size_t)ctx->L.N, 0);
+  for (int64_t n = 0; n < ctx->L.N; ++n) {
+    if (tau[n] == 0.0) continue;
+    size_t t = 0;
+    while (t < ctx->iv_tau.size() && !(ctx->iv_tau[t] > 0.0 && ctx->iv_tau[t] == tau[n])) ++t;
+    if (t == ctx->iv_tau.size())
+      return fail(ctx, HC_INVALID_ARG, "a node's tau is neither 0 nor the tau_inner of a type with interior viscosity");
+    cls[n] = (uint8_t)(t + 1);
+  }
+  if ((s = ensure_visc(ctx)) != HC_OK) return s;
+  HC_TRY(hipMemcpyAsync(ctx->d_visc, cls.data(), (size_t)ctx->L.N, hipMemcpyHostToDevice, ctx->stream),
+         "hipMemcpyAsync");
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  return HC_OK;
+}
+
+hc_status hc_stage_interior_viscosity(hc_ctx* ctx, int pass) {
+  hc_status s = ready_for_step(ctx);
+  if (s != HC_OK) return s;
+  if (pass != 0 && pass != 1) return fail(ctx, HC_INVALID_ARG, "pass must be 0 (entire grid) or 1 (membrane)");
+  if (!iv_active(ctx)) return fail(ctx, HC_STATE, "interior viscosity not enabled (hc_set_interior_viscosity)");
+  if (ctx->nv == 0) return HC_OK;
+  if ((s = ensure_visc(ctx)) != HC_OK) return s;
+  if (pass == 0) return do_visc_grid(ctx);
+  HC_TRY(hipMemcpyAsync(ctx->d_xold, ctx->x, sizeof(double) * 3 * ctx->vcap, hipMemcpyDeviceToDevice, ctx->stream),
+         "hipMemcpyAsync");
+  return do_visc_membrane(ctx, ctx->d_xold);
+}
+
 hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* ids, const double* forces) {
   if (!ctx || n < 0 || (n > 0 && (!ids || !forces))) return fail(ctx, HC_INVALID_ARG, "bad extra-force arguments");
   if (ctx->lattice_ready) {
@@ -1181,6 +1348,11 @@ hc_status hc_step(hc_ctx* ctx, int nsteps) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
   if (nsteps < 0) return fail(ctx, HC_INVALID_ARG, "nsteps must be >= 0");
+  // HemoCell::sanityCheck (hemoCell.cpp:614-620)
+  if (iv_active(ctx) &&
+      (ctx->iv_grid_timescale % ctx->pv_timescale != 0 || ctx->iv_timescale % ctx->pv_timescale != 0))
+    return fail(ctx, HC_INVALID_ARG,
+                "interior viscosity timescales must be multiples of the particle velocity timescale");
   return fp32(ctx) ? step_impl<float>(ctx, nsteps) : step_impl<double>(ctx, nsteps);
 }
 
@@ -1189,7 +1361,7 @@ hc_status hc_autotune(hc_ctx* ctx, int steps_per_trial, char* report, int report
   if (s != HC_OK) return s;
   if (steps_per_trial < 1) return fail(ctx, HC_INVALID_ARG, "steps_per_trial must be >= 1");
   std::string rep = "{}";
-  if (ctx->nv > 0 && ctx->fuse && !ctx->repulsion && !ctx->brep)
+  if (ctx->nv > 0 && ctx->fuse && !ctx->repulsion && !ctx->brep && !iv_active(ctx))
     s = fp32(ctx) ? autotune_impl<float>(ctx, steps_per_trial, rep) : autotune_impl<double>(ctx, steps_per_trial, rep);
   if (report && report_len > 0) {
     const size_t n = std::min<size_t>(rep.size(), (size_t)report_len - 1);
@@ -1342,7 +1514,7 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u) {
     if (dr) hipFree(dr);
     return s;
   }
-  CollideArgs a;
+  CollideArgs a{};
   a.omega = 1.0 / ctx->tau;
   for (int c = 0; c < 3; ++c) a.g[c] = ctx->g[c];
   std::memcpy(a.wall_u, ctx->wall_u, sizeof(a.wall_u));

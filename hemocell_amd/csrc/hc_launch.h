@@ -11,6 +11,10 @@ struct CollideArgs {
   double omega;
   double g[3];
   double wall_u[6][3];
+  // interior viscosity: per-node class (null when off) and the relaxation rate
+  // of each class (class 0 = omega)
+  const uint8_t* visc;
+  double omega_cls[16];
 };
 template <typename R>
 hipError_t launch_collide_stream(hipStream_t s, R* f, const R* Fcur, R* Fclear, const uint8_t* type,
@@ -88,6 +92,15 @@ hipError_t launch_boundary_flags(hipStream_t s, const uint8_t* type, const Latti
 hipError_t launch_boundary_repulsion(hipStream_t s, const VertexDev& V, const LatticeDev& L, const uint8_t* flags,
                                      double k, double r_c);
 hipError_t launch_vertex_bins(hipStream_t s, const VertexDev& V, const LatticeDev& L, int64_t* out);
+
+// interior viscosity (hc_visc.hip)
+hipError_t launch_visc_clear(hipStream_t s, uint8_t* visc, int64_t n);
+hipError_t launch_visc_grid(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
+                            const int64_t* cell_base, const LatticeDev& L, const uint8_t* type, int cls,
+                            uint8_t* visc);
+hipError_t launch_visc_membrane(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells,
+                                int ncells, const int64_t* cell_base, const double* xold, const LatticeDev& L,
+                                const uint8_t* type, int cls, uint32_t* stamp, uint8_t* visc, bool apply);
 
 // mechanics (hc_mech.hip)
 hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,

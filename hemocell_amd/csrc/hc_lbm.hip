@@ -183,7 +183,8 @@ __device__ __forceinline__ void velocity_node(R (&fl)[19], int face, const Colli
 // GENERAL: node-type mask present and/or some axis non-periodic.  VEL: the
 // velocity-boundary nodes are reconstructed in place (each node only touches its
 // own 19 slots); without VEL they are skipped and k_velocity_nodes updates them.
-template <typename R, bool ODD, bool GENERAL, bool VEL = false>
+// VISC: the relaxation rate of a node comes from its interior-viscosity class.
+template <typename R, bool ODD, bool GENERAL, bool VEL = false, bool VISC = false>
 __global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, const R* __restrict__ Fcur,
                                                           R* __restrict__ Fclr, const uint8_t* __restrict__ type,
                                                           LatticeDev L, CollideArgs a) {
@@ -209,7 +210,12 @@ __global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, con
   } else {
     if (VEL && ty >= HC_NODE_VELOCITY) velocity_node<R>(fl, ty - HC_NODE_VELOCITY, a);
     const R F0 = R(a.g[0]) + Fcur[n], F1 = R(a.g[1]) + Fcur[S + n], F2 = R(a.g[2]) + Fcur[2 * S + n];
-    collide_guo<R>(fl, F0, F1, F2, R(a.omega));
+    R om = R(a.omega);
+    if (VISC) {
+      const uint8_t vc = a.visc[n];
+      if (vc) om = R(a.omega_cls[vc & 15]);
+    }
+    collide_guo<R>(fl, F0, F1, F2, om);
   }
   Fclr[n] = R(0);
   Fclr[S + n] = R(0);
@@ -410,28 +416,37 @@ inline dim3 grid_capped(int64_t n) {
 
 }  // namespace
 
+template <typename R, bool VISC>
+void launch_bulk(hipStream_t s, dim3 g, R* f, const R* Fcur, R* Fclr, const uint8_t* type, const LatticeDev& L,
+                 const CollideArgs& a, int odd, bool general, bool vel) {
+  if (general && vel) {
+    // velocity nodes reconstructed inside the bulk kernel: one launch per step
+    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, true, true, VISC>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+    else hipLaunchKernelGGL((k_collide_stream<R, false, true, true, VISC>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+  } else if (general) {
+    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, true, false, VISC>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+    else hipLaunchKernelGGL((k_collide_stream<R, false, true, false, VISC>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+  } else {
+    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, false, false, VISC>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+    else hipLaunchKernelGGL((k_collide_stream<R, false, false, false, VISC>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+  }
+}
+
 template <typename R>
 hipError_t launch_collide_stream(hipStream_t s, R* f, const R* Fcur, R* Fclr, const uint8_t* type,
                                  const LatticeDev& L, const CollideArgs& a, int odd, bool general,
                                  const uint32_t* vel_nodes, int n_vel, bool fuse_vel) {
   dim3 g = grid_for(L.N);
-  if (general && n_vel > 0 && fuse_vel) {
-    // velocity nodes reconstructed inside the bulk kernel: one launch per step
-    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, true, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
-    else hipLaunchKernelGGL((k_collide_stream<R, false, true, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
-  } else if (general) {
-    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
-    else hipLaunchKernelGGL((k_collide_stream<R, false, true>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
-    if (n_vel > 0) {
-      dim3 gv((unsigned)((n_vel + 63) / 64));
-      if (odd)
-        hipLaunchKernelGGL((k_velocity_nodes<R, true>), gv, dim3(64), 0, s, f, Fcur, Fclr, type, vel_nodes, n_vel, L, a);
-      else
-        hipLaunchKernelGGL((k_velocity_nodes<R, false>), gv, dim3(64), 0, s, f, Fcur, Fclr, type, vel_nodes, n_vel, L, a);
-    }
-  } else {
-    if (odd) hipLaunchKernelGGL((k_collide_stream<R, true, false>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
-    else hipLaunchKernelGGL((k_collide_stream<R, false, false>), g, dim3(BLOCK), 0, s, f, Fcur, Fclr, type, L, a);
+  const bool vel = general && n_vel > 0 && fuse_vel;
+  if (a.visc) launch_bulk<R, true>(s, g, f, Fcur, Fclr, type, L, a, odd, general, vel);
+  else launch_bulk<R, false>(s, g, f, Fcur, Fclr, type, L, a, odd, general, vel);
+  if (general && !vel && n_vel > 0) {
+    // velocity nodes never hold an interior-viscosity class
+    dim3 gv((unsigned)((n_vel + 63) / 64));
+    if (odd)
+      hipLaunchKernelGGL((k_velocity_nodes<R, true>), gv, dim3(64), 0, s, f, Fcur, Fclr, type, vel_nodes, n_vel, L, a);
+    else
+      hipLaunchKernelGGL((k_velocity_nodes<R, false>), gv, dim3(64), 0, s, f, Fcur, Fclr, type, vel_nodes, n_vel, L, a);
   }
   return hipGetLastError();
 }

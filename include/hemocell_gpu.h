@@ -145,6 +145,26 @@ hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int time
  * cut-off (lattice units) with k (r_c / r) r_hat, every `timescale` steps, before the
  * spread; the force is recomputed at each application and kept between. */
 hc_status hc_set_boundary_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale);
+/* Replaces: the interior viscosity of a cell type, HemoCellField (core/hemoCellField.cpp:
+ * 99-110, INTERIOR_VISCOSITY build): bulk fluid nodes inside the membranes of type_id relax
+ * with tau_inner (the reference's viscosityRatio (tau - 1/2) + 1/2) instead of tau.
+ * tau_inner <= 0 turns it off for the type.  Type ids 0..14. */
+hc_status hc_set_interior_viscosity(hc_ctx* ctx, int type_id, double tau_inner);
+/* Replaces: HemoCell::setInteriorViscosityTimeScaleSeperation (core/hemoCell.cpp:404-410).
+ * After the constitutive model of every step whose iteration is a multiple of entire_grid,
+ * all classes are reset and each cell's inner nodes found by ray casting
+ * (findInternalParticleGridPoints, core/hemoCellParticleField.cpp:776-808); of every
+ * multiple of membrane, the support nodes of each vertex are re-classified against the
+ * membrane normal (internalGridPointsMembrane, :745-774).  Both must be multiples of the
+ * particle-velocity timescale (hc_step checks, as sanityCheck, hemoCell.cpp:614-620). */
+hc_status hc_set_interior_viscosity_timescales(hc_ctx* ctx, int membrane, int entire_grid);
+/* Replaces: the interiorViscosityField (helper/interiorViscosity.cpp:144-163), read for the
+ * InteriorViscosity output and the checkpoint: per node (z + nz (y + ny x)) 0 or the
+ * tau_inner of the type whose membrane holds it.  The setter restores such a field
+ * (InteriorViscosityHelper::restore, interiorViscosity.cpp:107-133): every value must be 0
+ * or the tau_inner of a type with interior viscosity. */
+hc_status hc_get_interior_viscosity(hc_ctx* ctx, double* tau);
+hc_status hc_set_interior_viscosity_field(hc_ctx* ctx, const double* tau);
 /* Replaces: HemoCellStretch::applyForce (helper/hemoCellStretch.cpp:63-110): an extra
  * force added to the listed vertices (global vertex ids) before every step's spread. */
 hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* vertex_ids,
@@ -178,6 +198,9 @@ hc_status hc_stage_advance(hc_ctx* ctx);         /* advanceParticles  hemoCellPa
 hc_status hc_stage_mechanics(hc_ctx* ctx, int forced); /* applyConstitutiveModel  :633-675 */
 hc_status hc_stage_repulsion(hc_ctx* ctx);       /* applyRepulsionForce  :677-743 */
 hc_status hc_stage_boundary_repulsion(hc_ctx* ctx); /* applyBoundaryRepulsionForce  :889-918 */
+/* pass 0: findInternalParticleGridPoints :776-808; pass 1: internalGridPointsMembrane
+ * :745-774 with the support of the current positions */
+hc_status hc_stage_interior_viscosity(hc_ctx* ctx, int pass);
 hc_status hc_stage_end(hc_ctx* ctx);             /* setExternalVector reset + iter++  hemoCell.cpp:369-374 */
 
 /* ---- state access (synchronous) ------------------------------------------ */

@@ -430,6 +430,10 @@ struct Oracle {
   int brep_timescale = 1;
   std::vector<long> bparticles;  // node indices, ascending x, y, z
   std::vector<double> fbr;       // 3*NV
+  // interior viscosity: tau inside the membranes per type (0 = off), node classes
+  std::vector<double> iv_tau;
+  std::vector<uint8_t> visc;     // N: 0 background, 1 + type inside
+  int iv_timescale = 1, iv_grid_timescale = 1;
 
   long idx(int X, int Y, int Z) const { return (long)Z + (long)nz * ((long)Y + (long)ny * X); }
   int dim(int a) const { return a == 0 ? nx : (a == 1 ? ny : nz); }
@@ -457,7 +461,9 @@ struct Oracle {
       velocity_bc_complete(fl, face / 2, (face % 2) ? 1 : -1, wall_u[face]);
     }
     double F[3] = {g[0] + Fibm[n], g[1] + Fibm[N + n], g[2] + Fibm[2 * N + n]};
-    collide_guo(fl, F, omega);
+    const double ti = (!visc.empty() && visc[n]) ? ivt(visc[n] - 1) : 0.0;
+    const double om = ti > 0.0 ? 1.0 / ti : omega;
+    collide_guo(fl, F, om);
   }
 
   void collide_and_stream() {
@@ -875,6 +881,157 @@ struct Oracle {
     }
   }
 
+  // ---- interior viscosity (the reference's INTERIOR_VISCOSITY build) ----
+  // Fluid nodes inside a membrane relax with the type's tau_inner
+  // (hemoCellField.cpp:99-110); visc holds 0 (background) or 1 + type per node.
+  double ivt(int t) const { return t < (int)iv_tau.size() ? iv_tau[t] : 0.0; }
+  bool iv_on() const {
+    for (double t : iv_tau)
+      if (t > 0.0) return true;
+    return false;
+  }
+
+  // Moller-Trumbore test of the ray from lattice site s along (-40,-40,-40)
+  // against one triangle: 1 when it crosses at t > 1e-7, t rounded to float
+  // (helper/mollerTrumbore.h:26-78).
+  static int ray_crosses(const double* v0, const double* v1, const double* v2, const long s[3]) {
+    const double EPS = 0.0000001;
+    const double ray[3] = {-40.0, -40.0, -40.0};
+    double e1[3], e2[3], sv[3];
+    for (int c = 0; c < 3; ++c) {
+      e1[c] = v1[c] - v0[c];
+      e2[c] = v2[c] - v0[c];
+    }
+    const double pv[3] = {ray[1] * e2[2] - ray[2] * e2[1], ray[2] * e2[0] - ray[0] * e2[2],
+                          ray[0] * e2[1] - ray[1] * e2[0]};
+    const double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+    if (det > -EPS && det < EPS) return 0;
+    const double inv = 1 / det;
+    for (int c = 0; c < 3; ++c) sv[c] = (double)s[c] - v0[c];
+    const double u = inv * (sv[0] * pv[0] + sv[1] * pv[1] + sv[2] * pv[2]);
+    if (u < 0.0 || u > 1.0) return 0;
+    const double qv[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2],
+                          sv[0] * e1[1] - sv[1] * e1[0]};
+    const double v = inv * (ray[0] * qv[0] + ray[1] * qv[1] + ray[2] * qv[2]);
+    if (v < 0.0 || u + v > 1.0) return 0;
+    const float t = (float)(inv * (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]));
+    return t > EPS ? 1 : 0;
+  }
+
+  // Entire-grid pass (hemoCellParticleField.cpp:776-808): every node back to the
+  // background, then each node of a cell's bounding box (corners truncated to
+  // integers, octree.h:107-150; clamped to the domain on non-periodic axes) whose
+  // ray crosses the cell's triangles an odd number of times is inner.  The
+  // reference culls triangles with an octree first; the cull only drops
+  // triangles the ray cannot reach, so every triangle is tested here.  Only
+  // bulk fluid nodes change.
+  void iv_entire_grid() {
+    std::fill(visc.begin(), visc.end(), (uint8_t)0);
+    const int nc = (int)cell_type.size();
+    for (int c = 0; c < nc; ++c) {
+      const int t = cell_type[c];
+      if (!(ivt(t) > 0.0)) continue;
+      const CellType& ct = types[t];
+      const long b = cell_base[c];
+      double lo[3], hi[3];
+      for (int a = 0; a < 3; ++a) lo[a] = hi[a] = x[3 * b + a];
+      for (int k = 1; k < ct.nv; ++k)
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(lo[a], x[3 * (b + k) + a]);
+          hi[a] = std::max(hi[a], x[3 * (b + k) + a]);
+        }
+      long a0[3], a1[3];
+      for (int a = 0; a < 3; ++a) {
+        a0[a] = (long)lo[a];
+        a1[a] = (long)hi[a];
+        if (!periodic[a]) {
+          a0[a] = std::max(a0[a], 0L);
+          a1[a] = std::min(a1[a], (long)dim(a) - 1);
+        }
+      }
+      for (long X = a0[0]; X <= a1[0]; ++X)
+        for (long Y = a0[1]; Y <= a1[1]; ++Y)
+          for (long Z = a0[2]; Z <= a1[2]; ++Z) {
+            const long s[3] = {X, Y, Z};
+            int cnt = 0;
+            for (int q = 0; q < ct.nt; ++q) {
+              const int* T = &ct.tri[3 * q];
+              cnt += ray_crosses(&x[3 * (b + T[0])], &x[3 * (b + T[1])], &x[3 * (b + T[2])], s);
+            }
+            if (!(cnt % 2)) continue;
+            const long n = idx(wrap(0, X), wrap(1, Y), wrap(2, Z));
+            if (type[n] == FLUID) visc[n] = (uint8_t)(1 + t);
+          }
+    }
+  }
+
+  // Membrane pass (hemoCellParticleField.cpp:745-774): each vertex, in vertex
+  // order, visits the support nodes of its position before this step's advance
+  // (the kernel coordinates of the interpolation); a node within edge_mean_eq
+  // of the new position is inner when (node - x) . normal < 0, else background.
+  // normal = sum over the vertex's triangles, ascending, of unit normal x
+  // area / area_mean_eq at the new positions (rbcHighOrderModel.cpp:114-121;
+  // only the RBC model computes it, so the other models' normal is zero).
+  void iv_membrane(const std::vector<double>& xold) {
+    std::vector<std::vector<std::vector<int>>> vt(types.size());
+    for (long i = 0; i < NV(); ++i) {
+      const int c = vcell[i], t = cell_type[c];
+      if (!(ivt(t) > 0.0)) continue;
+      const CellType& ct = types[t];
+      const long b = cell_base[c];
+      const int k = (int)(i - b);
+      double nrm[3] = {0.0, 0.0, 0.0};
+      if (ct.model == RBC) {
+        if (vt[t].empty()) vt[t] = vertex_triangles(ct);
+        for (int q : vt[t][k]) {
+          const int* T = &ct.tri[3 * q];
+          double area;
+          V3 n;
+          area_and_unit_normal(&x[3 * (b + T[0])], &x[3 * (b + T[1])], &x[3 * (b + T[2])], area, n);
+          const double s = area / ct.area_mean_eq;
+          for (int a = 0; a < 3; ++a) nrm[a] += n.x[a] * s;
+        }
+      }
+      long nd[27][3], nid[27];
+      const int ns = support_nodes(&xold[3 * i], nd, nid);
+      for (int j = 0; j < ns; ++j) {
+        double lp[3];
+        for (int a = 0; a < 3; ++a) lp[a] = (double)nd[j][a] - x[3 * i + a];
+        const double len = std::sqrt(lp[0] * lp[0] + lp[1] * lp[1] + lp[2] * lp[2]);
+        if (len > ct.edge_mean_eq) continue;
+        const double d = lp[0] * nrm[0] + lp[1] * nrm[1] + lp[2] * nrm[2];
+        visc[nid[j]] = d < 0.0 ? (uint8_t)(1 + t) : (uint8_t)0;
+      }
+    }
+  }
+
+  // the nodes make_kernel keeps for position p: unwrapped coordinates and index
+  int support_nodes(const double* p, long nd[27][3], long nid[27]) const {
+    int m = 0;
+    long ctr[3];
+    for (int a = 0; a < 3; ++a) ctr[a] = (long)std::floor(p[a] + 0.5);
+    for (int dx = -1; dx < 2; ++dx)
+      for (int dy = -1; dy < 2; ++dy)
+        for (int dz = -1; dz < 2; ++dz) {
+          const long q[3] = {ctr[0] + dx, ctr[1] + dy, ctr[2] + dz};
+          int w3[3];
+          bool out = false;
+          for (int a = 0; a < 3; ++a) {
+            w3[a] = wrap(a, q[a]);
+            if (w3[a] < 0) out = true;
+          }
+          if (out) continue;
+          double w = 1.0;
+          for (int a = 0; a < 3; ++a) w *= std::max(1.0 - std::fabs(p[a] - (double)q[a]), 0.0);
+          if (w == 0.0) continue;
+          const long n = idx(w3[0], w3[1], w3[2]);
+          if (is_boundary(type[n])) continue;
+          for (int a = 0; a < 3; ++a) nd[m][a] = q[a];
+          nid[m++] = n;
+        }
+    return m;
+  }
+
   void reset_force_field() { std::fill(Fibm.begin(), Fibm.end(), 0.0); }
 
   void step() {
@@ -883,8 +1040,14 @@ struct Oracle {
     spread();
     collide_and_stream();
     if (iter % pv_timescale == 0) interpolate();
+    const bool iv = iv_on();
+    std::vector<double> xold;
+    if (iv && iter % iv_timescale == 0) xold = x;
     advance();
     mechanics(false);
+    // hemoCell.cpp:347-357: after the constitutive model
+    if (iv && iter % iv_grid_timescale == 0) iv_entire_grid();
+    if (iv && iter % iv_timescale == 0) iv_membrane(xold);
     reset_force_field();
     iter++;
   }
@@ -1070,6 +1233,34 @@ void ho_set_boundary_repulsion(void* h, double k, double cutoff, int timescale) 
 void ho_get_boundary_repulsion(void* h, double* fr) {
   Oracle* o = (Oracle*)h;
   for (long i = 0; i < 3 * o->NV(); ++i) fr[i] = i < (long)o->fbr.size() ? o->fbr[i] : 0.0;
+}
+void ho_set_interior_viscosity(void* h, int t, double tau_inner) {
+  Oracle* o = (Oracle*)h;
+  if (o->iv_tau.size() < o->types.size()) o->iv_tau.resize(o->types.size(), 0.0);
+  o->iv_tau[t] = tau_inner > 0.0 ? tau_inner : 0.0;
+  if (o->visc.empty()) o->visc.assign(o->N, 0);
+}
+void ho_set_interior_viscosity_timescales(void* h, int membrane, int grid) {
+  ((Oracle*)h)->iv_timescale = membrane;
+  ((Oracle*)h)->iv_grid_timescale = grid;
+}
+// per node: 0, or the tau_inner of the type whose membrane holds it
+void ho_get_interior_viscosity(void* h, double* tau) {
+  Oracle* o = (Oracle*)h;
+  for (long n = 0; n < o->N; ++n) tau[n] = (!o->visc.empty() && o->visc[n]) ? o->ivt(o->visc[n] - 1) : 0.0;
+}
+void ho_set_interior_viscosity_classes(void* h, const uint8_t* cls) {
+  Oracle* o = (Oracle*)h;
+  o->visc.assign(cls, cls + o->N);
+}
+// pass 0: entire grid; pass 1: membrane with the support at the current positions
+void ho_stage_interior_viscosity(void* h, int pass) {
+  Oracle* o = (Oracle*)h;
+  if (pass == 0) o->iv_entire_grid();
+  else o->iv_membrane(o->x);
+}
+int ho_ray_crosses(const double* v0, const double* v1, const double* v2, const long* s) {
+  return Oracle::ray_crosses(v0, v1, v2, s);
 }
 void ho_stage_boundary_repulsion(void* h) { ((Oracle*)h)->boundary_repulsion_force(); }
 void ho_get_repulsion(void* h, double* fr) { Oracle* o = (Oracle*)h; std::copy(o->frep.begin(), o->frep.end(), fr); }

@@ -47,6 +47,12 @@ _SIG = {
     "ho_set_boundary_repulsion": (None, [_P, ctypes.c_double, ctypes.c_double, _INT]),
     "ho_get_boundary_repulsion": (None, [_P, _D]),
     "ho_stage_boundary_repulsion": (None, [_P]),
+    "ho_set_interior_viscosity": (None, [_P, _INT, ctypes.c_double]),
+    "ho_set_interior_viscosity_timescales": (None, [_P, _INT, _INT]),
+    "ho_get_interior_viscosity": (None, [_P, _D]),
+    "ho_set_interior_viscosity_classes": (None, [_P, ctypes.POINTER(ctypes.c_uint8)]),
+    "ho_stage_interior_viscosity": (None, [_P, _INT]),
+    "ho_ray_crosses": (_INT, [_D, _D, _D, _L]),
     "ho_vertex_bins": (None, [_P, ctypes.POINTER(ctypes.c_long)]),
     "ho_stage_spread": (None, [_P]),
     "ho_stage_collide_stream": (None, [_P]),
@@ -132,6 +138,30 @@ class OracleSim:
         fr = np.zeros((self.num_vertices, 3))
         self.lib.ho_get_boundary_repulsion(self.h, _dp(fr))
         return fr
+
+    def set_iteration(self, it):
+        self.lib.ho_set_iter(self.h, int(it))
+
+    def set_interior_viscosity(self, type_id, tau_inner):
+        self.lib.ho_set_interior_viscosity(self.h, type_id, tau_inner)
+
+    def set_interior_viscosity_timescales(self, membrane, entire_grid):
+        self.lib.ho_set_interior_viscosity_timescales(self.h, membrane, entire_grid)
+
+    def interior_viscosity(self):
+        """Relaxation time per node (nx, ny, nz), 0 where the background applies."""
+        t = np.zeros(self.N)
+        self.lib.ho_get_interior_viscosity(self.h, _dp(t))
+        return t.reshape(self.shape)
+
+    def set_interior_viscosity_classes(self, cls):
+        c = np.ascontiguousarray(cls, dtype=np.uint8).reshape(-1)
+        assert c.size == self.N
+        self.lib.ho_set_interior_viscosity_classes(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+
+    def stage_interior_viscosity(self, which):
+        """0: entire-grid ray-casting pass; 1: membrane pass (support at the current positions)."""
+        self.lib.ho_stage_interior_viscosity(self.h, which)
 
     def repulsion_force(self):
         fr = np.zeros((self.num_vertices, 3))
@@ -251,3 +281,10 @@ def velocity_bc_one(f, axis, orient, uw):
 
 def feq(i, rho_bar, j):
     return library().ho_feq(i, rho_bar, _dp(np.asarray(j, dtype=np.float64)))
+
+
+def ray_crosses(v0, v1, v2, site):
+    """1 when the ray from the integer site along (-1,-1,-1) crosses triangle (v0, v1, v2)."""
+    s = np.asarray(site, dtype=np.int64)
+    return library().ho_ray_crosses(_dp(np.asarray(v0, dtype=np.float64)), _dp(np.asarray(v1, dtype=np.float64)),
+                                    _dp(np.asarray(v2, dtype=np.float64)), s.ctypes.data_as(_L))
