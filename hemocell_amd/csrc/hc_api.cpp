@@ -595,7 +595,11 @@ hc_status one_step(hc_ctx* ctx) {
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_STEP, &ev);
   const bool iv = iv_active(ctx) && ctx->nv > 0;
-  const bool fused = ctx->fuse && !ctx->repulsion && !ctx->brep && !iv && ctx->nv > 0;
+  const bool fused = ctx->fuse && !ctx->repulsion && !ctx->brep && ctx->nv > 0;
+  // interior viscosity (hemoCell.cpp:347-357): after the constitutive model; the
+  // membrane pass takes the support of the positions before the advance
+  const bool memb = iv && ctx->iter % ctx->iv_timescale == 0;
+  const bool grid = iv && ctx->iter % ctx->iv_grid_timescale == 0;
   if (ctx->repulsion && ctx->nv > 0 && ctx->iter % ctx->rep_timescale == 0 && (s = do_repulsion(ctx)) != HC_OK)
     return s;
   if (ctx->brep && ctx->nv > 0 && ctx->iter % ctx->brep_timescale == 0 && (s = do_boundary_repulsion(ctx)) != HC_OK)
@@ -603,6 +607,10 @@ hc_status one_step(hc_ctx* ctx) {
   if (ctx->nv > 0 && !ctx->spread_ready && (s = do_spread<R>(ctx)) != HC_OK) return s;
   if ((s = do_collide<R>(ctx)) != HC_OK) return s;
   const int do_interp = ctx->iter % ctx->pv_timescale == 0;
+  if (iv && (s = ensure_visc(ctx)) != HC_OK) return s;
+  if (memb)
+    HC_TRY(hipMemcpyAsync(ctx->d_xold, ctx->x, sizeof(double) * 3 * ctx->vcap, hipMemcpyDeviceToDevice, ctx->stream),
+           "hipMemcpyAsync");
   if (fused) {
     if (do_interp) {
       hipEvent_t e2;
@@ -614,18 +622,11 @@ hc_status one_step(hc_ctx* ctx) {
     }
     if ((s = do_cell_step<R>(ctx, do_interp)) != HC_OK) return s;
   } else {
-    // interior viscosity (hemoCell.cpp:347-357): after the constitutive model,
-    // the membrane pass with the support of the positions before the advance
-    const bool memb = iv && ctx->iter % ctx->iv_timescale == 0;
-    if (iv && (s = ensure_visc(ctx)) != HC_OK) return s;
-    if (memb)
-      HC_TRY(hipMemcpyAsync(ctx->d_xold, ctx->x, sizeof(double) * 3 * ctx->vcap, hipMemcpyDeviceToDevice, ctx->stream),
-             "hipMemcpyAsync");
     if ((s = do_interp_advance<R>(ctx, do_interp, 1)) != HC_OK) return s;
     if ((s = do_mechanics(ctx, false)) != HC_OK) return s;
-    if (iv && ctx->iter % ctx->iv_grid_timescale == 0 && (s = do_visc_grid(ctx)) != HC_OK) return s;
-    if (memb && (s = do_visc_membrane(ctx, ctx->d_xold)) != HC_OK) return s;
   }
+  if (grid && (s = do_visc_grid(ctx)) != HC_OK) return s;
+  if (memb && (s = do_visc_membrane(ctx, ctx->d_xold)) != HC_OK) return s;
   if ((s = do_end<R>(ctx)) != HC_OK) return s;
   ctx->spread_ready = fused;
   prof_end(ctx, HC_PROF_STEP, ev);

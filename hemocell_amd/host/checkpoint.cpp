@@ -11,6 +11,9 @@
 //                      velocity, force and repulsion force (the fields of
 //                      HemoCellParticle::serializeValues_t,
 //                      core/hemoCellParticle.h:45-63)
+//   internalViscosity.dat  (with interior viscosity on) header + the tau inside
+//                      membranes per node, 0 elsewhere (InteriorViscosityHelper::
+//                      checkpoint / restore, helper/interiorViscosity.cpp:82-133)
 //   checkpoint.xml     <Checkpoint><hemocell>...</hemocell><General>
 //                      <Iteration>N</Iteration><OutDirectory>d</OutDirectory>
 //                      </General></Checkpoint>
@@ -30,6 +33,7 @@ namespace {
 
 const char kLatticeTag[8] = {'H', 'C', 'C', 'K', 'L', 'A', 'T', '1'};
 const char kParticleTag[8] = {'H', 'C', 'C', 'K', 'P', 'A', 'R', '1'};
+const char kViscosityTag[8] = {'H', 'C', 'C', 'K', 'I', 'V', 'F', '1'};
 
 std::string escape(const std::string& t) {
   std::string r;
@@ -106,6 +110,10 @@ void HemoCell::saveCheckPoint() {
   rename_to_old(dir + "particleField.dat");
   rename_to_old(dir + "checkpoint.xml");
   const int64_t it = (int64_t)iter;
+  bool iv = false;
+  if (cellfields)
+    for (auto& F : cellfields->fields) iv = iv || F->doInteriorViscosity;
+  if (iv) rename_to_old(dir + "internalViscosity.dat");
 
   // checkpoint.xml
   const XMLNode* hc = cfg->hemocellNode();
@@ -173,6 +181,19 @@ void HemoCell::saveCheckPoint() {
     put_vec(o, fv);
     put_vec(o, fr);
     if (!o) throw HemoCellError("saveCheckPoint: cannot write " + dir + "particleField.dat");
+  }
+
+  // internalViscosity.dat
+  if (iv) {
+    const int64_t N = hc_num_nodes(ctx);
+    std::vector<double> tau((size_t)N);
+    check(hc_get_interior_viscosity(ctx, tau.data()), "saveCheckPoint: interior viscosity");
+    std::ofstream o(dir + "internalViscosity.dat", std::ios::binary);
+    o.write(kViscosityTag, 8);
+    put<int64_t>(o, it);
+    put<int64_t>(o, N);
+    put_vec(o, tau);
+    if (!o) throw HemoCellError("saveCheckPoint: cannot write " + dir + "internalViscosity.dat");
   }
 }
 
@@ -266,6 +287,21 @@ void HemoCell::loadCheckPoint() {
       check(hc_set_vertices(ctx, x.data(), v.data(), fv.data()), "loadCheckPoint: vertices");
       check(hc_set_repulsion_force(ctx, fr.data()), "loadCheckPoint: repulsion force");
     }
+  }
+  bool iv = false;
+  for (auto& F : cellfields->fields) iv = iv || F->doInteriorViscosity;
+  if (iv) {
+    // helper/interiorViscosity.cpp:107-133: the field must exist when interior viscosity is on
+    const std::string file = dir + "internalViscosity.dat";
+    std::ifstream i(file, std::ios::binary);
+    if (!i.is_open()) throw HemoCellError("loadCheckPoint: interior viscosity is on but " + file + " does not exist");
+    check_tag(i, kViscosityTag, file);
+    const int64_t it_iv = get<int64_t>(i, file), N = get<int64_t>(i, file);
+    if (it_iv != it_lat) throw HemoCellError("loadCheckPoint: internalViscosity.dat is of another step");
+    if (N != hc_num_nodes(ctx)) throw HemoCellError("loadCheckPoint: " + file + " holds a lattice of another size");
+    std::vector<double> tau((size_t)N);
+    get_vec(i, tau, file);
+    check(hc_set_interior_viscosity_field(ctx, tau.data()), "loadCheckPoint: interior viscosity");
   }
   const int64_t it = it_xml >= 0 ? it_xml : it_lat;
   check(hc_set_iteration(ctx, it), "loadCheckPoint: iteration");

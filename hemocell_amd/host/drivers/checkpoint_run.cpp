@@ -10,7 +10,9 @@
 //   KIND:     RBC, PLT or WBC as for load_particles
 //   HEMOCELL_TIMESCALES=m,v,r  material, particle-velocity and repulsion
 //             timescale separations (default 1,1,1); with HEMOCELL_REPULSION=
-//             k,cutoff_um the repulsion is on
+//             k,cutoff_um the repulsion is on; HEMOCELL_INTERIOR_VISCOSITY=m,e
+//             sets the interior-viscosity timescales (membrane, entire grid) of
+//             the types whose material enables it
 // out.bin: int64 iteration, int64 ncells, int64 nvertices, int32 cell id[ncells],
 //          float64 x[nvertices][3], v[nvertices][3], f[nvertices][3],
 //          float64 populations[19][nodes]
@@ -72,6 +74,14 @@ static int run(int argc, char* argv[]) {
     hemocell.setRepulsionTimeScaleSeperation(ts[2]);
   }
 
+  if (const char* e = std::getenv("HEMOCELL_INTERIOR_VISCOSITY")) {
+    std::stringstream ss(e);
+    unsigned m = 1, g = 1;
+    char sep;
+    ss >> m >> sep >> g;
+    hemocell.setInteriorViscosityTimeScaleSeperation(m, g);
+  }
+
   if (!cfg->checkpointed) {
     hemocell.loadParticles();
   } else {
@@ -100,10 +110,14 @@ static int run(int argc, char* argv[]) {
   std::fwrite(pop.data(), 8, pop.size(), o);
   std::fclose(o);
   const hemo::FluidStatistics fs = hemo::FluidInfo::calculateVelocityStatistics(&hemocell);
+  std::vector<double> tau_iv((size_t)N);
+  hemocell.check(hc_get_interior_viscosity(ctx, tau_iv.data()), "interior viscosity");
+  long inner = 0;
+  for (double t : tau_iv) inner += t != 0.0;
   std::printf("{\"start\": %u, \"iteration\": %lld, \"cells\": %lld, \"vertices\": %lld, \"out\": \"%s\", "
-              "\"u_max\": %.17g, \"u_mean\": %.17g, \"fluid_nodes\": %lu}\n",
+              "\"u_max\": %.17g, \"u_mean\": %.17g, \"fluid_nodes\": %lu, \"inner_nodes\": %ld}\n",
               start, (long long)it, (long long)nc, (long long)nv, hemocell.outputDirectory().c_str(), fs.max, fs.avg,
-              (unsigned long)fs.ncells);
+              (unsigned long)fs.ncells, inner);
   return 0;
 }
 

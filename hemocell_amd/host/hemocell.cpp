@@ -495,6 +495,17 @@ HemoCellField::HemoCellField(HemoCellFields& cf, const std::string& name_, int c
   cf.hemocell.check(hc_add_cell_type(ctx, model, nv, nt, triangles.data(), rest.data(), (int)inner_edges.size() / 2,
                                      inner_edges.empty() ? nullptr : inner_edges.data(), &params, 1, &type_id),
                     "addCellType");
+  // interior viscosity (core/hemoCellField.cpp:99-110): tau inside the membrane
+  // from the viscosity ratio, on when the material enables it
+  interiorViscosityTau = read_or(m, "viscosityRatio", 1.0) * (param::tau - 0.5) + 0.5;
+  try {
+    doInteriorViscosity = m["MaterialModel"]["enableInteriorViscosity"].read<bool>();
+  } catch (std::invalid_argument&) {
+  }
+  if (doInteriorViscosity) {
+    std::cout << "(HemoCell) (AddCellType) (" << name << ") Enabling interior viscosity" << std::endl;
+    cf.hemocell.check(hc_set_interior_viscosity(ctx, type_id, interiorViscosityTau), "interior viscosity");
+  }
 }
 
 HemoCellField* HemoCellFields::operator[](const std::string& name) {
@@ -589,6 +600,16 @@ void HemoCell::latticeEquilibrium(T rho, Array<T, 3> vel) {
 
 void HemoCell::initializeCellfield() {
   if (!cellfields) cellfields = new HemoCellFields(*this);
+}
+
+// core/hemoCell.cpp:404-410
+void HemoCell::setInteriorViscosityTimeScaleSeperation(unsigned int separation, unsigned int separation_entire_grid) {
+  std::cout << "(HemoCell) (Interior Viscosity Timescale Seperation) Setting seperation to " << separation
+            << " timesteps" << std::endl;
+  std::cout << "(HemoCell) (Interior Viscosity Timescale Seperation) Setting entire grid refresh to "
+            << separation_entire_grid << " timesteps" << std::endl;
+  check(hc_set_interior_viscosity_timescales(ctx, (int)separation, (int)separation_entire_grid),
+        "setInteriorViscosityTimeScaleSeperation");
 }
 
 // core/hemoCell.cpp:419-426: the cut-off is given in micrometres

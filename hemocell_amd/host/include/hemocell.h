@@ -76,7 +76,7 @@ enum {
   OUTPUT_FORCE_AREA = 6, OUTPUT_FORCE_VISC = 7, OUTPUT_TRIANGLES = 8, OUTPUT_CELL_ID = 9, OUTPUT_VERTEX_ID = 10,
   OUTPUT_VELOCITY = 11, OUTPUT_DENSITY = 12, OUTPUT_FORCE_REPULSION = 13, OUTPUT_SHEAR_RATE = 14,
   OUTPUT_STRAIN_RATE = 15, OUTPUT_SHEAR_STRESS = 16, OUTPUT_BOUNDARY = 17, OUTPUT_OMEGA = 18,
-  OUTPUT_CELL_DENSITY = 19, OUTPUT_FORCE_INNER_LINK = 20
+  OUTPUT_CELL_DENSITY = 19, OUTPUT_FORCE_INNER_LINK = 20, OUTPUT_INTERIOR_POINTS = 21
 };
 
 class HemoCellError : public std::runtime_error {
@@ -268,6 +268,10 @@ class HemoCellField {
   int numVertex = 0;
   int timescale = 1;
   T minimumDistanceFromSolid = 0.0;
+  // interior viscosity: <MaterialModel><enableInteriorViscosity> and the tau inside
+  // the membrane, <viscosityRatio> (tau - 1/2) + 1/2
+  bool doInteriorViscosity = false;
+  T interiorViscosityTau = 1.0;
   std::unique_ptr<Config> materialCfg;
   std::vector<T> rest;                          // nv*3, lattice units, centred
   std::vector<int32_t> triangles;               // nt*3
@@ -322,6 +326,9 @@ class HemoCell {
   void setRepulsionTimeScaleSeperation(unsigned int separation);
   void setMaterialTimeScaleSeparation(std::string name, unsigned int separation);
   void setParticleVelocityUpdateTimeScaleSeparation(unsigned int separation);
+  // membrane pass every `separation` steps, ray-casting pass every
+  // `separation_entire_grid` steps (core/hemoCell.cpp:404-410)
+  void setInteriorViscosityTimeScaleSeperation(unsigned int separation, unsigned int separation_entire_grid);
   void setInitialMinimumDistanceFromSolid(std::string name, T distance);
   void setSystemPeriodicity(unsigned int axis, bool bePeriodic);
   void loadParticles();

@@ -253,9 +253,17 @@ void HemoCell::writeOutput() {
       if (lattice->periodicity().get(axis)) return (c + n) % n;
       return c < 0 ? (plint)0 : n - 1;
     };
+    // relaxation time inside membranes (0 elsewhere), for Omega and InteriorPoints
+    std::vector<double> tau_iv;
+    for (int o : fluidOutputs)
+      if ((o == OUTPUT_OMEGA || o == OUTPUT_INTERIOR_POINTS) && tau_iv.empty()) {
+        tau_iv.resize(N);
+        check(hc_get_interior_viscosity(ctx, tau_iv.data()), "writeOutput");
+      }
     for (int o : fluidOutputs) {
       const bool vel = o == OUTPUT_VELOCITY, den = o == OUTPUT_DENSITY, bnd = o == OUTPUT_BOUNDARY;
-      if (!vel && !den && !bnd) continue;  // shear, strain, force, omega, cell density: not written
+      const bool omg = o == OUTPUT_OMEGA, inp = o == OUTPUT_INTERIOR_POINTS;
+      if (!vel && !den && !bnd && !omg && !inp) continue;  // shear, strain, force, cell density: not written
       const hsize_t comp = vel ? 3 : 1;
       std::vector<float> d(Nx * Ny * Nz * comp);
       size_t n = 0;
@@ -268,11 +276,21 @@ void HemoCell::writeOutput() {
               for (int q = 0; q < 3; ++q) d[n++] = (float)(u[3 * node + q] * (si ? dx / dt : 1.0));
             else if (den)
               d[n++] = (float)(rho[node] * (si ? df / (dx * dx) : 1.0));
+            else if (omg)
+              // FluidHdf5IO.hh:354-370, scaled as the reference does in SI output;
+              // bounce-back nodes have no relaxation rate
+              d[n++] = lattice->isBoundary(sx, sy, sz)
+                           ? 0.f
+                           : (float)((1.0 / (tau_iv[node] > 0.0 ? tau_iv[node] : param::tau)) *
+                                     (si ? df / (dx * dx) : 1.0));
+            else if (inp)
+              d[n++] = (float)tau_iv[node];  // FluidHdf5IO.hh:333-352
             else
               d[n++] = lattice->isBoundary(sx, sy, sz) ? 1.f : 0.f;
           }
       const hsize_t dim[4] = {Nz, Ny, Nx, comp};
-      dataset4(file, vel ? "Velocity" : (den ? "Density" : "Boundary"), d, dim);
+      const char* dname = vel ? "Velocity" : den ? "Density" : omg ? "Omega" : inp ? "InteriorPoints" : "Boundary";
+      dataset4(file, dname, d, dim);
     }
     H5Fclose(file);
   }

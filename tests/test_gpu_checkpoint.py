@@ -212,3 +212,47 @@ def test_host_layer_checkpoint_files_and_restart(tmp_path):
     assert cases.rel_l2(a["pop"], b["pop"]) < 1e-11
     assert cases.rel_l2(a["x"], b["x"]) <= 100 * max(cases.rel_l2(a["x"], a2["x"]), 1e-15)
     assert cases.rel_l2(a["v"], b["v"]) <= 100 * max(cases.rel_l2(a["v"], a2["v"]), 1e-15)
+
+
+def test_host_layer_checkpoint_with_interior_viscosity(tmp_path):
+    """Interior viscosity through hemo::HemoCell: the RBC material enables it
+    (<enableInteriorViscosity>, <viscosityRatio> 5), setInteriorViscosityTimeScaleSeperation
+    sets the two passes, saveCheckPoint writes internalViscosity.dat and a restart
+    ends where the uninterrupted run ends, with the same inner nodes."""
+    nx, ny, nz = 48, 40, 40
+    rbc = dict(cases.RBC_MAT, viscosityRatio=5.0, enableInteriorViscosity=1)
+    files = {"RBC.xml": cases.material_xml(rbc, "RBC"),
+             "PLT.xml": cases.material_xml(cases.PLT_MAT, "PLT", inner_edges=21),
+             "RBC.pos": "2\n6 7 7 90 0 0\n17 12 12 0 30 0\n", "PLT.pos": "1\n12 4 10 10 20 30\n",
+             "config.xml": cases.config_xml()}
+    env = dict(os.environ, HEMOCELL_TIMESCALES="1,1,1", HEMOCELL_INTERIOR_VISCOSITY="2,4")
+    drv = os.path.join(BIN, "checkpoint_run")
+    common = [str(nx), str(ny), str(nz), "2e-3"]
+    specs = ["RBC:RBC", "PLT:PLT"]
+    runs = {}
+    for name, args in (("whole", ["config.xml"] + common + ["14", "0", "out.bin"]),
+                       ("split", ["config.xml"] + common + ["8", "4", "first.bin"])):
+        d = tmp_path / name
+        d.mkdir()
+        for k, v in files.items():
+            (d / k).write_text(v)
+        runs[name] = _run([drv] + args + specs, cwd=d, env=env)
+    chk = tmp_path / "split" / "tmp" / "checkpoint"
+    assert (chk / "internalViscosity.dat").is_file() and (chk / "internalViscosity.dat.old").is_file()
+    assert runs["split"]["inner_nodes"] > 500
+    res = _run([drv, "tmp/checkpoint/checkpoint.xml"] + common + ["14", "0", "resumed.bin"] + specs,
+               cwd=tmp_path / "split", env=env)
+    assert res["start"] == 8 and res["iteration"] == 14
+    assert res["inner_nodes"] == runs["whole"]["inner_nodes"] > 500
+    a, b = _read_out(tmp_path / "whole" / "out.bin"), _read_out(tmp_path / "split" / "resumed.bin")
+    assert cases.rel_l2(a["x"], b["x"]) < 1e-12
+    assert cases.rel_l2(a["pop"], b["pop"]) < 1e-11
+    # the inner nodes relax with the larger tau: the run differs from one without it
+    plain = tmp_path / "plain"
+    plain.mkdir()
+    for k, v in files.items():
+        (plain / k).write_text(v.replace("<enableInteriorViscosity> 1 </enableInteriorViscosity>", ""))
+    rp = _run([drv, "config.xml"] + common + ["14", "0", "out.bin"] + specs, cwd=plain, env=env)
+    assert rp["inner_nodes"] == 0
+    c = _read_out(plain / "out.bin")
+    assert cases.rel_l2(a["pop"], c["pop"]) > 1e3 * max(cases.rel_l2(a["pop"], b["pop"]), 1e-15)

@@ -70,6 +70,36 @@ def test_entire_grid_pass_with_walls_matches_oracle():
     assert np.array_equal(g.interior_viscosity(), o.interior_viscosity())
 
 
+def test_entire_grid_pass_large_cell_matches_oracle():
+    """A 2562-vertex WBC (its positions fill the LDS: nodes in x, y, z order)
+    beside an RBC stretched along x (more lines than the line table holds)."""
+    print("[iv] entire grid, large cells", flush=True)
+    u = cases.units()
+    shape = (72, 44, 72)
+    g = hc.Simulation(0)
+    g.lattice_init(*shape, u.tau)
+    o = OracleSim(*shape, u.tau)
+    rng = np.random.default_rng(21)
+    tids = []
+    # the RBC lies flat in y, stretched in x and z: (ex + ez) (ey + ez) lines > 4096
+    for kind, center, rot, stretch in (("WBC2562", (12.3, 10.6, 11.8), cases.rotation(rng), (1.0, 1.0, 1.0)),
+                                       ("RBC", (40.2, 31.7, 40.3), np.eye(3), (3.5, 1.0, 3.5))):
+        model, v, t, p, inner = cases.mesh(kind, u, hc.build_mesh)
+        v0 = v - v.mean(axis=0)
+        flat = v0[:, np.argsort(np.ptp(v0, axis=0))[::-1][[0, 2, 1]]]  # thin axis to y
+        x = cases.place(flat if kind == "RBC" else v0, (0, 0, 0), rot) * np.array(stretch) + np.array(center)
+        tid = g.add_cell_type(model, v, t, p, inner)
+        assert o.add_cell_type(model, v, t, p, inner) == tid
+        both(g, o, "add_cells", tid, x[None])
+        both(g, o, "set_interior_viscosity", tid, TAU_RBC + tid)
+        tids.append(tid)
+    g.stage("interior_viscosity", 0)
+    o.stage_interior_viscosity(0)
+    to = o.interior_viscosity()
+    assert (to == TAU_RBC).sum() > 2000 and (to == TAU_RBC + 1).sum() > 500
+    assert np.array_equal(g.interior_viscosity(), to)
+
+
 def test_membrane_pass_matches_oracle():
     print("[iv] membrane", flush=True)
     g, o, (rbc, plt, wbc) = _pair(seed=9)
