@@ -102,6 +102,8 @@ SIGNATURES = {
     "hc_set_boundary_repulsion": (_ST, [_P, ctypes.c_double, ctypes.c_double, _INT]),
     "hc_stage_boundary_repulsion": (_ST, [_P]),
     "hc_set_repulsion_force": (_ST, [_P, _D]),
+    "hc_set_lees_edwards": (_ST, [_P, ctypes.c_double, ctypes.c_double]),
+    "hc_stage_lees_edwards": (_ST, [_P]),
     "hc_set_interior_viscosity": (_ST, [_P, _INT, ctypes.c_double]),
     "hc_set_interior_viscosity_timescales": (_ST, [_P, _INT, _INT]),
     "hc_get_interior_viscosity": (_ST, [_P, _D]),
@@ -268,6 +270,10 @@ class Simulation:
         fr = np.zeros((self.num_vertices, 3))
         self._chk(self.lib.hc_get_boundary_repulsion_force(self.h, _dp(fr)))
         return fr
+
+    def set_lees_edwards(self, shear_rate, displacement_per_step):
+        """Lees-Edwards planes z = 0 and z = nz-1 driven at -+(nz-1) shear_rate / 2."""
+        self._chk(self.lib.hc_set_lees_edwards(self.h, float(shear_rate), float(displacement_per_step)))
 
     def set_interior_viscosity(self, type_id, tau_inner):
         """Fluid nodes inside the membranes of type_id relax with tau_inner (<= 0: off)."""

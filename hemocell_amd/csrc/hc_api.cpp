@@ -115,6 +115,11 @@ struct hc_ctx {
   double brep_k = 0, brep_cutoff = 0;
   int brep_timescale = 1;
   uint8_t* d_bflag = nullptr;
+  // Lees-Edwards planes z = 0 and z = nz-1: plane velocities, displacement per
+  // step and the planes' scratch populations
+  bool le = false;
+  double le_top_u = 0, le_bottom_u = 0, le_disp = 0;
+  void* d_le = nullptr;
   // interior viscosity: tau inside the membranes of each type (0 = off) and the
   // two refresh timescales; node classes, last-writer stamps of the membrane
   // pass (zero between passes) and the positions before the advance
@@ -546,6 +551,28 @@ hc_status do_collide(hc_ctx* ctx) {
   return HC_OK;
 }
 
+// Lees-Edwards planes after the collide-stream, with the displacement of this
+// iteration, fmod(displacement per step x iteration, nx) (leesEdwardsBC.h:360-363)
+template <typename R>
+hc_status do_lees_edwards(hc_ctx* ctx) {
+  hc_status s;
+  if (!ctx->d_le &&
+      (s = dalloc(ctx, &ctx->d_le, sizeof(double) * 19 * 2 * (size_t)ctx->L.nx * ctx->L.ny, "Lees-Edwards planes")) !=
+          HC_OK)
+    return s;
+  CollideArgs a{};
+  a.omega = 1.0 / ctx->tau;
+  visc_args(ctx, a);
+  const double d = std::fmod(ctx->le_disp * (double)ctx->iter, (double)ctx->L.nx);
+  hipEvent_t ev;
+  prof_begin(ctx, HC_PROF_COLLIDE, &ev);
+  HC_TRY(launch_lees_edwards<R>(ctx->stream, (R*)ctx->f, ctx->type, ctx->L, ctx->phase, a, d, ctx->le_top_u,
+                                ctx->le_bottom_u, (R*)ctx->d_le),
+         "Lees-Edwards");
+  prof_end(ctx, HC_PROF_COLLIDE, ev);
+  return HC_OK;
+}
+
 template <typename R>
 hc_status do_interp_advance(hc_ctx* ctx, int do_interp, int do_advance) {
   if (ctx->nv == 0) return HC_OK;
@@ -606,6 +633,7 @@ hc_status one_step(hc_ctx* ctx) {
     return s;
   if (ctx->nv > 0 && !ctx->spread_ready && (s = do_spread<R>(ctx)) != HC_OK) return s;
   if ((s = do_collide<R>(ctx)) != HC_OK) return s;
+  if (ctx->le && (s = do_lees_edwards<R>(ctx)) != HC_OK) return s;
   const int do_interp = ctx->iter % ctx->pv_timescale == 0;
   if (iv && (s = ensure_visc(ctx)) != HC_OK) return s;
   if (memb)
@@ -690,7 +718,7 @@ std::vector<unsigned char> launch_signature(const hc_ctx* c) {
 
 // Two steps repeat exactly when the step is fused and every timescale is 1.
 bool graph_eligible(const hc_ctx* c) {
-  if (!c->use_graphs || c->prof.on || !c->fuse || c->repulsion || c->brep || iv_active(c) || c->nv == 0 ||
+  if (!c->use_graphs || c->prof.on || !c->fuse || c->repulsion || c->brep || iv_active(c) || c->le || c->nv == 0 ||
       !c->spread_ready)
     return false;
   if (c->pv_timescale != 1) return false;
@@ -972,7 +1000,7 @@ void hc_destroy(hc_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->d_extra_dense, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
                 ctx->fbr, ctx->d_bflag, ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag,
-                ctx->d_visc, ctx->d_stamp, ctx->d_xold};
+                ctx->d_visc, ctx->d_stamp, ctx->d_xold, ctx->d_le};
   for (void* p : ps)
     if (p) hipFree(p);
   for (auto& tp : ctx->types) {
@@ -1251,6 +1279,28 @@ hc_status hc_set_boundary_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu,
   return HC_OK;
 }
 
+hc_status hc_set_lees_edwards(hc_ctx* ctx, double shear_rate, double displacement_per_step) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!std::isfinite(shear_rate) || !std::isfinite(displacement_per_step))
+    return fail(ctx, HC_INVALID_ARG, "Lees-Edwards needs a finite shear rate and displacement");
+  if (!ctx->L.periodic[0] || !ctx->L.periodic[2] || ctx->L.nz < 2)
+    return fail(ctx, HC_INVALID_ARG, "Lees-Edwards planes need a lattice periodic in x and z (nz >= 2)");
+  const double vhalf = (ctx->L.nz - 1) * shear_rate * 0.5;  // leesEdwardsBC.h:322-324
+  ctx->le = true;
+  ctx->le_top_u = -vhalf;
+  ctx->le_bottom_u = vhalf;
+  ctx->le_disp = displacement_per_step;
+  return HC_OK;
+}
+
+hc_status hc_stage_lees_edwards(hc_ctx* ctx) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!ctx->le) return fail(ctx, HC_STATE, "Lees-Edwards planes not set (hc_set_lees_edwards)");
+  return fp32(ctx) ? do_lees_edwards<float>(ctx) : do_lees_edwards<double>(ctx);
+}
+
 hc_status hc_set_interior_viscosity(hc_ctx* ctx, int type_id, double tau_inner) {
   if (!ctx) return HC_INVALID_ARG;
   if (type_id < 0 || type_id >= (int)ctx->types.size()) return fail(ctx, HC_INVALID_ARG, "bad type id");
@@ -1408,7 +1458,13 @@ hc_status hc_stage_collide_stream(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);
   if (s != HC_OK) return s;
   if ((s = invalidate_spread(ctx)) != HC_OK) return s;
-  return fp32(ctx) ? do_collide<float>(ctx) : do_collide<double>(ctx);
+  // the lattice's data processors (the Lees-Edwards planes) run with it
+  if (fp32(ctx)) {
+    if ((s = do_collide<float>(ctx)) != HC_OK) return s;
+    return ctx->le ? do_lees_edwards<float>(ctx) : HC_OK;
+  }
+  if ((s = do_collide<double>(ctx)) != HC_OK) return s;
+  return ctx->le ? do_lees_edwards<double>(ctx) : HC_OK;
 }
 hc_status hc_stage_interpolate(hc_ctx* ctx) {
   hc_status s = ready_for_step(ctx);

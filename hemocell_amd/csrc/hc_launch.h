@@ -93,6 +93,12 @@ hipError_t launch_boundary_repulsion(hipStream_t s, const VertexDev& V, const La
                                      double k, double r_c);
 hipError_t launch_vertex_bins(hipStream_t s, const VertexDev& V, const LatticeDev& L, int64_t* out);
 
+// Lees-Edwards planes after the collide-stream (hc_lbm.hip): d is the current
+// displacement; scratch holds 2 nx ny x 19 populations
+template <typename R>
+hipError_t launch_lees_edwards(hipStream_t s, R* f, const uint8_t* type, const LatticeDev& L, int phase,
+                               const CollideArgs& a, double d, double u_top, double u_bottom, R* scratch);
+
 // interior viscosity (hc_visc.hip)
 hipError_t launch_visc_clear(hipStream_t s, uint8_t* visc, int64_t n);
 hipError_t launch_visc_grid(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,

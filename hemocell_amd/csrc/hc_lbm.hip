@@ -406,6 +406,100 @@ __global__ void k_to_double(const R* in, double* out, int64_t n) {
   for (; i < n; i += step) out[i] = (double)in[i];
 }
 
+// Slot that holds the post-stream population (n, i) in the given phase of the
+// AA pattern (the one post_stream reads); -1 outside a non-periodic domain.
+__device__ __forceinline__ int64_t post_stream_slot(const LatticeDev& L, int phase, int x, int y, int z, int i,
+                                                    int64_t n) {
+  if (phase == 0) return i * L.stride + n;
+  const int X = shift_coord(x, -cx(i), L.nx, L.periodic[0]);
+  const int Y = shift_coord(y, -cy(i), L.ny, L.periodic[1]);
+  const int Z = shift_coord(z, -cz(i), L.nz, L.periodic[2]);
+  if (X < 0 || Y < 0 || Z < 0) return -1;
+  return opp(i) * L.stride + (int64_t)Z + (int64_t)L.nz * ((int64_t)Y + (int64_t)L.ny * X);
+}
+
+__device__ __forceinline__ int64_t mod_nonneg(int64_t a, int64_t b) { return (a % b + b) % b; }
+
+// Lees-Edwards planes, first half (helper/leesEdwardsBC.h:101-215): for every
+// node of the top (z = nz-1, side 0) and bottom (z = 0, side 1) plane, a copy
+// of its post-stream populations relaxed once more by BGK, with the node's own
+// omega, towards the equilibrium of its rhoBar and the momentum (u_side, 0, 0);
+// the five populations that entered across the plane are replaced by the
+// linear interpolation, at x + d (top) or x - d (bottom), of the plane's
+// post-stream populations, the diagonal pair exchanged.  Wall nodes are
+// copied unchanged.  out: 2 nx ny x 19, written back by k_lees_edwards_set.
+template <typename R>
+__global__ void k_lees_edwards_get(const R* __restrict__ f, const uint8_t* __restrict__ type, LatticeDev L,
+                                   int phase, CollideArgs a, double d, double g, double u_top, double u_bottom,
+                                   R* __restrict__ out) {
+  const int64_t P = (int64_t)L.nx * L.ny;
+  const int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= 2 * P) return;
+  const int side = (int)(k / P);
+  const int X = (int)((k % P) / L.ny), Y = (int)((k % P) % L.ny), Z = side == 0 ? L.nz - 1 : 0;
+  const int64_t n = (int64_t)Z + (int64_t)L.nz * ((int64_t)Y + (int64_t)L.ny * X);
+  R fl[19];
+#pragma unroll
+  for (int i = 0; i < 19; ++i) fl[i] = post_stream(f, L, phase, X, Y, Z, i, n);
+  R* o = out + 19 * k;
+  if (type && type[n] != HC_NODE_FLUID) {
+#pragma unroll
+    for (int i = 0; i < 19; ++i) o[i] = fl[i];
+    return;
+  }
+  R rhoBar, j0, j1, j2;
+  rhobar_j(fl, rhoBar, j0, j1, j2);
+  const R uw = R(side == 0 ? u_top : u_bottom);
+  const R jSqr = uw * uw;
+  const R invRho = R(1) / (rhoBar + R(1));
+  R om = R(a.omega);
+  if (a.visc && a.visc[n]) om = R(a.omega_cls[a.visc[n] & 15]);
+#pragma unroll
+  for (int i = 0; i < 19; ++i) o[i] = fl[i] * (R(1) - om) + om * feq<R>(i, rhoBar, invRho, uw, R(0), R(0), jSqr);
+  int64_t s1, s2;
+  if (side == 0) {
+    s1 = mod_nonneg((int64_t)ceil(d + X), L.nx);
+    s2 = mod_nonneg((int64_t)floor(d + X), L.nx);
+  } else {
+    s1 = mod_nonneg((int64_t)floor(-d + X), L.nx);
+    s2 = mod_nonneg((int64_t)ceil(-d + X), L.nx);
+  }
+  const int64_t n1 = (int64_t)Z + (int64_t)L.nz * ((int64_t)Y + (int64_t)L.ny * s1);
+  const int64_t n2 = (int64_t)Z + (int64_t)L.nz * ((int64_t)Y + (int64_t)L.ny * s2);
+  const R gr = R(g), hr = R(1) - R(g);
+  auto lerp = [&](int i) {
+    return gr * post_stream(f, L, phase, (int)s1, Y, Z, i, n1) + hr * post_stream(f, L, phase, (int)s2, Y, Z, i, n2);
+  };
+  if (side == 0) {
+    o[3] = lerp(3);
+    o[6] = lerp(16);
+    o[8] = lerp(8);
+    o[16] = lerp(6);
+    o[18] = lerp(18);
+  } else {
+    o[7] = lerp(15);
+    o[9] = lerp(9);
+    o[12] = lerp(12);
+    o[15] = lerp(7);
+    o[17] = lerp(17);
+  }
+}
+
+// second half (helper/leesEdwardsBC.h:275-298): the copies back into the planes
+template <typename R>
+__global__ void k_lees_edwards_set(R* __restrict__ f, LatticeDev L, int phase, const R* __restrict__ out) {
+  const int64_t P = (int64_t)L.nx * L.ny;
+  const int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= 2 * P) return;
+  const int side = (int)(k / P);
+  const int X = (int)((k % P) / L.ny), Y = (int)((k % P) % L.ny), Z = side == 0 ? L.nz - 1 : 0;
+  const int64_t n = (int64_t)Z + (int64_t)L.nz * ((int64_t)Y + (int64_t)L.ny * X);
+  for (int i = 0; i < 19; ++i) {
+    const int64_t slot = post_stream_slot(L, phase, X, Y, Z, i, n);
+    if (slot >= 0) f[slot] = out[19 * k + i];
+  }
+}
+
 inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 inline dim3 grid_capped(int64_t n) {
   int64_t b = (n + BLOCK - 1) / BLOCK;
@@ -498,6 +592,17 @@ hipError_t launch_to_double(hipStream_t s, const R* in, double* out, int64_t n) 
   return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_lees_edwards(hipStream_t s, R* f, const uint8_t* type, const LatticeDev& L, int phase,
+                               const CollideArgs& a, double d, double u_top, double u_bottom, R* scratch) {
+  const int64_t n = 2 * (int64_t)L.nx * L.ny;
+  const double g = std::fmod(d, 1.0);
+  hipLaunchKernelGGL((k_lees_edwards_get<R>), grid_for(n), dim3(BLOCK), 0, s, f, type, L, phase, a, d, g, u_top,
+                     u_bottom, scratch);
+  hipLaunchKernelGGL((k_lees_edwards_set<R>), grid_for(n), dim3(BLOCK), 0, s, f, L, phase, scratch);
+  return hipGetLastError();
+}
+
 #define HC_INSTANTIATE(R)                                                                                      \
   template hipError_t launch_collide_stream<R>(hipStream_t, R*, const R*, R*, const uint8_t*, const LatticeDev&, \
                                                const CollideArgs&, int, bool, const uint32_t*, int, bool);    \
@@ -510,7 +615,9 @@ hipError_t launch_to_double(hipStream_t s, const R* in, double* out, int64_t n) 
   template hipError_t launch_fluid_stats<R>(hipStream_t, const R*, const R*, const uint8_t*, const LatticeDev&,    \
                                             int, const CollideArgs&, const R*, int, double*, int);             \
   template hipError_t launch_fill_zero<R>(hipStream_t, R*, int64_t);                                            \
-  template hipError_t launch_to_double<R>(hipStream_t, const R*, double*, int64_t);
+  template hipError_t launch_to_double<R>(hipStream_t, const R*, double*, int64_t);                             \
+  template hipError_t launch_lees_edwards<R>(hipStream_t, R*, const uint8_t*, const LatticeDev&, int,             \
+                                             const CollideArgs&, double, double, double, R*);
 HC_INSTANTIATE(float)
 HC_INSTANTIATE(double)
 #undef HC_INSTANTIATE

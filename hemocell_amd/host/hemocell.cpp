@@ -589,7 +589,37 @@ void HemoCell::bindLattice() {
     check(hc_set_boundary_repulsion(ctx, boundaryRepulsionConstant, boundaryRepulsionCutoff,
                                     (int)boundaryRepulsionTimescale),
           "boundary repulsion");
+  if (L.leesEdwards) {
+    check(hc_set_lees_edwards(ctx, L.leesEdwardsShear, L.leesEdwardsDisplacement), "Lees-Edwards");
+    leesEdwardsBC = true;
+  }
   bound = true;
+}
+
+// helper/leesEdwardsBC.h:301-364
+double LeesEdwardsBC::LEcurrentDisplacement = 0.0;
+
+LeesEdwardsBC::LeesEdwardsBC(Lattice& l, T shearRate, T dt_, double** hemoCellLEcurrentDisplacement,
+                             plint dataProcessorLevel_)
+    : lattice(l), nx(l.getNx()), ny(l.getNy()), nz(l.getNz()), LEdisplacement(shearRate * dt_), dt(dt_),
+      dataProcessorLevel(dataProcessorLevel_) {
+  const T vHalf = (nz - 1) * shearRate * 0.5;
+  topVelocity = -vHalf;
+  bottomVelocity = vHalf;
+  if (hemoCellLEcurrentDisplacement) *hemoCellLEcurrentDisplacement = &LEcurrentDisplacement;
+  lattice.leesEdwardsShear = shearRate;
+  lattice.leesEdwardsDisplacement = LEdisplacement;
+}
+
+void LeesEdwardsBC::initialize() {
+  lattice.periodicity().toggleAll(true);
+  lattice.leesEdwards = true;
+  if (lattice.owner && lattice.owner->context() && hc_num_nodes(lattice.owner->context()) > 0)
+    throw HemoCellError("LeesEdwardsBC::initialize: the lattice is already bound; initialize it before first use");
+}
+
+void LeesEdwardsBC::updateLECurDisplacement(unsigned int iter) {
+  LEcurrentDisplacement = std::fmod(LEdisplacement * iter, (double)nx);
 }
 
 // core/hemoCell.cpp:129-133

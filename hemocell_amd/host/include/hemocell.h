@@ -212,10 +212,35 @@ class Lattice {
   T wall_u[6][3] = {{0}};
   std::vector<uint8_t> mask;
   HemoCell* owner = nullptr;  // set when a HemoCell binds the lattice
+  // Lees-Edwards planes registered by LeesEdwardsBC::initialize()
+  bool leesEdwards = false;
+  T leesEdwardsShear = 0;
+  double leesEdwardsDisplacement = 0;  // per step
 
  private:
   plint nx, ny, nz;
   Periodicity period;
+};
+
+// Lees-Edwards planes z = 0 and z = nz-1 (helper/leesEdwardsBC.h:301-364): the
+// lattice becomes periodic on every axis; after each collide-stream the planes
+// are driven at -+ (nz-1) shearRate / 2 and the populations crossing them are
+// taken from the plane displaced by fmod(shearRate dt iter, nx) along x (the
+// reference's rule: dt as passed, in seconds).  Construct and initialize() it
+// before the lattice is first used (addCellType, latticeEquilibrium).
+class LeesEdwardsBC {
+ public:
+  LeesEdwardsBC(Lattice& lattice, T shearRate, T dt, double** hemoCellLEcurrentDisplacement,
+                plint dataProcessorLevel = 1);
+  void initialize();
+  // the host copy of the current displacement (the device derives its own from the iteration)
+  void updateLECurDisplacement(unsigned int iter);
+  Lattice& lattice;
+  plint nx, ny, nz;
+  double LEdisplacement;  // per step
+  static double LEcurrentDisplacement;
+  T dt, topVelocity, bottomVelocity;
+  plint dataProcessorLevel;
 };
 
 // Full-way bounce-back on every node of the box (defineDynamics(..., new BounceBack),
@@ -358,6 +383,9 @@ class HemoCell {
   void check(hc_status s, const char* what) const;
 
   bool outputInSiUnits = true;
+  // Lees-Edwards planes on (hemocell.h:146-147 of the reference)
+  bool leesEdwardsBC = false;
+  double* LEcurrentDisplacement = nullptr;
   bool repulsionEnabled = false;
   bool boundaryRepulsionEnabled = false;
   Lattice* lattice = nullptr;

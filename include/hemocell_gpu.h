@@ -145,6 +145,14 @@ hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int time
  * cut-off (lattice units) with k (r_c / r) r_hat, every `timescale` steps, before the
  * spread; the force is recomputed at each application and kept between. */
 hc_status hc_set_boundary_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale);
+/* Replaces: LeesEdwardsBC (helper/leesEdwardsBC.h:301-364, as set up in
+ * cases/leesEdwards/leesEdwards.cpp:43-44,73): after every collide-stream the nodes of the
+ * planes z = nz-1 and z = 0 relax once more towards the equilibrium of the momentum
+ * (-+ (nz-1) shear_rate / 2, 0, 0), and their five populations that entered across the
+ * plane are interpolated from the plane at x +- d, d = fmod(displacement_per_step x
+ * iteration, nx).  The reference passes shear_rate x dt (seconds) as the displacement per
+ * step.  The lattice must be periodic in x and z. */
+hc_status hc_set_lees_edwards(hc_ctx* ctx, double shear_rate, double displacement_per_step);
 /* Replaces: the interior viscosity of a cell type, HemoCellField (core/hemoCellField.cpp:
  * 99-110, INTERIOR_VISCOSITY build): bulk fluid nodes inside the membranes of type_id relax
  * with tau_inner (the reference's viscosityRatio (tau - 1/2) + 1/2) instead of tau.
@@ -192,7 +200,8 @@ hc_status hc_set_iteration(hc_ctx* ctx, int64_t iter);
 
 /* Single stages of iterate(), in its order (parity tests and partial drivers). */
 hc_status hc_stage_spread(hc_ctx* ctx);          /* spreadParticleForce  hemoCellParticleField.cpp:841-863 */
-hc_status hc_stage_collide_stream(hc_ctx* ctx);  /* lattice->collideAndStream()  hemoCell.cpp:317 */
+hc_status hc_stage_collide_stream(hc_ctx* ctx);  /* lattice->collideAndStream()  hemoCell.cpp:317, with the
+                                                    Lees-Edwards planes when set (its data processors) */
 hc_status hc_stage_interpolate(hc_ctx* ctx);     /* interpolateFluidVelocity  hemoCellParticleField.cpp:819-839 */
 hc_status hc_stage_advance(hc_ctx* ctx);         /* advanceParticles  hemoCellParticleField.cpp:566-588 */
 hc_status hc_stage_mechanics(hc_ctx* ctx, int forced); /* applyConstitutiveModel  :633-675 */
@@ -201,6 +210,7 @@ hc_status hc_stage_boundary_repulsion(hc_ctx* ctx); /* applyBoundaryRepulsionFor
 /* pass 0: findInternalParticleGridPoints :776-808; pass 1: internalGridPointsMembrane
  * :745-774 with the support of the current positions */
 hc_status hc_stage_interior_viscosity(hc_ctx* ctx, int pass);
+hc_status hc_stage_lees_edwards(hc_ctx* ctx); /* LeesEdwardsBCGet/SetPopulations  leesEdwardsBC.h:101-298 */
 hc_status hc_stage_end(hc_ctx* ctx);             /* setExternalVector reset + iter++  hemoCell.cpp:369-374 */
 
 /* ---- state access (synchronous) ------------------------------------------ */

@@ -430,6 +430,10 @@ struct Oracle {
   int brep_timescale = 1;
   std::vector<long> bparticles;  // node indices, ascending x, y, z
   std::vector<double> fbr;       // 3*NV
+  // Lees-Edwards planes (z = 0 and z = nz-1): plane velocities and the
+  // displacement per step
+  bool le = false;
+  double le_top_u = 0, le_bottom_u = 0, le_disp = 0;
   // interior viscosity: tau inside the membranes per type (0 = off), node classes
   std::vector<double> iv_tau;
   std::vector<uint8_t> visc;     // N: 0 background, 1 + type inside
@@ -1032,6 +1036,78 @@ struct Oracle {
     return m;
   }
 
+  // Lees-Edwards planes after the collide-and-stream (helper/leesEdwardsBC.h:
+  // 101-215, then :275-298).  Each node of the top (z = nz-1) and bottom (z = 0)
+  // plane keeps a copy of its populations relaxed once more, by BGK with its
+  // own omega, towards the equilibrium of its rhoBar and the momentum
+  // (u_plane, 0, 0); the five populations that entered across the plane are
+  // then replaced by the linear interpolation, at the x displaced by the
+  // current displacement d, of the plane's un-relaxed populations (the two
+  // diagonal ones exchanged); all copies are written back at the end.
+  void lees_edwards() {
+    const double d = std::fmod(le_disp * (double)iter, (double)nx);
+    const double g = std::fmod(d, 1.0);
+    auto mod = [](long a, long b) { return (a % b + b) % b; };
+    std::vector<double> out((size_t)2 * nx * ny * 19);
+    for (int side = 0; side < 2; ++side) {  // 0 top, 1 bottom
+      const int Z = side == 0 ? nz - 1 : 0;
+      const double uw = side == 0 ? le_top_u : le_bottom_u;
+      for (int X = 0; X < nx; ++X)
+        for (int Y = 0; Y < ny; ++Y) {
+          const long n = idx(X, Y, Z);
+          double* o = &out[(((size_t)side * nx + X) * ny + Y) * 19];
+          double fl[19];
+          for (int i = 0; i < 19; ++i) fl[i] = f[(long)i * N + n];
+          if (type[n] != FLUID) {  // wall nodes keep their own dynamics
+            for (int i = 0; i < 19; ++i) o[i] = fl[i];
+            continue;
+          }
+          double rhoBar, j[3];
+          rhobar_j(fl, rhoBar, j);
+          const double je[3] = {uw, 0.0, 0.0};
+          const double jSqr = je[0] * je[0] + je[1] * je[1] + je[2] * je[2];
+          const double invRho = 1.0 / (rhoBar + 1.0);
+          const double ti = (!visc.empty() && visc[n]) ? ivt(visc[n] - 1) : 0.0;
+          const double om = ti > 0.0 ? 1.0 / ti : omega;
+          for (int i = 0; i < 19; ++i) {
+            o[i] = fl[i] * (1.0 - om);
+            o[i] += om * feq(i, rhoBar, invRho, je, jSqr);
+          }
+          long s1, s2;
+          if (side == 0) {
+            s1 = mod((long)std::ceil(d + X), nx);
+            s2 = mod((long)std::floor(d + X), nx);
+          } else {
+            s1 = mod((long)std::floor(-d + X), nx);
+            s2 = mod((long)std::ceil(-d + X), nx);
+          }
+          auto P = [&](long sx, int i) { return f[(long)i * N + idx((int)sx, Y, Z)]; };
+          auto lerp = [&](int i) { return g * P(s1, i) + (1 - g) * P(s2, i); };
+          if (side == 0) {
+            o[3] = lerp(3);
+            o[6] = lerp(16);
+            o[8] = lerp(8);
+            o[16] = lerp(6);
+            o[18] = lerp(18);
+          } else {
+            o[7] = lerp(15);
+            o[9] = lerp(9);
+            o[12] = lerp(12);
+            o[15] = lerp(7);
+            o[17] = lerp(17);
+          }
+        }
+    }
+    for (int side = 0; side < 2; ++side) {
+      const int Z = side == 0 ? nz - 1 : 0;
+      for (int X = 0; X < nx; ++X)
+        for (int Y = 0; Y < ny; ++Y) {
+          const double* o = &out[(((size_t)side * nx + X) * ny + Y) * 19];
+          for (int i = 0; i < 19; ++i) f[(long)i * N + idx(X, Y, Z)] = o[i];
+        }
+    }
+  }
+
   void reset_force_field() { std::fill(Fibm.begin(), Fibm.end(), 0.0); }
 
   void step() {
@@ -1039,6 +1115,7 @@ struct Oracle {
     if (brep && iter % brep_timescale == 0) boundary_repulsion_force();
     spread();
     collide_and_stream();
+    if (le) lees_edwards();
     if (iter % pv_timescale == 0) interpolate();
     const bool iv = iv_on();
     std::vector<double> xold;
@@ -1259,6 +1336,15 @@ void ho_stage_interior_viscosity(void* h, int pass) {
   if (pass == 0) o->iv_entire_grid();
   else o->iv_membrane(o->x);
 }
+void ho_set_lees_edwards(void* h, double shear_rate, double displacement_per_step) {
+  Oracle* o = (Oracle*)h;
+  o->le = true;
+  const double vhalf = (o->nz - 1) * shear_rate * 0.5;
+  o->le_top_u = -vhalf;
+  o->le_bottom_u = vhalf;
+  o->le_disp = displacement_per_step;
+}
+void ho_stage_lees_edwards(void* h) { ((Oracle*)h)->lees_edwards(); }
 int ho_ray_crosses(const double* v0, const double* v1, const double* v2, const long* s) {
   return Oracle::ray_crosses(v0, v1, v2, s);
 }
@@ -1268,7 +1354,11 @@ void ho_get_repulsion(void* h, double* fr) { Oracle* o = (Oracle*)h; std::copy(o
 // Individual stages, in iterate() order, for stage-level parity tests.
 void ho_stage_repulsion(void* h) { ((Oracle*)h)->repulsion_force(); }
 void ho_stage_spread(void* h) { ((Oracle*)h)->spread(); }
-void ho_stage_collide_stream(void* h) { ((Oracle*)h)->collide_and_stream(); }
+void ho_stage_collide_stream(void* h) {
+  Oracle* o = (Oracle*)h;
+  o->collide_and_stream();
+  if (o->le) o->lees_edwards();
+}
 void ho_stage_interpolate(void* h) { ((Oracle*)h)->interpolate(); }
 void ho_stage_advance(void* h) { ((Oracle*)h)->advance(); }
 void ho_stage_mechanics(void* h, int forced) { ((Oracle*)h)->mechanics(forced != 0); }

@@ -53,6 +53,8 @@ _SIG = {
     "ho_set_interior_viscosity_classes": (None, [_P, ctypes.POINTER(ctypes.c_uint8)]),
     "ho_stage_interior_viscosity": (None, [_P, _INT]),
     "ho_ray_crosses": (_INT, [_D, _D, _D, _L]),
+    "ho_set_lees_edwards": (None, [_P, ctypes.c_double, ctypes.c_double]),
+    "ho_stage_lees_edwards": (None, [_P]),
     "ho_vertex_bins": (None, [_P, ctypes.POINTER(ctypes.c_long)]),
     "ho_stage_spread": (None, [_P]),
     "ho_stage_collide_stream": (None, [_P]),
@@ -141,6 +143,12 @@ class OracleSim:
 
     def set_iteration(self, it):
         self.lib.ho_set_iter(self.h, int(it))
+
+    def set_lees_edwards(self, shear_rate, displacement_per_step):
+        self.lib.ho_set_lees_edwards(self.h, shear_rate, displacement_per_step)
+
+    def stage_lees_edwards(self):
+        self.lib.ho_stage_lees_edwards(self.h)
 
     def set_interior_viscosity(self, type_id, tau_inner):
         self.lib.ho_set_interior_viscosity(self.h, type_id, tau_inner)
