@@ -145,18 +145,20 @@ def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension", au
 def serve_stream(n, rate):
     """p50/p99 latency of a seeded Poisson stream of single-cell cases (shear 10-1000 1/s
     or stretch 0-200 pN on one RBC, PLT or WBC; cubic edge 32-96; 2000-20000 steps;
-    FP32-shifted), each case in its own context and stream."""
+    FP32-shifted), each case in its own context and stream, the active cases stepped
+    together in chunks (hc_step_batch)."""
     import hemocell_amd as hc
     from hemocell_amd import serving
     units = hc.lbm_base_parameters(5e-7, 1e-7)
     stream = serving.make_cases(n, rate, seed=7)
     print("[bench] serving %d cases at %.1f/s" % (n, rate), file=sys.stderr, flush=True)
-    done, wall = serving.serve(stream, units, max_active=16, chunk=1000,
+    done, wall = serving.serve(stream, units, max_active=32, chunk=250,
                                progress=lambda c, k: print("[bench] case %d done (%d/%d) %.2fs" % (
                                    c.ident, k, n, c.latency), file=sys.stderr, flush=True) if k % 8 == 0 else None)
     st = serving.latency_stats(done, wall)
-    st.update(arrival_rate=rate, max_active=16, workload="Poisson seed 7, shear or stretch, one RBC/PLT/WBC, "
-                                                        "edge 32-96, 2000-20000 steps, FP32-shifted")
+    st.update(arrival_rate=rate, max_active=32, chunk=250, batched=True,
+              workload="Poisson seed 7, shear or stretch, one RBC/PLT/WBC, edge 32-96, 2000-20000 steps, "
+                       "FP32-shifted")
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
 
 

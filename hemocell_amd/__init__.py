@@ -102,6 +102,7 @@ SIGNATURES = {
     "hc_set_boundary_repulsion": (_ST, [_P, ctypes.c_double, ctypes.c_double, _INT]),
     "hc_stage_boundary_repulsion": (_ST, [_P]),
     "hc_set_repulsion_force": (_ST, [_P, _D]),
+    "hc_step_batch": (_ST, [ctypes.POINTER(_P), _INT, _INT]),
     "hc_set_lees_edwards": (_ST, [_P, ctypes.c_double, ctypes.c_double]),
     "hc_stage_lees_edwards": (_ST, [_P]),
     "hc_set_interior_viscosity": (_ST, [_P, _INT, ctypes.c_double]),
@@ -176,6 +177,20 @@ def build_mesh(shape, radius_lu, min_triangles, aspect_ratio=0.3):
     lib.hc_build_mesh(shape, radius_lu, min_triangles, aspect_ratio, ctypes.byref(nv), ctypes.byref(nt),
                       _dp(v), t.ctypes.data_as(_I32))
     return v, t
+
+
+def step_batch(sims, n=1):
+    """n steps of every simulation, stepped together (hc_step_batch): a few kernel
+    launches per step for all of them; asynchronous like Simulation.step."""
+    sims = list(sims)
+    if not sims:
+        return
+    lib = sims[0].lib
+    arr = (_P * len(sims))(*[s.h for s in sims])
+    st = lib.hc_step_batch(arr, len(sims), int(n))
+    if st:
+        msgs = [s.lib.hc_last_error(s.h).decode() for s in sims]
+        raise HcError(st, next((m for m in msgs if m), "hc_step_batch failed"))
 
 
 class Simulation:

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <functional>
+#include <map>
+#include <tuple>
 #include <sstream>
 #include <cmath>
 #include <cstdio>
@@ -120,6 +122,15 @@ struct hc_ctx {
   bool le = false;
   double le_top_u = 0, le_bottom_u = 0, le_disp = 0;
   void* d_le = nullptr;
+  // batched steps led by this context: two descriptor buffers (device and
+  // pinned host staging) used in turn, each with the event of the last batch
+  // that read it, so the host prepares one batch while the previous one runs
+  void* d_batch[2] = {nullptr, nullptr};
+  void* h_batch[2] = {nullptr, nullptr};
+  size_t batch_cap[2] = {0, 0};
+  hipEvent_t batch_slot_ev[2] = {nullptr, nullptr};
+  int batch_slot = 0;
+  hipEvent_t batch_ev = nullptr;  // this context's queued work / the batch's end
   // interior viscosity: tau inside the membranes of each type (0 = off) and the
   // two refresh timescales; node classes, last-writer stamps of the membrane
   // pass (zero between passes) and the positions before the advance
@@ -796,6 +807,200 @@ hc_status step_impl(hc_ctx* ctx, int nsteps) {
   return HC_OK;
 }
 
+// ---- batched steps of several contexts ---------------------------------------
+// A context can join a batch when its step is the plain fused one: no
+// repulsion, interior viscosity or Lees-Edwards planes, every timescale 1,
+// velocity nodes (if any) inside the collide kernel, no profiling.
+bool batchable(const hc_ctx* c) {
+  if (!c->fuse || c->repulsion || c->brep || iv_active(c) || c->le || c->nv == 0 || c->prof.on) return false;
+  if (c->pv_timescale != 1) return false;
+  if (c->general && c->n_vel > 0 && !c->fuse_vel) return false;
+  for (const auto& tp : c->types)
+    if (tp->ncells && tp->host.timescale != 1) return false;
+  return true;
+}
+
+template <typename T>
+void append_bytes(std::vector<unsigned char>& buf, const std::vector<T>& v) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(v.data());
+  buf.insert(buf.end(), p, p + v.size() * sizeof(T));
+}
+
+// Steps the contexts together: every step launches the collide-stream of all
+// cases of one AA parity and boundary kind at once, then their velocity
+// fields, then the cells of one mesh at once (each cell spreading into its own
+// case's next force buffer).  The launches go to the first context's stream,
+// which waits for every context's queued work; every context's stream then
+// waits for the batch.  Each case computes exactly what its own fused step
+// computes.
+template <typename R>
+hc_status batch_impl(hc_ctx* const* cs, int n, int nsteps) {
+  hc_ctx* lead = cs[0];
+  hc_ctx* ctx = lead;  // for HC_TRY
+  for (int i = 0; i < n; ++i) {
+    hc_ctx* c = cs[i];
+    if (!c->batch_ev) HC_TRY(hipEventCreateWithFlags(&c->batch_ev, hipEventDisableTiming), "hipEventCreate");
+    if (!c->spread_ready) {
+      hc_status s = do_spread<R>(c);
+      if (s != HC_OK) return s;
+      c->spread_ready = true;
+    }
+  }
+  // descriptors for the two parities of the step counter
+  struct Group {
+    int kind;  // 0 collide, 1 velocity field, 2 cells
+    int a, b, c;
+    size_t offset;
+    int count;
+    uint32_t blocks;
+    TypeDev T;
+  };
+  std::vector<Group> groups[2];
+  std::vector<unsigned char> bytes;
+  for (int p = 0; p < 2; ++p) {
+    // collide: key (odd, general, vel)
+    std::map<std::tuple<int, int, int>, std::vector<LatCase>> lat;
+    std::map<std::tuple<int, int>, std::vector<VelCase>> vel;
+    std::map<std::tuple<int, int, int>, std::vector<CellCase>> cel;
+    std::map<std::tuple<int, int, int>, TypeDev> cel_t;
+    for (int i = 0; i < n; ++i) {
+      hc_ctx* c = cs[i];
+      const int phase = c->phase ^ p, cur = c->cur ^ p;
+      LatCase lc{};
+      lc.f = c->f;
+      lc.Fcur = c->Fibm[cur];
+      lc.Fclr = c->Fibm[1 - cur];
+      lc.type = c->type;
+      lc.L = c->L;
+      lc.a.omega = 1.0 / c->tau;
+      for (int k = 0; k < 3; ++k) lc.a.g[k] = c->g[k];
+      std::memcpy(lc.a.wall_u, c->wall_u, sizeof(lc.a.wall_u));
+      const int vflag = c->general && c->n_vel > 0 && c->fuse_vel;
+      lat[std::make_tuple(phase, (int)c->general, vflag)].push_back(lc);
+      VelCase vc{};
+      vc.f = c->f;
+      vc.Fcur = c->Fibm[cur];
+      vc.uf = c->ufield;
+      vc.L = c->L;
+      for (int k = 0; k < 3; ++k) vc.g[k] = c->g[k];
+      vel[std::make_tuple(phase ^ 1, (int)c->general)].push_back(vc);
+      for (size_t k = 0; k < c->cell_base.size(); ++k) {
+        const TypeBufs& tb = *c->types[(size_t)c->cell_type[k]];
+        CellCase cc{};
+        cc.V = vdev(c);
+        cc.T = tb.dev;
+        cc.base = c->cell_base[k];
+        cc.uf = c->ufield;
+        cc.type = c->type;
+        cc.L = c->L;
+        cc.Fnext = c->Fibm[1 - cur];
+        cc.extra = c->d_extra_dense;
+        cc.scratch = c->bend;
+        cc.f_limit = c->f_limit;
+        cc.flags = CELL_FLAG_SPREAD | CELL_FLAG_INTERP | CELL_FLAG_MECH;
+        cc.wall_hits = c->d_wall_hits;
+        const int visc = (tb.dev.model != HC_MODEL_RBC) || (tb.dev.eta_m != 0.0);
+        const auto key = std::make_tuple(tb.dev.model, tb.dev.nv, visc);
+        cel[key].push_back(cc);
+        cel_t[key] = tb.dev;
+      }
+    }
+    auto align = [&]() {
+      while (bytes.size() % 64) bytes.push_back(0);
+    };
+    for (auto& kv : lat) {
+      uint32_t blocks = 0;
+      for (auto& lc : kv.second) {
+        lc.block0 = blocks;
+        blocks += (uint32_t)((lc.L.N + 255) / 256);
+      }
+      align();
+      Group g{0, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), bytes.size(),
+              (int)kv.second.size(), blocks, TypeDev{}};
+      append_bytes(bytes, kv.second);
+      groups[p].push_back(g);
+    }
+    for (auto& kv : vel) {
+      uint32_t blocks = 0;
+      for (auto& vc : kv.second) {
+        vc.block0 = blocks;
+        blocks += (uint32_t)((vc.L.N + 255) / 256);
+      }
+      align();
+      Group g{1, std::get<0>(kv.first), std::get<1>(kv.first), 0, bytes.size(), (int)kv.second.size(), blocks,
+              TypeDev{}};
+      append_bytes(bytes, kv.second);
+      groups[p].push_back(g);
+    }
+    for (auto& kv : cel) {
+      align();
+      Group g{2, 0, 0, 0, bytes.size(), (int)kv.second.size(), (uint32_t)kv.second.size(), cel_t[kv.first]};
+      append_bytes(bytes, kv.second);
+      groups[p].push_back(g);
+    }
+  }
+  // this slot's buffers may still be read by the batch before the previous one
+  const int slot = lead->batch_slot;
+  lead->batch_slot ^= 1;
+  if (!lead->batch_slot_ev[slot])
+    HC_TRY(hipEventCreateWithFlags(&lead->batch_slot_ev[slot], hipEventDisableTiming), "hipEventCreate");
+  else
+    HC_TRY(hipEventSynchronize(lead->batch_slot_ev[slot]), "hipEventSynchronize");
+  if (bytes.size() > lead->batch_cap[slot]) {
+    if (lead->d_batch[slot]) hipFree(lead->d_batch[slot]);
+    if (lead->h_batch[slot]) hipHostFree(lead->h_batch[slot]);
+    lead->d_batch[slot] = lead->h_batch[slot] = nullptr;
+    lead->batch_cap[slot] = 0;
+    const size_t cap = bytes.size() * 2;
+    hc_status s = dalloc(lead, &lead->d_batch[slot], cap, "batch descriptors");
+    if (s != HC_OK) return s;
+    HC_TRY(hipHostMalloc(&lead->h_batch[slot], cap, hipHostMallocDefault), "hipHostMalloc");
+    lead->batch_cap[slot] = cap;
+  }
+  std::memcpy(lead->h_batch[slot], bytes.data(), bytes.size());
+  for (int i = 1; i < n; ++i) {
+    HC_TRY(hipEventRecord(cs[i]->batch_ev, cs[i]->stream), "hipEventRecord");
+    HC_TRY(hipStreamWaitEvent(lead->stream, cs[i]->batch_ev, 0), "hipStreamWaitEvent");
+  }
+  HC_TRY(hipMemcpyAsync(lead->d_batch[slot], lead->h_batch[slot], bytes.size(), hipMemcpyHostToDevice, lead->stream),
+         "hipMemcpyAsync");
+  const unsigned char* base = (const unsigned char*)lead->d_batch[slot];
+  const int box = lead->spread_lds > 0 ? lead->spread_lds : 12288;
+  for (int k = 0; k < nsteps; ++k) {
+    for (const Group& g : groups[k & 1]) {
+      if (g.kind == 0) {
+        HC_TRY(launch_collide_multi<R>(lead->stream, (const LatCase*)(base + g.offset), g.count, g.blocks, g.a,
+                                       g.b != 0, g.c != 0),
+               "batched collide_stream");
+      } else if (g.kind == 1) {
+        HC_TRY(launch_velocity_field_multi<R>(lead->stream, (const VelCase*)(base + g.offset), g.count, g.blocks,
+                                              g.a, g.b != 0),
+               "batched velocity_field");
+      }
+    }
+    for (const Group& g : groups[k & 1])
+      if (g.kind == 2) {
+        int block = g.T.model == HC_MODEL_WBC ? lead->wbc_block : lead->cell_block;
+        if (g.count < lead->n_cu) block = 1024;
+        HC_TRY(launch_cell_step_multi<R>(lead->stream, g.T, (const CellCase*)(base + g.offset), g.count, box,
+                                         lead->lds_device, block),
+               "batched cell_step");
+      }
+  }
+  HC_TRY(hipEventRecord(lead->batch_ev, lead->stream), "hipEventRecord");
+  HC_TRY(hipEventRecord(lead->batch_slot_ev[slot], lead->stream), "hipEventRecord");
+  for (int i = 0; i < n; ++i) {
+    hc_ctx* c = cs[i];
+    if (i > 0) HC_TRY(hipStreamWaitEvent(c->stream, lead->batch_ev, 0), "hipStreamWaitEvent");
+    c->phase ^= nsteps & 1;
+    c->cur ^= nsteps & 1;
+    c->iter += nsteps;
+    c->other_clean = false;
+    c->spread_ready = true;
+  }
+  return HC_OK;
+}
+
 // Autotuner: times real steps of this context (HIP events on its stream, steps
 // launched eagerly) for each value of one parameter at a time -- the spread
 // box, the threads per RBC cell, the side types' layout, the threads per WBC
@@ -1000,7 +1205,7 @@ void hc_destroy(hc_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->d_extra_dense, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
                 ctx->fbr, ctx->d_bflag, ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag,
-                ctx->d_visc, ctx->d_stamp, ctx->d_xold, ctx->d_le};
+                ctx->d_visc, ctx->d_stamp, ctx->d_xold, ctx->d_le, ctx->d_batch[0], ctx->d_batch[1]};
   for (void* p : ps)
     if (p) hipFree(p);
   for (auto& tp : ctx->types) {
@@ -1016,6 +1221,11 @@ void hc_destroy(hc_ctx* ctx) {
     if (ctx->join_ev[j]) hipEventDestroy(ctx->join_ev[j]);
   }
   if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
+  if (ctx->batch_ev) hipEventDestroy(ctx->batch_ev);
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->batch_slot_ev[k]) hipEventDestroy(ctx->batch_slot_ev[k]);
+    if (ctx->h_batch[k]) hipHostFree(ctx->h_batch[k]);
+  }
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1405,6 +1615,35 @@ hc_status hc_step(hc_ctx* ctx, int nsteps) {
     return fail(ctx, HC_INVALID_ARG,
                 "interior viscosity timescales must be multiples of the particle velocity timescale");
   return fp32(ctx) ? step_impl<float>(ctx, nsteps) : step_impl<double>(ctx, nsteps);
+}
+
+hc_status hc_step_batch(hc_ctx* const* ctxs, int n, int nsteps) {
+  if (!ctxs || n < 0 || nsteps < 0) return HC_INVALID_ARG;
+  if (n == 0 || nsteps == 0) return HC_OK;
+  std::vector<hc_ctx*> batch;
+  for (int i = 0; i < n; ++i) {
+    hc_ctx* c = ctxs[i];
+    if (!c) return HC_INVALID_ARG;
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == c) return fail(c, HC_INVALID_ARG, "a context appears twice in the batch");
+    hc_status s = ready_for_step(c);
+    if (s != HC_OK) return s;
+    if (c->device != ctxs[0]->device || fp32(c) != fp32(ctxs[0]))
+      return fail(c, HC_INVALID_ARG, "batched contexts need one device and one precision");
+  }
+  for (int i = 0; i < n; ++i) {
+    hc_ctx* c = ctxs[i];
+    if (batchable(c)) {
+      batch.push_back(c);
+    } else {
+      // its own steps on its own stream
+      hc_status s = hc_step(c, nsteps);
+      if (s != HC_OK) return s;
+    }
+  }
+  if (batch.empty()) return HC_OK;
+  return fp32(batch[0]) ? batch_impl<float>(batch.data(), (int)batch.size(), nsteps)
+                        : batch_impl<double>(batch.data(), (int)batch.size(), nsteps);
 }
 
 hc_status hc_autotune(hc_ctx* ctx, int steps_per_trial, char* report, int report_len) {

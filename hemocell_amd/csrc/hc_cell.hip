@@ -16,6 +16,8 @@
 // Step t+1's spread needs exactly x_t and F(x_t), which steps 1-2 produce, so
 // doing it here saves reloading the cell.  Positions stay in LDS for the whole
 // kernel; per-vertex bending forces and velocities too when they fit.
+#include <algorithm>
+
 #include "hc_ibm_dev.h"
 #include "hc_mech_dev.h"
 
@@ -31,19 +33,17 @@ constexpr int kMaxPlanes = 64;
 // workgroup needs LDS only for the spread box and fits beside the many small
 // workgroups of another cell type running at the same time.
 template <typename R, int MODEL, int BS, bool STAGED, bool GPOS>
-__global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
-                                                  const int64_t* __restrict__ cell_base, const R* __restrict__ uf,
-                                                  const uint8_t* __restrict__ type, LatticeDev L, R* __restrict__ Fnext,
-                                                  const double* __restrict__ extra, double* __restrict__ scratch,
-                                                  double f_limit, int flags, int box_nodes,
-                                                  unsigned long long* wall_hits) {
+__device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev& T, const int64_t base,
+                                               const R* __restrict__ uf, const uint8_t* __restrict__ type,
+                                               const LatticeDev& L, R* __restrict__ Fnext,
+                                               const double* __restrict__ extra, double* __restrict__ scratch,
+                                               double f_limit, int flags, int box_nodes,
+                                               unsigned long long* wall_hits) {
   extern __shared__ __align__(16) double sm[];
   __shared__ double red[BS];
   __shared__ double vforce_sh;
   __shared__ int bmin[3], bmax[3], bad;
   __shared__ int pcount[kMaxPlanes + 1], pstart[kMaxPlanes + 1];
-  const int cell = cells[blockIdx.x];
-  const int64_t base = cell_base[cell];
   const int tid = threadIdx.x;
   const int nv = T.nv;
   const int64_t vs = V.vs;
@@ -360,6 +360,25 @@ __global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const 
   }
 }
 
+template <typename R, int MODEL, int BS, bool STAGED, bool GPOS>
+__global__ __launch_bounds__(BS) void k_cell_step(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
+                                                  const int64_t* __restrict__ cell_base, const R* __restrict__ uf,
+                                                  const uint8_t* __restrict__ type, LatticeDev L, R* __restrict__ Fnext,
+                                                  const double* __restrict__ extra, double* __restrict__ scratch,
+                                                  double f_limit, int flags, int box_nodes,
+                                                  unsigned long long* wall_hits) {
+  cell_step_body<R, MODEL, BS, STAGED, GPOS>(V, T, cell_base[cells[blockIdx.x]], uf, type, L, Fnext, extra, scratch,
+                                             f_limit, flags, box_nodes, wall_hits);
+}
+
+// cells of several independent cases in one launch, one workgroup per cell
+template <typename R, int MODEL, int BS, bool STAGED, bool GPOS>
+__global__ __launch_bounds__(BS) void k_cell_step_multi(const CellCase* __restrict__ cs, int box_nodes) {
+  const CellCase& c = cs[blockIdx.x];
+  cell_step_body<R, MODEL, BS, STAGED, GPOS>(c.V, c.T, c.base, (const R*)c.uf, c.type, c.L, (R*)c.Fnext, c.extra,
+                                             c.scratch, c.f_limit, c.flags, box_nodes, c.wall_hits);
+}
+
 }  // namespace
 
 // Fixed LDS of the kernel (bytes) for a type: positions, plus velocities and bending
@@ -389,9 +408,12 @@ template <typename R, int M, int B>
 static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                               const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L, R* Fnext,
                               const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
-                              int lds_device, unsigned long long* wall_hits, bool gpos) {
-  static const size_t st_staged = static_lds(k_cell_step<R, M, B, true, false>);
-  static const size_t st_plain = static_lds(k_cell_step<R, M, B, false, false>);
+                              int lds_device, unsigned long long* wall_hits, bool gpos,
+                              const CellCase* multi = nullptr) {
+  static const size_t st_staged = std::max(static_lds(k_cell_step<R, M, B, true, false>),
+                                           static_lds(k_cell_step_multi<R, M, B, true, false>));
+  static const size_t st_plain = std::max(static_lds(k_cell_step<R, M, B, false, false>),
+                                          static_lds(k_cell_step_multi<R, M, B, false, false>));
   static const size_t st_gpos = static_lds(k_cell_step<R, M, B, false, true>);
   const size_t margin = 256, dev = (size_t)lds_device;
   const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * sizeof(double) : 0;
@@ -409,6 +431,16 @@ static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& 
   if (fixed + region + lists + st + margin > dev) return hipErrorInvalidConfiguration;
   const int box_nodes = (int)(region / (3 * sizeof(double)));
   const size_t dyn = fixed + region + lists;
+  if (multi) {
+    if (gpos) return hipErrorInvalidValue;
+    if (staged)
+      hipLaunchKernelGGL((k_cell_step_multi<R, M, B, true, false>), dim3((unsigned)ncells), dim3(B), dyn, s, multi,
+                         box_nodes);
+    else
+      hipLaunchKernelGGL((k_cell_step_multi<R, M, B, false, false>), dim3((unsigned)ncells), dim3(B), dyn, s, multi,
+                         box_nodes);
+    return hipGetLastError();
+  }
   if (staged)
     hipLaunchKernelGGL((k_cell_step<R, M, B, true, false>), dim3((unsigned)ncells), dim3(B), dyn, s, V, T, cells,
                        cell_base, uf, type, L, Fnext, extra, scratch, f_limit, flags, box_nodes, wall_hits);
@@ -454,6 +486,39 @@ hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T,
 #undef HC_CELL
 }
 
+template <typename R>
+hipError_t launch_cell_step_multi(hipStream_t s, const TypeDev& T, const CellCase* cases, int ncells, int box_bytes,
+                                  int lds_device, int block) {
+  if (ncells <= 0) return hipSuccess;
+  const VertexDev V{};
+  const LatticeDev L{};
+  hipError_t e = hipErrorInvalidConfiguration;
+#define HC_CELL(M, B)                                                                                             \
+  if (e == hipErrorInvalidConfiguration)                                                                          \
+    e = launch_cell<R, M, B>(s, V, T, nullptr, ncells, nullptr, nullptr, nullptr, L, nullptr, nullptr, nullptr, 0.0, \
+                             0, box_bytes, lds_device, nullptr, false, cases);
+  switch (T.model) {
+    case HC_MODEL_RBC:
+      if (block >= 1024) HC_CELL(HC_MODEL_RBC, 1024)
+      if (block >= 512) HC_CELL(HC_MODEL_RBC, 512)
+      HC_CELL(HC_MODEL_RBC, 256)
+      return e;
+    case HC_MODEL_PLT:
+      HC_CELL(HC_MODEL_PLT, 64)
+      return e;
+    case HC_MODEL_WBC:
+      if (block >= 1024) HC_CELL(HC_MODEL_WBC, 1024)
+      if (block >= 512) HC_CELL(HC_MODEL_WBC, 512)
+      HC_CELL(HC_MODEL_WBC, 256)
+      return e;
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef HC_CELL
+}
+
+template hipError_t launch_cell_step_multi<float>(hipStream_t, const TypeDev&, const CellCase*, int, int, int, int);
+template hipError_t launch_cell_step_multi<double>(hipStream_t, const TypeDev&, const CellCase*, int, int, int, int);
 template hipError_t launch_cell_step<float>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
                                             const int64_t*, const float*, const uint8_t*, const LatticeDev&, float*,
                                             const double*, double*, double, int, int, int, unsigned long long*, int,

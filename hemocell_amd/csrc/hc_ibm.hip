@@ -91,10 +91,9 @@ __global__ __launch_bounds__(BLOCK) void k_spread(VertexDev V, R* __restrict__ F
 // coalesced pass writing (u0, u1, u2, 0) per node, so the interpolation reads
 // each support node with one vector load.
 template <typename R, bool PHASE1, bool CHECK>
-__global__ __launch_bounds__(BLOCK) void k_velocity_field(const R* __restrict__ f, const R* __restrict__ Fcur,
-                                                          R* __restrict__ uf, LatticeDev L, double g0, double g1,
-                                                          double g2) {
-  const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void velocity_node_field(const R* __restrict__ f, const R* __restrict__ Fcur,
+                                                    R* __restrict__ uf, const LatticeDev& L, double g0, double g1,
+                                                    double g2, const uint32_t n) {
   if (n >= (uint32_t)L.N) return;
   const int64_t S = L.stride;
   const int z = (int)(n % (uint32_t)L.nz);
@@ -112,6 +111,27 @@ __global__ __launch_bounds__(BLOCK) void k_velocity_field(const R* __restrict__ 
   q[1] = j1 * invRho + (R(g1) + Fcur[S + n]) / R(2);
   q[2] = j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2);
   q[3] = R(0);
+}
+
+template <typename R, bool PHASE1, bool CHECK>
+__global__ __launch_bounds__(BLOCK) void k_velocity_field(const R* __restrict__ f, const R* __restrict__ Fcur,
+                                                          R* __restrict__ uf, LatticeDev L, double g0, double g1,
+                                                          double g2) {
+  velocity_node_field<R, PHASE1, CHECK>(f, Fcur, uf, L, g0, g1, g2, blockIdx.x * BLOCK + threadIdx.x);
+}
+
+// several independent lattices in one launch; case c owns the blocks from c.block0
+template <typename R, bool PHASE1, bool CHECK>
+__global__ __launch_bounds__(BLOCK) void k_velocity_field_multi(const VelCase* __restrict__ cs, int ncase) {
+  int lo = 0, hi = ncase - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cs[mid].block0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const VelCase& c = cs[lo];
+  velocity_node_field<R, PHASE1, CHECK>((const R*)c.f, (const R*)c.Fcur, (R*)c.uf, c.L, c.g[0], c.g[1], c.g[2],
+                                        (blockIdx.x - c.block0) * BLOCK + threadIdx.x);
 }
 
 template <typename R>
@@ -345,6 +365,20 @@ hipError_t launch_velocity_field(hipStream_t s, const R* f, const R* Fcur, R* uf
 }
 
 template <typename R>
+hipError_t launch_velocity_field_multi(hipStream_t s, const VelCase* cases, int ncase, uint32_t blocks, int phase,
+                                       bool general) {
+  if (ncase <= 0 || blocks == 0) return hipSuccess;
+  const dim3 g(blocks), b(BLOCK);
+  if (phase) {
+    if (general) hipLaunchKernelGGL((k_velocity_field_multi<R, true, true>), g, b, 0, s, cases, ncase);
+    else hipLaunchKernelGGL((k_velocity_field_multi<R, true, false>), g, b, 0, s, cases, ncase);
+  } else {
+    hipLaunchKernelGGL((k_velocity_field_multi<R, false, false>), g, b, 0, s, cases, ncase);
+  }
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_interp_advance(hipStream_t s, const VertexDev& V, const R* uf, const uint8_t* type,
                                  const LatticeDev& L, int do_interp, int do_advance, unsigned long long* wall_hits) {
   if (V.n == 0) return hipSuccess;
@@ -367,7 +401,8 @@ hipError_t launch_add_extra_force(hipStream_t s, const VertexDev& V, const int64
   template hipError_t launch_velocity_field<R>(hipStream_t, const R*, const R*, R*, const LatticeDev&, int, bool,     \
                                                const double*);                                                       \
   template hipError_t launch_interp_advance<R>(hipStream_t, const VertexDev&, const R*, const uint8_t*,               \
-                                               const LatticeDev&, int, int, unsigned long long*);
+                                               const LatticeDev&, int, int, unsigned long long*);                     \
+  template hipError_t launch_velocity_field_multi<R>(hipStream_t, const VelCase*, int, uint32_t, int, bool);
 HC_INSTANTIATE(float)
 HC_INSTANTIATE(double)
 #undef HC_INSTANTIATE

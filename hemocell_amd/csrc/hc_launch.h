@@ -93,6 +93,56 @@ hipError_t launch_boundary_repulsion(hipStream_t s, const VertexDev& V, const La
                                      double k, double r_c);
 hipError_t launch_vertex_bins(hipStream_t s, const VertexDev& V, const LatticeDev& L, int64_t* out);
 
+// One lattice of a batched launch: its buffers, geometry and collision
+// constants, and the first block of its range (blocks of 256 nodes).
+struct LatCase {
+  void* f;
+  const void* Fcur;
+  void* Fclr;
+  const uint8_t* type;
+  LatticeDev L;
+  CollideArgs a;
+  uint32_t block0;
+};
+// ncase lattices of one parity and boundary kind in one launch of `blocks` blocks
+template <typename R>
+hipError_t launch_collide_multi(hipStream_t s, const LatCase* cases, int ncase, uint32_t blocks, int odd,
+                                bool general, bool vel);
+// the same for the velocity field: uf of each case from its post-stream populations
+struct VelCase {
+  const void* f;
+  const void* Fcur;
+  void* uf;
+  LatticeDev L;
+  double g[3];
+  uint32_t block0;
+};
+template <typename R>
+hipError_t launch_velocity_field_multi(hipStream_t s, const VelCase* cases, int ncase, uint32_t blocks, int phase,
+                                       bool general);
+
+// one cell of a batched cell-step launch (hc_cell.hip): everything the per-type
+// launch passes, for the cell at vertex base `base` of its case
+struct CellCase {
+  VertexDev V;
+  TypeDev T;
+  int64_t base;
+  const void* uf;
+  const uint8_t* type;
+  LatticeDev L;
+  void* Fnext;
+  const double* extra;
+  double* scratch;
+  double f_limit;
+  int flags;
+  unsigned long long* wall_hits;
+};
+// cells of one mesh (T gives the model, vertex count and viscosity for the LDS
+// layout, identical over the cases) from several cases in one launch
+template <typename R>
+hipError_t launch_cell_step_multi(hipStream_t s, const TypeDev& T, const CellCase* cases, int ncells, int box_bytes,
+                                  int lds_device, int block);
+
 // Lees-Edwards planes after the collide-stream (hc_lbm.hip): d is the current
 // displacement; scratch holds 2 nx ny x 19 populations
 template <typename R>

@@ -184,11 +184,10 @@ __device__ __forceinline__ void velocity_node(R (&fl)[19], int face, const Colli
 // velocity-boundary nodes are reconstructed in place (each node only touches its
 // own 19 slots); without VEL they are skipped and k_velocity_nodes updates them.
 // VISC: the relaxation rate of a node comes from its interior-viscosity class.
-template <typename R, bool ODD, bool GENERAL, bool VEL = false, bool VISC = false>
-__global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, const R* __restrict__ Fcur,
-                                                          R* __restrict__ Fclr, const uint8_t* __restrict__ type,
-                                                          LatticeDev L, CollideArgs a) {
-  const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+template <typename R, bool ODD, bool GENERAL, bool VEL, bool VISC>
+__device__ __forceinline__ void collide_node(R* __restrict__ f, const R* __restrict__ Fcur, R* __restrict__ Fclr,
+                                             const uint8_t* __restrict__ type, const LatticeDev& L,
+                                             const CollideArgs& a, const uint32_t n) {
   if (n >= (uint32_t)L.N) return;
   const int64_t S = L.stride;
   const uint8_t ty = GENERAL && type ? type[n] : (uint8_t)HC_NODE_FLUID;
@@ -221,6 +220,34 @@ __global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, con
   Fclr[S + n] = R(0);
   Fclr[2 * S + n] = R(0);
   aa_store<R, ODD, GENERAL>(f, S, n, o, fl, std::make_integer_sequence<int, 19>{});
+}
+
+template <typename R, bool ODD, bool GENERAL, bool VEL = false, bool VISC = false>
+__global__ __launch_bounds__(BLOCK) void k_collide_stream(R* __restrict__ f, const R* __restrict__ Fcur,
+                                                          R* __restrict__ Fclr, const uint8_t* __restrict__ type,
+                                                          LatticeDev L, CollideArgs a) {
+  collide_node<R, ODD, GENERAL, VEL, VISC>(f, Fcur, Fclr, type, L, a, blockIdx.x * BLOCK + threadIdx.x);
+}
+
+// the case of block b in a batched launch: the last case whose first block is <= b
+template <typename C>
+__device__ __forceinline__ const C& case_of_block(const C* __restrict__ cs, int ncase, uint32_t b) {
+  int lo = 0, hi = ncase - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return cs[lo];
+}
+
+// Several independent lattices in one launch (small cases served together):
+// each case owns a contiguous range of blocks starting at block0.
+template <typename R, bool ODD, bool GENERAL, bool VEL>
+__global__ __launch_bounds__(BLOCK) void k_collide_multi(const LatCase* __restrict__ cs, int ncase) {
+  const LatCase& c = case_of_block(cs, ncase, blockIdx.x);
+  collide_node<R, ODD, GENERAL, VEL, false>((R*)c.f, (const R*)c.Fcur, (R*)c.Fclr, c.type, c.L, c.a,
+                                            (blockIdx.x - c.block0) * BLOCK + threadIdx.x);
 }
 
 // Velocity-boundary nodes (listed in `nodes`): regularized reconstruction, then
@@ -593,6 +620,24 @@ hipError_t launch_to_double(hipStream_t s, const R* in, double* out, int64_t n) 
 }
 
 template <typename R>
+hipError_t launch_collide_multi(hipStream_t s, const LatCase* cases, int ncase, uint32_t blocks, int odd,
+                                bool general, bool vel) {
+  if (ncase <= 0 || blocks == 0) return hipSuccess;
+  const dim3 g(blocks), b(BLOCK);
+  if (general && vel) {
+    if (odd) hipLaunchKernelGGL((k_collide_multi<R, true, true, true>), g, b, 0, s, cases, ncase);
+    else hipLaunchKernelGGL((k_collide_multi<R, false, true, true>), g, b, 0, s, cases, ncase);
+  } else if (general) {
+    if (odd) hipLaunchKernelGGL((k_collide_multi<R, true, true, false>), g, b, 0, s, cases, ncase);
+    else hipLaunchKernelGGL((k_collide_multi<R, false, true, false>), g, b, 0, s, cases, ncase);
+  } else {
+    if (odd) hipLaunchKernelGGL((k_collide_multi<R, true, false, false>), g, b, 0, s, cases, ncase);
+    else hipLaunchKernelGGL((k_collide_multi<R, false, false, false>), g, b, 0, s, cases, ncase);
+  }
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_lees_edwards(hipStream_t s, R* f, const uint8_t* type, const LatticeDev& L, int phase,
                                const CollideArgs& a, double d, double u_top, double u_bottom, R* scratch) {
   const int64_t n = 2 * (int64_t)L.nx * L.ny;
@@ -617,7 +662,8 @@ hipError_t launch_lees_edwards(hipStream_t s, R* f, const uint8_t* type, const L
   template hipError_t launch_fill_zero<R>(hipStream_t, R*, int64_t);                                            \
   template hipError_t launch_to_double<R>(hipStream_t, const R*, double*, int64_t);                             \
   template hipError_t launch_lees_edwards<R>(hipStream_t, R*, const uint8_t*, const LatticeDev&, int,             \
-                                             const CollideArgs&, double, double, double, R*);
+                                             const CollideArgs&, double, double, double, R*);                  \
+  template hipError_t launch_collide_multi<R>(hipStream_t, const LatCase*, int, uint32_t, int, bool, bool);
 HC_INSTANTIATE(float)
 HC_INSTANTIATE(double)
 #undef HC_INSTANTIATE

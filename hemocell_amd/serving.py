@@ -6,10 +6,13 @@ pulled by a stretching force on 7 vertices per end inside zero-velocity walls
 (helper/hemoCellStretch.cpp:30-110, as in tests/validation/stretch_cell).
 Cases arrive as a Poisson stream; every case gets its own context (its own
 HIP stream and device buffers), so cases never share state and a case's
-result does not depend on what else runs.  One host thread admits arrivals,
-feeds every active case a chunk of steps at a time (the launches are
-asynchronous) and retires the cases whose stream has drained; the latency of
-a case is its completion time minus its arrival time, set-up included.
+result does not depend on what else runs.  One host thread admits arrivals
+while slots and the HBM budget allow (continuous batching: a case joins at the
+next chunk boundary), steps all active cases together in chunks
+(hc_step_batch: one launch per kernel and case group per step, the launches
+asynchronous, the next chunk prepared while the previous one runs) and
+retires the cases whose stream has drained; the latency of a case is its
+completion time minus its arrival time, set-up included.
 """
 import math
 import time
@@ -129,9 +132,19 @@ def run_alone(case, units, precision=hc.HC_FP32_SHIFTED):
     return r, x
 
 
-def serve(cases, units, max_active=16, chunk=500, precision=hc.HC_FP32_SHIFTED, keep_vertices=False,
-          progress=None):
-    """Run the stream in real time; returns the cases with latencies and results."""
+def footprint(case, precision=hc.HC_FP32_SHIFTED):
+    """Device bytes of one case: populations, two force buffers, the velocity
+    field and the node types per node, plus the cell's vertex arrays."""
+    s = 4 if precision == hc.HC_FP32_SHIFTED else 8
+    nv = {"RBC": 642, "PLT": 66, "WBC": 642}[case.cell]
+    return case.edge ** 3 * ((19 + 6 + 4) * s + 1) + nv * 7 * 3 * 8
+
+
+def serve(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SHIFTED, keep_vertices=False,
+          progress=None, hbm_budget=64 << 30, batched=True):
+    """Run the stream in real time; returns the cases with latencies and results.
+    Arrivals are admitted while fewer than max_active cases run and their
+    footprints fit in hbm_budget bytes."""
     pending = sorted(cases, key=lambda c: c.arrival)
     active = []  # (case, sim, steps_left)
     done = []
@@ -139,13 +152,25 @@ def serve(cases, units, max_active=16, chunk=500, precision=hc.HC_FP32_SHIFTED, 
     k = 0
     while k < len(pending) or active:
         now = time.perf_counter() - t0
-        while k < len(pending) and pending[k].arrival <= now and len(active) < max_active:
+        used = sum(footprint(a[0], precision) for a in active)
+        while (k < len(pending) and pending[k].arrival <= now and len(active) < max_active
+               and (not active or used + footprint(pending[k], precision) <= hbm_budget)):
             c = pending[k]
             k += 1
             c.started = time.perf_counter() - t0
+            used += footprint(c, precision)
             active.append([c, build(c, units, precision), c.steps])
         idle = True
-        for a in active:
+        stepping = [a for a in active if a[2] > 0]
+        if batched and stepping:
+            # every active case advances by the same chunk, ending exactly at the
+            # first case's last step
+            n = min(chunk, min(a[2] for a in stepping))
+            hc.step_batch([a[1] for a in stepping], n)
+            for a in stepping:
+                a[2] -= n
+            idle = False
+        for a in ([] if batched else active):
             c, sim, left = a
             if left > 0 and not sim.busy():
                 n = min(chunk, left)

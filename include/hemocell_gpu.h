@@ -183,6 +183,15 @@ hc_status hc_set_extra_vertex_force(hc_ctx* ctx, int n, const int64_t* vertex_id
  * on the context stream; errors of the launches are reported. */
 hc_status hc_step(hc_ctx* ctx, int nsteps);
 hc_status hc_synchronize(hc_ctx* ctx);
+/* nsteps of each of n contexts, stepped together: every step launches the collide-stream of
+ * all contexts of one AA parity and boundary kind at once, then their velocity fields, then
+ * the cells of one mesh at once, so many small cases cost a few launches per step.  Each
+ * context computes what hc_step computes; a context whose step is not the plain fused one
+ * (repulsion, interior viscosity, Lees-Edwards planes, a timescale above one) is stepped by
+ * hc_step instead.  All contexts share one device and precision.  Asynchronous: each
+ * context's stream is ordered after the batch.  No counterpart in the reference (one
+ * HemoCell per process); used by the serving loop to batch independent cases. */
+hc_status hc_step_batch(hc_ctx* const* ctxs, int n, int nsteps);
 /* Non-blocking: 1 while work queued on the context is still running, 0 when idle, -1 on
  * error.  No counterpart in the reference (its iterate() is synchronous); used by hosts
  * that drive several contexts from one thread. */
