@@ -142,7 +142,7 @@ def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension", au
     return res
 
 
-def serve_stream(n, rate):
+def serve_stream(n, rate, label="serving"):
     """p50/p99 latency of a seeded Poisson stream of single-cell cases (shear 10-1000 1/s
     or stretch 0-200 pN on one RBC, PLT or WBC; cubic edge 32-96; 2000-20000 steps;
     FP32-shifted), each case in its own context and stream, the active cases stepped
@@ -151,7 +151,7 @@ def serve_stream(n, rate):
     from hemocell_amd import serving
     units = hc.lbm_base_parameters(5e-7, 1e-7)
     stream = serving.make_cases(n, rate, seed=7)
-    print("[bench] serving %d cases at %.1f/s" % (n, rate), file=sys.stderr, flush=True)
+    print("[bench] %s: %d cases at %.1f/s" % (label, n, rate), file=sys.stderr, flush=True)
     done, wall = serving.serve(stream, units, max_active=32, chunk=250,
                                progress=lambda c, k: print("[bench] case %d done (%d/%d) %.2fs" % (
                                    c.ident, k, n, c.latency), file=sys.stderr, flush=True) if k % 8 == 0 else None)
@@ -293,6 +293,9 @@ def main():
     # the serving stream and the CPU baseline belong to the headline workload
     if args.serve > 0 and args.workload == "suspension":
         line["serving"] = serve_stream(args.serve, args.serve_rate)
+        # the same kind of stream arriving faster than it can be served: its
+        # cases/s is the sustained rate, with the latencies at that rate
+        line["serving_saturated"] = serve_stream(max(8, (3 * args.serve) // 4), 50.0, "serving (saturated)")
     if not args.no_cpu and args.workload == "suspension":
         line["cpu_baseline"] = cpu_baseline(hc, susp)
     print(json.dumps(line), flush=True)

@@ -978,14 +978,30 @@ hc_status batch_impl(hc_ctx* const* cs, int n, int nsteps) {
                "batched velocity_field");
       }
     }
+    // the cell groups (one per mesh) run side by side: the first on the lead
+    // stream, the others on its two side streams, joined before the next step
+    int ncell = 0;
+    for (const Group& g : groups[k & 1]) ncell += g.kind == 2;
+    const int nside = std::min(ncell - 1, lead->side[0] && lead->side[1] ? 2 : 0);
+    if (nside > 0) {
+      HC_TRY(hipEventRecord(lead->fork_ev, lead->stream), "hipEventRecord");
+      for (int j = 0; j < nside; ++j) HC_TRY(hipStreamWaitEvent(lead->side[j], lead->fork_ev, 0), "hipStreamWaitEvent");
+    }
+    int gi = 0;
     for (const Group& g : groups[k & 1])
       if (g.kind == 2) {
+        hipStream_t st = (gi == 0 || nside == 0) ? lead->stream : lead->side[(gi - 1) % nside];
+        ++gi;
         int block = g.T.model == HC_MODEL_WBC ? lead->wbc_block : lead->cell_block;
         if (g.count < lead->n_cu) block = 1024;
-        HC_TRY(launch_cell_step_multi<R>(lead->stream, g.T, (const CellCase*)(base + g.offset), g.count, box,
-                                         lead->lds_device, block),
+        HC_TRY(launch_cell_step_multi<R>(st, g.T, (const CellCase*)(base + g.offset), g.count, box, lead->lds_device,
+                                         block),
                "batched cell_step");
       }
+    for (int j = 0; j < nside; ++j) {
+      HC_TRY(hipEventRecord(lead->join_ev[j], lead->side[j]), "hipEventRecord");
+      HC_TRY(hipStreamWaitEvent(lead->stream, lead->join_ev[j], 0), "hipStreamWaitEvent");
+    }
   }
   HC_TRY(hipEventRecord(lead->batch_ev, lead->stream), "hipEventRecord");
   HC_TRY(hipEventRecord(lead->batch_slot_ev[slot], lead->stream), "hipEventRecord");
