@@ -103,6 +103,7 @@ SIGNATURES = {
     "hc_stage_boundary_repulsion": (_ST, [_P]),
     "hc_set_repulsion_force": (_ST, [_P, _D]),
     "hc_step_batch": (_ST, [ctypes.POINTER(_P), _INT, _INT]),
+    "hc_set_periodicity": (_ST, [_P, ctypes.POINTER(_INT)]),
     "hc_set_lees_edwards": (_ST, [_P, ctypes.c_double, ctypes.c_double]),
     "hc_stage_lees_edwards": (_ST, [_P]),
     "hc_set_interior_viscosity": (_ST, [_P, _INT, ctypes.c_double]),
@@ -285,6 +286,10 @@ class Simulation:
         fr = np.zeros((self.num_vertices, 3))
         self._chk(self.lib.hc_get_boundary_repulsion_force(self.h, _dp(fr)))
         return fr
+
+    def set_periodicity(self, periodic):
+        per = (_INT * 3)(*[int(p) for p in periodic])
+        self._chk(self.lib.hc_set_periodicity(self.h, per))
 
     def set_lees_edwards(self, shear_rate, displacement_per_step):
         """Lees-Edwards planes z = 0 and z = nz-1 driven at -+(nz-1) shear_rate / 2."""

@@ -65,6 +65,7 @@ struct hc_ctx {
   uint32_t* vel_nodes = nullptr;  // nodes with a velocity boundary
   int n_vel = 0;
   bool general = false;
+  bool has_walls = false;  // a node of the mask is not bulk fluid
   int cur = 0;
   bool other_clean = true;
   int phase = 0;  // AA layout: 0 natural, 1 after an even step
@@ -1291,6 +1292,7 @@ hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* no
   }
   // grouped by face, so that the lanes of a wavefront take the same face branch
   std::stable_sort(vel.begin(), vel.end(), [&](uint32_t a, uint32_t b) { return node_type[a] < node_type[b]; });
+  ctx->has_walls = has_walls;
   ctx->general = has_walls || !all_periodic;
   const size_t rs = rsize(ctx);
   hc_status s;
@@ -1502,6 +1504,26 @@ hc_status hc_set_boundary_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu,
   ctx->brep_k = k_rep;
   ctx->brep_cutoff = cutoff_lu;
   ctx->brep_timescale = timescale;
+  return HC_OK;
+}
+
+hc_status hc_set_periodicity(hc_ctx* ctx, const int periodic[3]) {
+  hc_status s = need_lattice(ctx);
+  if (s != HC_OK) return s;
+  if (!periodic) return fail(ctx, HC_INVALID_ARG, "null periodicity");
+  if (ctx->le && (!periodic[0] || !periodic[2]))
+    return fail(ctx, HC_INVALID_ARG, "Lees-Edwards planes need a lattice periodic in x and z");
+  HC_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  if ((s = invalidate_spread(ctx)) != HC_OK) return s;
+  bool all = true;
+  for (int a = 0; a < 3; ++a) {
+    ctx->L.periodic[a] = periodic[a] ? 1 : 0;
+    all = all && periodic[a];
+  }
+  ctx->general = ctx->has_walls || !all;
+  if (ctx->d_bflag) hipFree(ctx->d_bflag);  // boundary particles are flagged again on first use
+  ctx->d_bflag = nullptr;
+  drop_graph(ctx);
   return HC_OK;
 }
 

@@ -233,6 +233,19 @@ void Parameters::lbm_pipe_parameters(Config& cfg, int nY) {
   u_lbm_max = re * nu_lbm / (pipe_radius * 2);
 }
 
+// constantConversion.cpp:61-72
+void Parameters::lbm_pipe_parameters(Config& cfg, const Lattice& L) {
+  lbm_base_parameters(cfg);
+  re = cfg["domain"]["Re"].read<T>();
+  T area = 0;
+  for (plint y = 0; y < L.getNy(); ++y)
+    for (plint z = 0; z < L.getNz(); ++z) area += L.isBoundary(0, y, z) ? 0 : 1;
+  pipe_radius = std::sqrt(area / M_PI);
+  hlog << "(Parameters) The channel has a calculated radius of " << pipe_radius
+       << " LU, assuming a perfect circle cross-section." << std::endl;
+  u_lbm_max = re * nu_lbm / (pipe_radius * 2);
+}
+
 // constantConversion.cpp:88-94
 void Parameters::lbm_shear_parameters(Config& cfg, T nx) {
   lbm_base_parameters(cfg);
@@ -687,9 +700,14 @@ void HemoCell::setInitialMinimumDistanceFromSolid(std::string name, T distance) 
   (*cellfields)[name]->minimumDistanceFromSolid = distance;
 }
 
+// core/hemoCell.cpp:168-180
 void HemoCell::setSystemPeriodicity(unsigned int axis, bool bePeriodic) {
-  if (bound) throw HemoCellError("setSystemPeriodicity must be called before the lattice is used");
+  if (!lattice || axis > 2) throw HemoCellError("setSystemPeriodicity: no lattice or a bad axis");
   lattice->periodicity().toggle((int)axis, bePeriodic);
+  if (bound) {
+    const int p[3] = {lattice->periodicity().get(0), lattice->periodicity().get(1), lattice->periodicity().get(2)};
+    check(hc_set_periodicity(ctx, p), "setSystemPeriodicity");
+  }
 }
 
 // io/readPositionsBloodCells.cpp:36-361: <name>.pos holds the cell count, then
@@ -840,6 +858,40 @@ FluidStatistics fluid_statistics(HemoCell* h, int what) {
 
 FluidStatistics FluidInfo::calculateVelocityStatistics(HemoCell* hemocell) { return fluid_statistics(hemocell, 0); }
 FluidStatistics FluidInfo::calculateForceStatistics(HemoCell* hemocell) { return fluid_statistics(hemocell, 1); }
+
+// ========================================================== particleInfo ====
+// helper/particleInfo.cpp:57-150: every vertex of every cell
+namespace {
+ParticleStatistics particle_statistics(HemoCell* h, bool force) {
+  h->bindLattice();
+  hc_ctx* ctx = h->context();
+  const int64_t nv = hc_num_vertices(ctx);
+  ParticleStatistics r;
+  if (nv == 0) return r;
+  std::vector<double> x((size_t)(3 * nv)), v((size_t)(3 * nv)), f((size_t)(3 * nv)), fr((size_t)(3 * nv), 0.0);
+  h->check(hc_get_vertices(ctx, x.data(), v.data(), f.data()), "ParticleInfo");
+  if (force && h->repulsionEnabled) h->check(hc_get_repulsion_force(ctx, fr.data()), "ParticleInfo");
+  double sum = 0;
+  for (int64_t i = 0; i < nv; ++i) {
+    double q[3];
+    for (int c = 0; c < 3; ++c) q[c] = force ? f[(size_t)(3 * i + c)] + fr[(size_t)(3 * i + c)] : v[(size_t)(3 * i + c)];
+    const double m = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    if (i == 0 || m < r.min) r.min = m;
+    if (i == 0 || m > r.max) r.max = m;
+    sum += m;
+  }
+  r.avg = sum / (double)nv;
+  r.ncells = (pluint)nv;
+  return r;
+}
+}  // namespace
+
+ParticleStatistics ParticleInfo::calculateForceStatistics(HemoCell* hemocell) {
+  return particle_statistics(hemocell, true);
+}
+ParticleStatistics ParticleInfo::calculateVelocityStatistics(HemoCell* hemocell) {
+  return particle_statistics(hemocell, false);
+}
 
 // ============================================================== cellInfo ====
 std::map<int, CellInformation> CellInformationFunctionals::info_per_cell;

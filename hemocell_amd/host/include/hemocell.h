@@ -150,6 +150,7 @@ std::string toXml(const XMLNode& node, int indent = 0);
 
 // ----------------------------------------------------------- Parameters -----
 // SI -> lattice conversion (mechanics/constantConversion.{h,cpp}).
+class Lattice;
 class Parameters {
  public:
   static T dt, dx, dm, df;
@@ -161,6 +162,9 @@ class Parameters {
   static T pipe_radius;
   static void lbm_base_parameters(Config& cfg);
   static void lbm_pipe_parameters(Config& cfg, int nY);
+  // the radius from the fluid nodes of the lattice's x = 0 plane, sqrt(area / pi)
+  // (the reference's overload taking the voxelised flag matrix)
+  static void lbm_pipe_parameters(Config& cfg, const Lattice& lattice);
   static void lbm_shear_parameters(Config& cfg, T nx);
   static void printParameters();
   static hc_units units();
@@ -426,6 +430,20 @@ class FluidInfo {
   static FluidStatistics calculateVelocityStatistics(HemoCell* hemocell);
   // spreads the vertex forces into a reset field first (helper/fluidInfo.cpp:117-120)
   static FluidStatistics calculateForceStatistics(HemoCell* hemocell);
+};
+
+// --------------------------------------------------------- particleInfo -----
+// helper/particleInfo.h: minimum, maximum and mean over all vertices of
+// |force + repulsion force| or of |velocity| (lattice units); ncells is the
+// number of vertices counted.
+struct ParticleStatistics {
+  double min = 0, max = 0, avg = 0;
+  pluint ncells = 0;
+};
+class ParticleInfo {
+ public:
+  static ParticleStatistics calculateForceStatistics(HemoCell* hemocell);
+  static ParticleStatistics calculateVelocityStatistics(HemoCell* hemocell);
 };
 
 // -------------------------------------------------------------- cellInfo -----

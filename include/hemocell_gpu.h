@@ -145,6 +145,11 @@ hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int time
  * cut-off (lattice units) with k (r_c / r) r_hat, every `timescale` steps, before the
  * spread; the force is recomputed at each application and kept between. */
 hc_status hc_set_boundary_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale);
+/* Replaces: HemoCell::setSystemPeriodicity after the lattice exists (core/hemoCell.cpp:
+ * 168-180, which toggles the lattice's and the particle field's periodicity): the periodic
+ * flags of the three axes; positions wrap and populations stream across the periodic faces
+ * from the next step on. */
+hc_status hc_set_periodicity(hc_ctx* ctx, const int periodic[3]);
 /* Replaces: LeesEdwardsBC (helper/leesEdwardsBC.h:301-364, as set up in
  * cases/leesEdwards/leesEdwards.cpp:43-44,73): after every collide-stream the nodes of the
  * planes z = nz-1 and z = 0 relax once more towards the equilibrium of the momentum
