@@ -184,6 +184,17 @@ __device__ __forceinline__ void velocity_node(R (&fl)[19], int face, const Colli
 // velocity-boundary nodes are reconstructed in place (each node only touches its
 // own 19 slots); without VEL they are skipped and k_velocity_nodes updates them.
 // VISC: the relaxation rate of a node comes from its interior-viscosity class.
+// relaxation rate of interior-viscosity class vc (1..15): selected with
+// constant indices, so the table is read from the kernel arguments without a
+// private copy for a run-time index
+__device__ __forceinline__ double class_omega(const CollideArgs& a, unsigned vc) {
+  double o = a.omega;
+#pragma unroll
+  for (unsigned k = 1; k < 16; ++k)
+    if (vc == k) o = a.omega_cls[k];
+  return o;
+}
+
 template <typename R, bool ODD, bool GENERAL, bool VEL, bool VISC>
 __device__ __forceinline__ void collide_node(R* __restrict__ f, const R* __restrict__ Fcur, R* __restrict__ Fclr,
                                              const uint8_t* __restrict__ type, const LatticeDev& L,
@@ -211,8 +222,8 @@ __device__ __forceinline__ void collide_node(R* __restrict__ f, const R* __restr
     const R F0 = R(a.g[0]) + Fcur[n], F1 = R(a.g[1]) + Fcur[S + n], F2 = R(a.g[2]) + Fcur[2 * S + n];
     R om = R(a.omega);
     if (VISC) {
-      const uint8_t vc = a.visc[n];
-      if (vc) om = R(a.omega_cls[vc & 15]);
+      const unsigned vc = a.visc[n];
+      if (vc) om = R(class_omega(a, vc));
     }
     collide_guo<R>(fl, F0, F1, F2, om);
   }
@@ -480,7 +491,7 @@ __global__ void k_lees_edwards_get(const R* __restrict__ f, const uint8_t* __res
   const R jSqr = uw * uw;
   const R invRho = R(1) / (rhoBar + R(1));
   R om = R(a.omega);
-  if (a.visc && a.visc[n]) om = R(a.omega_cls[a.visc[n] & 15]);
+  if (a.visc && a.visc[n]) om = R(class_omega(a, a.visc[n]));
 #pragma unroll
   for (int i = 0; i < 19; ++i) o[i] = fl[i] * (R(1) - om) + om * feq<R>(i, rhoBar, invRho, uw, R(0), R(0), jSqr);
   int64_t s1, s2;

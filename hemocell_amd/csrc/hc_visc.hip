@@ -23,9 +23,9 @@
 // (x - z, y - z), keeps clear of the node's (with a margin far above the test's
 // rounding), and one whose corners all lie above the node on some axis (the
 // ray decreases in every coordinate).  The nodes are visited line by line
-// along the ray direction, so the lanes of a wave share a few lines and the
-// first cull skips a triangle for the whole wave; the triangle loop is uniform
-// over the workgroup (scalar loads and LDS broadcasts).
+// along the ray direction: all nodes of a line share (x - z, y - z), so one
+// thread lists the line's candidate triangles once (a uniform loop of LDS
+// broadcasts) and tests only those for each node of the line.
 //
 // Membrane pass: the reference visits particles in order and the last write to
 // a node wins.  Two kernels reproduce that without ordering the threads: the
@@ -117,30 +117,26 @@ __device__ __forceinline__ float upper_f(double a) {
   return f + (fabsf(f) * 1e-6f + 1e-2f);
 }
 
-constexpr int LINES_MAX = 4096;
+constexpr int CAND_MAX = 32;  // triangles listed per line before it falls back
 
 // LINES: the box's nodes are visited line by line along the ray direction
-// (fixed x - z and y - z), consecutive nodes of a wave sharing a few lines, so
-// that the projected-box test skips a triangle for the whole wave at once; the
-// dynamic LDS then holds the positions, one float4 projected box per triangle
-// and the line offsets.  A box with more than LINES_MAX lines, and LINES false,
-// visit the nodes in x, y, z order.
+// (fixed x - z and y - z), the triangles that can cross a line being listed
+// once per line; the dynamic LDS then holds the positions, one float4
+// projected box per triangle and each thread's candidate list.  Without LINES
+// (a mesh too large for that LDS) the nodes are visited in x, y, z order.
 template <bool LINES>
 __global__ __launch_bounds__(VB) void k_visc_grid(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
                                                   const int64_t* __restrict__ cell_base, LatticeDev L,
                                                   const uint8_t* __restrict__ type, uint8_t cls,
                                                   uint8_t* __restrict__ visc) {
-  extern __shared__ __attribute__((aligned(16))) double sp[];  // 3 * nv positions [, nt float4, LINES_MAX + 1 ints]
+  extern __shared__ __attribute__((aligned(16))) double sp[];  // 3 nv positions [, nt float4, VB x CAND_MAX]
   __shared__ double red[6][VB / 64];
   __shared__ int64_t box[6];
-  __shared__ int part[VB];
-  __shared__ int total_s;
   const int nv = T.nv;
   double* px = sp;
   double* py = sp + nv;
   double* pz = sp + 2 * nv;
   float4* pb = reinterpret_cast<float4*>(sp + ((3 * nv + 1) & ~1));
-  int* prefix = reinterpret_cast<int*>(pb + T.nt);
   const int64_t b = cell_base[cells[blockIdx.x]];
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int k = threadIdx.x; k < nv; k += VB) {
@@ -193,63 +189,58 @@ __global__ __launch_bounds__(VB) void k_visc_grid(VertexDev V, TypeDev T, const 
   const int64_t ex = box[3] - box[0] + 1, ey = box[4] - box[1] + 1, ez = box[5] - box[2] + 1;
   if (ex <= 0 || ey <= 0 || ez <= 0) return;
   if (LINES) {
+    for (int t = threadIdx.x; t < T.nt; t += VB) {
+      const int i0 = T.tri[3 * t], i1 = T.tri[3 * t + 1], i2 = T.tri[3 * t + 2];
+      const double u0 = px[i0] - pz[i0], u1 = px[i1] - pz[i1], u2 = px[i2] - pz[i2];
+      const double v0 = py[i0] - pz[i0], v1 = py[i1] - pz[i1], v2 = py[i2] - pz[i2];
+      pb[t] = make_float4(lower_f(fmin(fmin(u0, u1), u2)), upper_f(fmax(fmax(u0, u1), u2)),
+                          lower_f(fmin(fmin(v0, v1), v2)), upper_f(fmax(fmax(v0, v1), v2)));
+    }
+    __syncthreads();
+    // one thread per line along the ray direction (fixed u = x - z, v = y - z):
+    // the triangles whose projected box holds (u, v) are listed once, then
+    // every node of the line, Z in [max(z0, x0 - u, y0 - v), min(z1, x1 - u,
+    // y1 - v)], counts its crossings over that list alone
     const int64_t ulo = box[0] - box[5], vlo = box[1] - box[5];
     const int64_t nu = box[3] - box[2] - ulo + 1, nw = box[4] - box[2] - vlo + 1;
-    if (nu * nw <= LINES_MAX) {
-      const int nl = (int)(nu * nw);
-      for (int t = threadIdx.x; t < T.nt; t += VB) {
-        const int i0 = T.tri[3 * t], i1 = T.tri[3 * t + 1], i2 = T.tri[3 * t + 2];
-        const double u0 = px[i0] - pz[i0], u1 = px[i1] - pz[i1], u2 = px[i2] - pz[i2];
-        const double v0 = py[i0] - pz[i0], v1 = py[i1] - pz[i1], v2 = py[i2] - pz[i2];
-        pb[t] = make_float4(lower_f(fmin(fmin(u0, u1), u2)), upper_f(fmax(fmax(u0, u1), u2)),
-                            lower_f(fmin(fmin(v0, v1), v2)), upper_f(fmax(fmax(v0, v1), v2)));
+    uint16_t* cand = reinterpret_cast<uint16_t*>(pb + T.nt) + threadIdx.x * CAND_MAX;
+    for (int64_t l = threadIdx.x; l < nu * nw; l += VB) {
+      const int64_t u = ulo + l / nw, v = vlo + l % nw;
+      const int64_t zs = max(box[2], max(box[0] - u, box[1] - v));
+      const int64_t ze = min(box[5], min(box[3] - u, box[4] - v));
+      if (ze < zs) continue;
+      const float us = (float)u, vs = (float)v;
+      int cnt = 0;
+      for (int t = 0; t < T.nt; ++t) {
+        const float4 b = pb[t];
+        if (us < b.x || us > b.y || vs < b.z || vs > b.w) continue;
+        if (cnt < CAND_MAX) cand[cnt] = (uint16_t)t;
+        ++cnt;
       }
-      // nodes of line l: Z in [max(z0, x0 - u, y0 - v), min(z1, x1 - u, y1 - v)]
-      auto zrange = [&](int l, int64_t& u, int64_t& v, int64_t& zs) -> int64_t {
-        u = ulo + l / nw;
-        v = vlo + l % nw;
-        zs = max(box[2], max(box[0] - u, box[1] - v));
-        const int64_t ze = min(box[5], min(box[3] - u, box[4] - v));
-        return ze >= zs ? ze - zs + 1 : 0;
-      };
-      const int chunk = (nl + VB - 1) / VB;
-      const int c0 = min(nl, (int)threadIdx.x * chunk), c1 = min(nl, c0 + chunk);
-      int sum = 0;
-      int64_t u, v, zs;
-      for (int l = c0; l < c1; ++l) sum += (int)zrange(l, u, v, zs);
-      part[threadIdx.x] = sum;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < VB; ++k) {
-          const int q = part[k];
-          part[k] = acc;
-          acc += q;
+      for (int64_t Z = zs; Z <= ze; ++Z) {
+        if (cnt > CAND_MAX) {  // a crowded line: every triangle, culled per node
+          classify_node(u + Z, v + Z, Z, T, px, py, pz, pb, L, type, cls, visc);
+          continue;
         }
-        total_s = acc;
-      }
-      __syncthreads();
-      int acc = part[threadIdx.x];
-      for (int l = c0; l < c1; ++l) {
-        prefix[l] = acc;
-        acc += (int)zrange(l, u, v, zs);
-      }
-      __syncthreads();
-      const int total = total_s;
-      for (int q = threadIdx.x; q < total; q += VB) {
-        // the last line whose first node is at or before q
-        int a = 0, z = nl - 1;
-        while (a < z) {
-          const int mid = (a + z + 1) >> 1;
-          if (prefix[mid] <= q) a = mid;
-          else z = mid - 1;
+        const int64_t X = u + Z, Y = v + Z;
+        const int64_t n =
+            wrap_node(Z, L.nz) + (int64_t)L.nz * (wrap_node(Y, L.ny) + (int64_t)L.ny * wrap_node(X, L.nx));
+        if (type != nullptr && type[n] != HC_NODE_FLUID) continue;
+        const double sx = (double)X, sy = (double)Y, sz = (double)Z;
+        int cross = 0;
+        for (int c = 0; c < cnt; ++c) {
+          const int t = cand[c];
+          const int i0 = T.tri[3 * t], i1 = T.tri[3 * t + 1], i2 = T.tri[3 * t + 2];
+          const double ax = px[i0], ay = py[i0], az = pz[i0];
+          const double bx = px[i1], by = py[i1], bz = pz[i1];
+          const double cx = px[i2], cy = py[i2], cz = pz[i2];
+          if (fmin(fmin(ax, bx), cx) > sx || fmin(fmin(ay, by), cy) > sy || fmin(fmin(az, bz), cz) > sz) continue;
+          cross += ray_crosses(ax, ay, az, bx, by, bz, cx, cy, cz, sx, sy, sz);
         }
-        zrange(a, u, v, zs);
-        const int64_t Z = zs + (q - prefix[a]);
-        classify_node(u + Z, v + Z, Z, T, px, py, pz, pb, L, type, cls, visc);
+        if (cross & 1) visc[n] = cls;
       }
-      return;
     }
+    return;
   }
   const int64_t total = ex * ey * ez;
   for (int64_t q = threadIdx.x; q < total; q += VB) {
@@ -341,8 +332,8 @@ hipError_t launch_visc_grid(hipStream_t s, const VertexDev& V, const TypeDev& T,
   // 64 KB of LDS per workgroup, less the static arrays
   constexpr size_t kBudget = 65536 - 2048;
   const size_t pos = sizeof(double) * (((size_t)3 * T.nv + 1) & ~(size_t)1);
-  const size_t lines = pos + sizeof(float4) * (size_t)T.nt + sizeof(int) * (LINES_MAX + 1);
-  if (lines <= kBudget) {
+  const size_t lines = pos + sizeof(float4) * (size_t)T.nt + sizeof(uint16_t) * VB * CAND_MAX;
+  if (lines <= kBudget && T.nt < 65536) {
     hipLaunchKernelGGL((k_visc_grid<true>), dim3((unsigned)ncells), dim3(VB), lines, s, V, T, cells, cell_base, L,
                        type, (uint8_t)cls, visc);
   } else {

@@ -72,7 +72,7 @@ def test_entire_grid_pass_with_walls_matches_oracle():
 
 def test_entire_grid_pass_large_cell_matches_oracle():
     """A 2562-vertex WBC (its positions fill the LDS: nodes in x, y, z order)
-    beside an RBC stretched along x (more lines than the line table holds)."""
+    beside an RBC stretched flat in x and z (6660 lines along the ray)."""
     print("[iv] entire grid, large cells", flush=True)
     u = cases.units()
     shape = (72, 44, 72)
@@ -81,7 +81,7 @@ def test_entire_grid_pass_large_cell_matches_oracle():
     o = OracleSim(*shape, u.tau)
     rng = np.random.default_rng(21)
     tids = []
-    # the RBC lies flat in y, stretched in x and z: (ex + ez) (ey + ez) lines > 4096
+    # the RBC lies flat in y, stretched in x and z
     for kind, center, rot, stretch in (("WBC2562", (12.3, 10.6, 11.8), cases.rotation(rng), (1.0, 1.0, 1.0)),
                                        ("RBC", (40.2, 31.7, 40.3), np.eye(3), (3.5, 1.0, 3.5))):
         model, v, t, p, inner = cases.mesh(kind, u, hc.build_mesh)
