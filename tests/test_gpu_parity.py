@@ -623,3 +623,31 @@ def test_velocity_walls_in_their_own_kernel(monkeypatch):
     _, ug = g.fluid()
     _, uo = o.fluid()
     assert cases.rel_l2(ug, uo) < 1e-10
+
+
+def test_periodicity_changed_after_binding():
+    """hc_set_periodicity on a bound lattice (setSystemPeriodicity after the lattice
+    exists, core/hemoCell.cpp:168-180) steps as a lattice created with that
+    periodicity, cells included."""
+    print("[parity] periodicity", flush=True)
+    u = cases.units()
+    shape = (24, 20, 22)
+    mask = cases.channel_mask(*shape, axis=1)
+    rng = np.random.default_rng(31)
+    f0 = cases.random_populations(int(np.prod(shape)), rng, amp=1e-4)
+    sims = []
+    for start in ((0, 0, 0), (1, 0, 1)):
+        g = hc.Simulation(0)
+        g.lattice_init(*shape, u.tau, periodic=start, node_type=mask)
+        g.set_populations(f0)
+        g.set_body_force([0, 0, 1e-6])
+        g.set_force_limit(u.f_limit)
+        model, v, t, p, inner = cases.mesh("PLT", u, hc.build_mesh)
+        tid = g.add_cell_type(model, v, t, p, inner)
+        g.add_cells(tid, cases.place(v, (1.2, 10.3, 20.9), cases.rotation(np.random.default_rng(2)))[None])
+        sims.append(g)
+    sims[0].set_periodicity((1, 0, 1))
+    for g in sims:
+        g.step(7)
+    assert cases.rel_l2(sims[0].vertices()[0], sims[1].vertices()[0]) < 1e-12
+    assert cases.rel_l2(sims[0].populations(), sims[1].populations()) < 1e-12
