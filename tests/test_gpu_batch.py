@@ -80,3 +80,44 @@ def test_batch_arguments():
     assert a.iteration == 0
     a.close()
     b.close()
+
+
+def test_empty_and_ragged_inputs():
+    """A cell-free context and a context whose cell type has no cells step in a
+    batch as alone; interior viscosity, the stages and the statistics accept an
+    empty suspension; zero steps and an empty batch are no-ops."""
+    print("[batch] empty", flush=True)
+    u = tcases.units()
+    shape = (16, 12, 10)
+    f0 = tcases.random_populations(int(np.prod(shape)), np.random.default_rng(3))
+    sims = []
+    for _ in range(2):
+        pair = []
+        for k in range(2):
+            g = hc.Simulation(0)
+            g.lattice_init(*shape, u.tau)
+            g.set_populations(f0)
+            g.set_body_force([1e-6, 0, 0])
+            if k == 1:  # a registered type without cells
+                model, v, t, p, inner = tcases.mesh("PLT", u, hc.build_mesh)
+                tid = g.add_cell_type(model, v, t, p, inner)
+                g.set_interior_viscosity(tid, 2.0)
+            pair.append(g)
+        sims.append(pair)
+    for g in sims[0]:
+        g.step(9)
+    hc.step_batch(sims[1], 9)
+    hc.step_batch([], 5)
+    for a, b in zip(sims[0], sims[1]):
+        assert b.iteration == a.iteration == 9
+        assert np.array_equal(b.populations(), a.populations())
+    g = sims[1][1]
+    g.stage("interior_viscosity", 0)
+    assert np.all(g.interior_viscosity() == 0.0)
+    assert g.cells()[0].shape == (0,)
+    st = g.fluid_stats()
+    assert st["ncells"] == int(np.prod(shape))
+    for pair in sims:
+        for s in pair:
+            s.close()
+
