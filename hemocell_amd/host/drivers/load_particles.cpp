@@ -77,7 +77,8 @@ static int run(int argc, char* argv[]) {
                                      hemo::OUTPUT_FORCE_AREA, hemo::OUTPUT_FORCE_VOLUME, hemo::OUTPUT_FORCE_BENDING,
                                      hemo::OUTPUT_FORCE_LINK, hemo::OUTPUT_FORCE_VISC,
                                      hemo::OUTPUT_FORCE_INNER_LINK});
-    hemocell.setFluidOutputs({hemo::OUTPUT_VELOCITY, hemo::OUTPUT_DENSITY, hemo::OUTPUT_BOUNDARY});
+    hemocell.setFluidOutputs({hemo::OUTPUT_VELOCITY, hemo::OUTPUT_DENSITY, hemo::OUTPUT_BOUNDARY,
+                              hemo::OUTPUT_OMEGA, hemo::OUTPUT_INTERIOR_POINTS});
     const int n = std::atoi(steps);
     for (int k = 0; k < n; ++k) hemocell.iterate();
     hemocell.writeOutput();

@@ -105,7 +105,9 @@ def test_write_output_in_the_reference_layout(tmp_path):
     u = cases.units()
     nx, ny, nz = 40, 30, 36
     (tmp_path / "config.xml").write_text(cases.config_xml())
-    (tmp_path / "RBC.xml").write_text(cases.material_xml(cases.RBC_MAT, "RBC"))
+    # interior viscosity on the RBCs: the Omega and InteriorPoints datasets show it
+    rbc_mat = dict(cases.RBC_MAT, viscosityRatio=5.0, enableInteriorViscosity=1)
+    (tmp_path / "RBC.xml").write_text(cases.material_xml(rbc_mat, "RBC"))
     (tmp_path / "PLT.xml").write_text(cases.material_xml(cases.PLT_MAT, "PLT", inner_edges=21))
     (tmp_path / "RBC.pos").write_text("2\n6 7.5 6 90 0 0\n14 7.5 12 0 30 0\n")
     (tmp_path / "PLT.pos").write_text("1\n3 11 15 10 20 30\n")
@@ -149,6 +151,17 @@ def test_write_output_in_the_reference_layout(tmp_path):
     assert vel["dims"] == [nz + 2, ny + 2, nx + 2, 3] and np.isfinite(vel["data"]).all()
     rho = np.array(_h5(fl, "Density")["data"])
     assert np.allclose(rho, u.df / u.dx ** 2, rtol=1e-3)
+    # FluidHdf5IO.hh:333-370: tau inside the RBC membranes, 0 elsewhere; omega of
+    # every node in the reference's SI scaling, 0 on the bounce-back walls
+    tau_in = 5.0 * (u.tau - 0.5) + 0.5
+    ip = np.array(_h5(fl, "InteriorPoints")["data"]).reshape(nz + 2, ny + 2, nx + 2)
+    inner = np.isclose(ip, tau_in, rtol=1e-6)
+    assert np.all(inner | (ip == 0)) and inner.sum() > 500
+    om = np.array(_h5(fl, "Omega")["data"]).reshape(nz + 2, ny + 2, nx + 2)
+    si = u.df / u.dx ** 2
+    assert np.all(om[b == 1] == 0)
+    assert np.allclose(om[inner], si / tau_in, rtol=1e-6)
+    assert np.allclose(om[(b == 0) & ~inner], si / u.tau, rtol=1e-6)
     rows = (tmp_path / "tmp" / "csv" / ("RBC.%s.csv" % it)).read_text().splitlines()
     assert rows[0] == "X,Y,Z,area,volume,atomic_block,cellId,baseCellId,velocity_x,velocity_y,velocity_z"
     assert len(rows) == 3
