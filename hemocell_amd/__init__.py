@@ -35,6 +35,16 @@ class ModelParams(ctypes.Structure):
                                                "k_inner_rigid", "k_cytoskeleton", "core_radius", "radius")]
 
 
+class CaseSpec(ctypes.Structure):
+    """hc_case_spec: one single-cell case for the request loop (lattice units)."""
+    _fields_ = [("kind", ctypes.c_int), ("edge", ctypes.c_int), ("steps", ctypes.c_int64),
+                ("precision", ctypes.c_int), ("tau", ctypes.c_double), ("f_limit", ctypes.c_double),
+                ("wall_u", ctypes.c_double), ("stretch_force", ctypes.c_double), ("stretch_n", ctypes.c_int),
+                ("model", ctypes.c_int), ("nv", ctypes.c_int), ("nt", ctypes.c_int), ("n_inner", ctypes.c_int),
+                ("vertices", ctypes.POINTER(ctypes.c_double)), ("triangles", ctypes.POINTER(ctypes.c_int32)),
+                ("inner_edges", ctypes.POINTER(ctypes.c_int32)), ("params", ModelParams)]
+
+
 class Units(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("dx", "dt", "nu_p", "rho_p", "kBT_p",
                                                "tau", "nu_lbm", "dm", "df", "f_limit", "kBT_lbm")]
@@ -103,6 +113,13 @@ SIGNATURES = {
     "hc_stage_boundary_repulsion": (_ST, [_P]),
     "hc_set_repulsion_force": (_ST, [_P, _D]),
     "hc_step_batch": (_ST, [ctypes.POINTER(_P), _INT, _INT]),
+    "hc_server_create": (_ST, [_INT, _INT, ctypes.c_double, ctypes.POINTER(_P)]),
+    "hc_server_destroy": (None, [_P]),
+    "hc_server_last_error": (ctypes.c_char_p, [_P]),
+    "hc_case_submit": (_ST, [_P, ctypes.POINTER(CaseSpec), ctypes.POINTER(ctypes.c_int64)]),
+    "hc_server_poll": (_ST, [_P, _INT, ctypes.POINTER(_INT)]),
+    "hc_case_status": (_INT, [_P, ctypes.c_int64]),
+    "hc_case_result": (_ST, [_P, ctypes.c_int64, _D, _D]),
     "hc_set_periodicity": (_ST, [_P, ctypes.POINTER(_INT)]),
     "hc_set_lees_edwards": (_ST, [_P, ctypes.c_double, ctypes.c_double]),
     "hc_stage_lees_edwards": (_ST, [_P]),
@@ -178,6 +195,62 @@ def build_mesh(shape, radius_lu, min_triangles, aspect_ratio=0.3):
     lib.hc_build_mesh(shape, radius_lu, min_triangles, aspect_ratio, ctypes.byref(nv), ctypes.byref(nt),
                       _dp(v), t.ctypes.data_as(_I32))
     return v, t
+
+
+CASE_SHEAR, CASE_STRETCH = 0, 1
+CASE_QUEUED, CASE_RUNNING, CASE_DONE, CASE_FAILED = 0, 1, 2, 3
+
+
+class Server:
+    """The native request loop (hc_server_*): submit single-cell cases, poll to
+    admit, step (batched) and retire them, read the finished ones."""
+
+    def __init__(self, device=0, max_active=32, hbm_budget=64 << 30):
+        self.lib = library()
+        h = _P()
+        st = self.lib.hc_server_create(device, max_active, float(hbm_budget), ctypes.byref(h))
+        if st:
+            raise HcError(st, "hc_server_create failed")
+        self.h = h
+        self._nv = {}
+
+    def _chk(self, st):
+        if st:
+            raise HcError(st, self.lib.hc_server_last_error(self.h).decode())
+
+    def submit(self, kind, edge, steps, model, vertices, triangles, params, tau, f_limit, wall_u=0.0,
+               stretch_force=0.0, stretch_n=7, inner_edges=None, precision=HC_FP32_SHIFTED):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        t = np.ascontiguousarray(triangles, dtype=np.int32)
+        ie = None if inner_edges is None else np.ascontiguousarray(inner_edges, dtype=np.int32)
+        spec = CaseSpec(kind=kind, edge=edge, steps=steps, precision=precision, tau=tau, f_limit=f_limit,
+                        wall_u=wall_u, stretch_force=stretch_force, stretch_n=stretch_n, model=model,
+                        nv=len(v), nt=len(t), n_inner=0 if ie is None else len(ie),
+                        vertices=v.ctypes.data_as(_D), triangles=t.ctypes.data_as(_I32),
+                        inner_edges=None if ie is None else ie.ctypes.data_as(_I32), params=params)
+        cid = ctypes.c_int64()
+        self._chk(self.lib.hc_case_submit(self.h, ctypes.byref(spec), ctypes.byref(cid)))
+        self._nv[cid.value] = len(v)
+        return cid.value
+
+    def poll(self, chunk=250):
+        n = _INT()
+        self._chk(self.lib.hc_server_poll(self.h, int(chunk), ctypes.byref(n)))
+        return n.value
+
+    def status(self, case_id):
+        return self.lib.hc_case_status(self.h, case_id)
+
+    def result(self, case_id):
+        x = np.zeros((self._nv[case_id], 3))
+        st = np.zeros(3)
+        self._chk(self.lib.hc_case_result(self.h, case_id, _dp(x), _dp(st)))
+        return x, dict(volume=st[0], area=st[1], stretch=st[2])
+
+    def close(self):
+        if self.h:
+            self.lib.hc_server_destroy(self.h)
+            self.h = None
 
 
 def step_batch(sims, n=1):

@@ -200,6 +200,55 @@ def serve(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SHIFTED, 
     return done, time.perf_counter() - t0
 
 
+def spec(case, units):
+    """Keyword arguments of Server.submit for one case (the set-up of build())."""
+    t = _cell_types(units)[case.cell]
+    kw = dict(kind=hc.CASE_SHEAR if case.kind == "shear" else hc.CASE_STRETCH, edge=case.edge, steps=case.steps,
+              model=t.model, vertices=t.verts, triangles=t.tris, params=t.params, inner_edges=t.inner,
+              tau=units.tau, f_limit=units.f_limit)
+    if case.kind == "shear":
+        kw["wall_u"] = (case.edge - 1) * case.shear_rate * units.dt * 0.5
+    else:
+        kw["stretch_force"] = case.force_pN * 1e-12 / units.df
+        kw["stretch_n"] = 7
+    return kw
+
+
+def serve_native(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SHIFTED, keep_vertices=False,
+                 hbm_budget=64 << 30):
+    """The same stream through the library's own request loop (hc_server_*):
+    arrivals are submitted as they come, every poll retires, admits and steps."""
+    server = hc.Server(0, max_active, hbm_budget)
+    pending = sorted(cases, key=lambda c: c.arrival)
+    ids = {}
+    done = []
+    t0 = time.perf_counter()
+    k = 0
+    try:
+        while k < len(pending) or len(done) < len(pending):
+            now = time.perf_counter() - t0
+            while k < len(pending) and pending[k].arrival <= now:
+                c = pending[k]
+                k += 1
+                c.started = now
+                ids[c.ident] = (c, server.submit(precision=precision, **spec(c, units)))
+            server.poll(chunk)
+            for ident, (c, cid) in list(ids.items()):
+                if server.status(cid) == hc.CASE_DONE:
+                    c.finished = time.perf_counter() - t0
+                    x, st = server.result(cid)
+                    c.result = dict(volume=float(st["volume"]), stretch=float(st["stretch"]))
+                    if keep_vertices:
+                        c.result["x"] = x
+                    done.append(c)
+                    del ids[ident]
+            if not ids and k < len(pending):
+                time.sleep(max(0.0, min(0.01, pending[k].arrival - (time.perf_counter() - t0))))
+    finally:
+        server.close()
+    return done, time.perf_counter() - t0
+
+
 def latency_stats(done, wall):
     lat = np.array([c.latency for c in done])
     return dict(cases=len(done), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),

@@ -197,6 +197,48 @@ hc_status hc_synchronize(hc_ctx* ctx);
  * context's stream is ordered after the batch.  No counterpart in the reference (one
  * HemoCell per process); used by the serving loop to batch independent cases. */
 hc_status hc_step_batch(hc_ctx* const* ctxs, int n, int nsteps);
+
+/* ---- request loop for small single-cell cases ------------------------------
+ * No counterpart in the reference (one HemoCell per process, one case per run): a server
+ * keeps each submitted case in its own context, admits queued cases in arrival order while
+ * fewer than max_active run and their device footprints fit hbm_budget_bytes, and advances
+ * every running case together (hc_step_batch) by a chunk per poll.  A shear case is a cubic
+ * box with velocity walls at +-wall_u (x velocity) on its z faces, periodic in x and y
+ * (helper/hemocellInit.hh:58-92); a stretch case has walls at rest on every face and pulls
+ * the stretch_n lowest and highest vertices in x with -+stretch_force / stretch_n
+ * (helper/hemoCellStretch.cpp:30-110).  The cell (a mesh from hc_build_mesh with constants
+ * from hc_model_constants) is centred in the box; lattice units throughout. */
+typedef struct hc_server hc_server;
+enum { HC_CASE_SHEAR = 0, HC_CASE_STRETCH = 1 };
+enum { HC_CASE_QUEUED = 0, HC_CASE_RUNNING = 1, HC_CASE_DONE = 2, HC_CASE_FAILED = 3 };
+typedef struct {
+  int kind;               /* HC_CASE_SHEAR or HC_CASE_STRETCH */
+  int edge;               /* nodes per side of the cubic box */
+  int64_t steps;
+  int precision;          /* hc_precision */
+  double tau;
+  double f_limit;
+  double wall_u;          /* shear: wall speed */
+  double stretch_force;   /* stretch: total force per end */
+  int stretch_n;          /* stretch: vertices per end */
+  int model, nv, nt, n_inner;
+  const double* vertices; /* nv*3, copied at submission */
+  const int32_t* triangles;
+  const int32_t* inner_edges;
+  hc_model_params params;
+} hc_case_spec;
+hc_status hc_server_create(int device, int max_active, double hbm_budget_bytes, hc_server** out);
+void hc_server_destroy(hc_server* s);
+const char* hc_server_last_error(const hc_server* s);
+hc_status hc_case_submit(hc_server* s, const hc_case_spec* spec, int64_t* case_id);
+/* retire drained cases, admit, step every running case by up to `chunk` steps
+ * (asynchronous); n_running (may be NULL) receives the cases not yet done */
+hc_status hc_server_poll(hc_server* s, int chunk, int* n_running);
+/* HC_CASE_QUEUED / RUNNING / DONE / FAILED, -1 for an unknown id */
+int hc_case_status(const hc_server* s, int64_t case_id);
+/* a finished case's final vertex positions (nv*3) and stats {volume, area, largest
+ * vertex-vertex distance}; either pointer may be NULL */
+hc_status hc_case_result(hc_server* s, int64_t case_id, double* x, double* stats);
 /* Non-blocking: 1 while work queued on the context is still running, 0 when idle, -1 on
  * error.  No counterpart in the reference (its iterate() is synchronous); used by hosts
  * that drive several contexts from one thread. */

@@ -33,3 +33,41 @@ def test_alone_and_in_a_mix_are_identical(batched):
         assert c.latency > 0 and c.finished >= c.started
     st = serving.latency_stats(done, wall)
     assert st["cases"] == 6 and st["p99_s"] >= st["p50_s"] > 0
+
+
+def test_native_request_loop_matches_alone():
+    """The library's request loop (hc_server_*): every case of a stream finishes,
+    each as close to the case run alone as two lone runs are to each other."""
+    print("[serving] native", flush=True)
+    u = hc.lbm_base_parameters(5e-7, 1e-7)
+    stream = serving.make_cases(5, rate=500.0, seed=11, edge=(24, 36), steps=(120, 300))
+    done, wall = serving.serve_native(stream, u, max_active=3, chunk=50, keep_vertices=True)
+    assert sorted(c.ident for c in done) == list(range(5))
+    for c in done:
+        alone, x = serving.run_alone(c, u)
+        _, x2 = serving.run_alone(c, u)
+        noise = cases.rel_l2(x2, x)
+        mix = cases.rel_l2(c.result["x"], x)
+        assert mix < 1e-9 and mix <= 100 * max(noise, 1e-13), (c, mix, noise)
+        assert abs(c.result["volume"] - alone["volume"]) <= 1e-9 * alone["volume"]
+    st = serving.latency_stats(done, wall)
+    assert st["cases"] == 5 and st["p99_s"] >= st["p50_s"] > 0
+
+
+def test_native_server_arguments():
+    print("[serving] native arguments", flush=True)
+    u = hc.lbm_base_parameters(5e-7, 1e-7)
+    s = hc.Server(0, 2, 1 << 30)
+    c = serving.make_cases(1, rate=1.0, seed=3)[0]
+    kw = serving.spec(c, u)
+    with pytest.raises(hc.HcError):
+        s.submit(**dict(kw, edge=2))
+    cid = s.submit(**dict(kw, steps=3))
+    assert s.status(cid) == hc.CASE_QUEUED and s.status(cid + 99) == -1
+    with pytest.raises(hc.HcError):
+        s.result(cid)
+    while s.status(cid) != hc.CASE_DONE:
+        s.poll(10)
+    x, st = s.result(cid)
+    assert x.shape == (len(kw["vertices"]), 3) and st["volume"] > 0
+    s.close()
