@@ -233,22 +233,40 @@ hc_status hc_server_poll(hc_server* s, int chunk, int* n_running) {
   int64_t n = chunk;
   for (auto* c : s->active)
     if (c->left > 0) n = std::min(n, c->left);
+  hc_status first = HC_OK;
   for (int prec : {(int)HC_FP64, (int)HC_FP32_SHIFTED}) {
+    std::vector<hc_server::Case*> group;
     std::vector<hc_ctx*> ctxs;
     for (auto* c : s->active)
-      if (c->left > 0 && c->spec.precision == prec) ctxs.push_back(c->ctx);
+      if (c->left > 0 && c->spec.precision == prec) {
+        group.push_back(c);
+        ctxs.push_back(c->ctx);
+      }
     if (ctxs.empty()) continue;
-    if ((st = hc_step_batch(ctxs.data(), (int)ctxs.size(), (int)n)) != HC_OK) {
-      std::string m;
-      for (hc_ctx* x : ctxs)
-        if (*hc_last_error(x)) m = hc_last_error(x);
-      return serr(s, st, m);
+    if ((st = hc_step_batch(ctxs.data(), (int)ctxs.size(), (int)n)) == HC_OK) {
+      for (auto* c : group) c->left -= n;
+      continue;
     }
+    // a batch that failed leaves its cases in an unknown state: they fail, the
+    // cases of the other precision keep their steps, and the error is reported
+    std::string m;
+    for (hc_ctx* x : ctxs)
+      if (*hc_last_error(x)) m = hc_last_error(x);
+    if (first == HC_OK) {
+      first = st;
+      s->err = m;
+    }
+    for (auto* c : group) {
+      hc_destroy(c->ctx);
+      c->ctx = nullptr;
+      c->status = HC_CASE_FAILED;
+    }
+    s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
+                                   [](hc_server::Case* c) { return c->status == HC_CASE_FAILED; }),
+                    s->active.end());
   }
-  for (auto* c : s->active)
-    if (c->left > 0) c->left -= n;
   if (n_running) *n_running = (int)(s->active.size() + s->queued.size());
-  return HC_OK;
+  return first;
 }
 
 int hc_case_status(const hc_server* s, int64_t case_id) {

@@ -232,7 +232,9 @@ void hc_server_destroy(hc_server* s);
 const char* hc_server_last_error(const hc_server* s);
 hc_status hc_case_submit(hc_server* s, const hc_case_spec* spec, int64_t* case_id);
 /* retire drained cases, admit, step every running case by up to `chunk` steps
- * (asynchronous); n_running (may be NULL) receives the cases not yet done */
+ * (asynchronous); n_running (may be NULL) receives the cases not yet done.  A case
+ * whose set-up or batch fails is marked HC_CASE_FAILED and its context released; the
+ * error is returned (hc_server_last_error) and the other cases go on at the next poll. */
 hc_status hc_server_poll(hc_server* s, int chunk, int* n_running);
 /* HC_CASE_QUEUED / RUNNING / DONE / FAILED, -1 for an unknown id */
 int hc_case_status(const hc_server* s, int64_t case_id);

@@ -4,6 +4,8 @@ result whether it runs alone or inside a stream of other cases -- as close as
 two runs of it alone are to each other (the spread's float atomics commit in
 no fixed order, so repeated runs agree to round-off, not bit for bit) -- and
 every case of a stream completes."""
+import time
+
 import numpy as np
 import pytest
 
@@ -70,4 +72,29 @@ def test_native_server_arguments():
         s.poll(10)
     x, st = s.result(cid)
     assert x.shape == (len(kw["vertices"]), 3) and st["volume"] > 0
+    s.close()
+
+
+def test_native_server_failed_case_does_not_stop_the_others():
+    """A case whose mesh the library rejects at admission fails alone: poll reports
+    the error, the case reads FAILED and the case admitted beside it finishes."""
+    print("[serving] native failed case", flush=True)
+    u = hc.lbm_base_parameters(5e-7, 1e-7)
+    s = hc.Server(0, 4, 1 << 30)
+    c = serving.make_cases(1, rate=1.0, seed=5)[0]
+    kw = serving.spec(c, u)
+    good = s.submit(**dict(kw, steps=20))
+    tris = np.array(kw["triangles"], dtype=np.int32).copy()
+    tris.flat[0] = len(kw["vertices"]) + 7  # a vertex index past the mesh
+    bad = s.submit(**dict(kw, steps=20, triangles=tris))
+    with pytest.raises(hc.HcError):
+        s.poll(10)
+    assert s.status(bad) == hc.CASE_FAILED
+    t0 = time.time()
+    while s.status(good) != hc.CASE_DONE and time.time() - t0 < 60:  # poll does not block
+        s.poll(10)
+        time.sleep(1e-3)
+    assert s.status(good) == hc.CASE_DONE
+    x, st = s.result(good)
+    assert np.isfinite(x).all() and st["volume"] > 0
     s.close()
