@@ -1178,12 +1178,19 @@ hc_status hc_create(int device, hc_ctx** out) {
     ctx->side[0] = ctx->side[1] = nullptr;
   }
   if (hipMalloc((void**)&ctx->d_wall_hits, sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc((void**)&ctx->d_flag, sizeof(unsigned int)) != hipSuccess) {
-    hipStreamDestroy(ctx->stream);
+      hipMalloc((void**)&ctx->d_flag, sizeof(unsigned int)) != hipSuccess ||
+      hipMemset(ctx->d_wall_hits, 0, sizeof(unsigned long long)) != hipSuccess) {
+    if (ctx->d_wall_hits) (void)hipFree(ctx->d_wall_hits);
+    if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+    for (int j = 0; j < 2; ++j) {
+      if (ctx->side[j]) (void)hipStreamDestroy(ctx->side[j]);
+      if (ctx->join_ev[j]) (void)hipEventDestroy(ctx->join_ev[j]);
+    }
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HC_OOM;
   }
-  hipMemset(ctx->d_wall_hits, 0, sizeof(unsigned long long));
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
   {
@@ -1258,7 +1265,7 @@ hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* no
   if (precision != HC_FP64 && precision != HC_FP32_SHIFTED) return fail(ctx, HC_INVALID_ARG, "unknown precision");
   const int64_t N = (int64_t)nx * ny * nz;
   if (N >= ((int64_t)1 << 31)) return fail(ctx, HC_INVALID_ARG, "lattice too large (at most 2^31-1 nodes)");
-  hipSetDevice(ctx->device);
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, HC_DEVICE, "hipSetDevice failed");
   ctx->precision = precision;
   ctx->tau = tau;
   ctx->L.nx = nx;

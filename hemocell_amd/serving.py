@@ -255,3 +255,38 @@ def latency_stats(done, wall):
                 mean_s=float(lat.mean()), cases_per_s=len(done) / wall,
                 steps_per_s=float(sum(c.steps for c in done) / wall),
                 lups_per_s=float(sum(c.steps * c.edge ** 3 for c in done) / wall))
+
+
+def main(argv=None):
+    """Serve a seeded Poisson stream and print its latency statistics as one JSON line:
+    python -m hemocell_amd.serving --cases 24 --rate 50 [--native] [--per-case]"""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(prog="python -m hemocell_amd.serving")
+    ap.add_argument("--cases", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=50.0, help="Poisson arrival rate, cases/s")
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--max-active", type=int, default=32)
+    ap.add_argument("--chunk", type=int, default=250)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--native", action="store_true", help="the library's request loop (hc_server_*)")
+    ap.add_argument("--per-case", action="store_true", help="step each case on its own (no hc_step_batch)")
+    a = ap.parse_args(argv)
+    prec = hc.HC_FP32_SHIFTED if a.precision == "fp32" else hc.HC_FP64
+    units = hc.lbm_base_parameters(5e-7, 1e-7)
+    stream = make_cases(a.cases, a.rate, seed=a.seed)
+    print("[serving] %d cases at %.1f/s, %s" % (a.cases, a.rate, "native" if a.native else "python loop"),
+          file=sys.stderr, flush=True)
+    if a.native:
+        done, wall = serve_native(stream, units, a.max_active, a.chunk, prec)
+    else:
+        done, wall = serve(stream, units, a.max_active, a.chunk, prec, batched=not a.per_case)
+    st = latency_stats(done, wall)
+    st.update(arrival_rate=a.rate, max_active=a.max_active, chunk=a.chunk, precision=a.precision,
+              loop="native" if a.native else ("per-case" if a.per_case else "batched"))
+    print(json.dumps(st))
+
+
+if __name__ == "__main__":
+    main()
