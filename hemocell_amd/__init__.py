@@ -120,6 +120,7 @@ SIGNATURES = {
     "hc_server_poll": (_ST, [_P, _INT, ctypes.POINTER(_INT)]),
     "hc_case_status": (_INT, [_P, ctypes.c_int64]),
     "hc_case_result": (_ST, [_P, ctypes.c_int64, _D, _D]),
+    "hc_server_inject_failure": (_ST, [_P, ctypes.c_int64, _INT]),
     "hc_set_periodicity": (_ST, [_P, ctypes.POINTER(_INT)]),
     "hc_set_lees_edwards": (_ST, [_P, ctypes.c_double, ctypes.c_double]),
     "hc_stage_lees_edwards": (_ST, [_P]),
@@ -199,6 +200,7 @@ def build_mesh(shape, radius_lu, min_triangles, aspect_ratio=0.3):
 
 CASE_SHEAR, CASE_STRETCH = 0, 1
 CASE_QUEUED, CASE_RUNNING, CASE_DONE, CASE_FAILED = 0, 1, 2, 3
+INJECT_NONE, INJECT_RETIRE, INJECT_STREAM_ERROR = 0, 1, 2
 
 
 class Server:
@@ -240,6 +242,14 @@ class Server:
 
     def status(self, case_id):
         return self.lib.hc_case_status(self.h, case_id)
+
+    def inject_failure(self, case_id, what):
+        """Test hook: the case fails at its next poll, as a failed read-back at
+        retirement (INJECT_RETIRE) or a stream error (INJECT_STREAM_ERROR)."""
+        self._chk(self.lib.hc_server_inject_failure(self.h, case_id, int(what)))
+
+    def error(self):
+        return self.lib.hc_server_last_error(self.h).decode()
 
     def result(self, case_id):
         x = np.zeros((self._nv[case_id], 3))

@@ -993,8 +993,12 @@ hc_status batch_impl(hc_ctx* const* cs, int n, int nsteps) {
       if (g.kind == 2) {
         hipStream_t st = (gi == 0 || nside == 0) ? lead->stream : lead->side[(gi - 1) % nside];
         ++gi;
-        int block = g.T.model == HC_MODEL_WBC ? lead->wbc_block : lead->cell_block;
-        if (g.count < lead->n_cu) block = 1024;
+        // the lead context's launch shapes, chosen as do_cell_step chooses them:
+        // fewer cells than compute units take the widest block (platelets: 256
+        // threads instead of 64), unless HC_CELL_BLOCK fixes the RBC block
+        int block = g.T.model == HC_MODEL_WBC ? lead->wbc_block
+                                              : (g.T.model == HC_MODEL_PLT ? 64 : lead->cell_block);
+        if (g.count < lead->n_cu && !(lead->cell_block_env && g.T.model == HC_MODEL_RBC)) block = 1024;
         HC_TRY(launch_cell_step_multi<R>(st, g.T, (const CellCase*)(base + g.offset), g.count, box, lead->lds_device,
                                          block),
                "batched cell_step");

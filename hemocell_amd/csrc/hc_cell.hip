@@ -504,7 +504,8 @@ hipError_t launch_cell_step_multi(hipStream_t s, const TypeDev& T, const CellCas
       HC_CELL(HC_MODEL_RBC, 256)
       return e;
     case HC_MODEL_PLT:
-      // few cases per launch: four waves per platelet shorten its box clear and flush
+      // fewer platelets than compute units (the caller passes block 1024 then):
+      // four waves per platelet shorten its box clear and flush
       if (block >= 256) HC_CELL(HC_MODEL_PLT, 256)
       HC_CELL(HC_MODEL_PLT, 64)
       return e;

@@ -254,6 +254,14 @@ bool build_type_tables(TypeHost& T, std::string& err) {
       err = "mesh is not closed: an edge is not shared by exactly two triangles";
       return false;
     }
+    // first the triangle holding the directed edge b->a, then the one holding
+    // a->b: with outward triangles the platelet's dihedral bending force
+    // (pltSimpleModel.cpp:160-178) then opposes a fold of the edge
+    for (int k = 0; k < 3; ++k)
+      if (T.tri[3 * adj[0] + k] == a && T.tri[3 * adj[0] + (k + 1) % 3] == b) {
+        std::swap(adj[0], adj[1]);
+        break;
+      }
     T.edge_tris[2 * e] = adj[0];
     T.edge_tris[2 * e + 1] = adj[1];
     P3 n1 = tri_unit_normal(adj[0]), n2 = tri_unit_normal(adj[1]);

@@ -12,9 +12,12 @@
 //   * hc_server_poll retires the cases whose stream has drained (keeping their
 //     final positions and cell statistics), admits queued cases in arrival
 //     order while fewer than max_active run and their footprints fit the HBM
-//     budget, and advances every running case by the same chunk with
-//     hc_step_batch (one launch per kernel and case group per step), ending
-//     exactly at the first case's last step.  The launches are asynchronous.
+//     budget, and advances every running case by min(chunk, its steps left)
+//     with hc_step_batch (one launch per kernel and case group per step; one
+//     batch per precision and step count).  The launches are asynchronous.
+//   * a case fails alone: a set-up, batch, stream or read-back error marks it
+//     HC_CASE_FAILED and releases its context; the other cases go on and poll
+//     returns the first error of the pass.
 #include <algorithm>
 #include <cstring>
 #include <deque>
@@ -41,6 +44,7 @@ struct hc_server {
     double bytes = 0;
     std::vector<double> x;   // final positions
     double stats[3] = {0, 0, 0};  // volume, area, largest vertex-vertex distance
+    int inject = HC_SERVER_INJECT_NONE;  // host-side failure forced by hc_server_inject_failure
   };
   std::map<int64_t, std::unique_ptr<Case>> cases;
   std::deque<Case*> queued;
@@ -140,6 +144,7 @@ hc_status build(hc_server* s, hc_server::Case& c) {
 
 hc_status retire(hc_server* s, hc_server::Case& c) {
   hc_status st;
+  if (c.inject == HC_SERVER_INJECT_RETIRE) return serr(s, HC_STATE, "injected retire failure");
   c.x.assign((size_t)3 * c.spec.nv, 0.0);
   std::vector<double> v(c.x.size()), f(c.x.size());
   if ((st = hc_get_vertices(c.ctx, c.x.data(), v.data(), f.data())) != HC_OK) return serr(s, st, hc_last_error(c.ctx));
@@ -200,12 +205,28 @@ hc_status hc_case_submit(hc_server* s, const hc_case_spec* spec, int64_t* case_i
 
 hc_status hc_server_poll(hc_server* s, int chunk, int* n_running) {
   if (!s || chunk < 1) return HC_INVALID_ARG;
-  hc_status st;
-  // retire the cases that have taken all their steps and whose stream drained
+  hc_status st, first = HC_OK;
+  // a case that cannot go on: its context is released, the others keep going,
+  // and the first such error of this poll is reported
+  auto fail_case = [&](hc_server::Case* c, hc_status why, const std::string& m) {
+    if (c->ctx) hc_destroy(c->ctx);
+    c->ctx = nullptr;
+    c->status = HC_CASE_FAILED;
+    if (first == HC_OK) {
+      first = why;
+      s->err = "case " + std::to_string(c->id) + ": " + m;
+    }
+  };
+  // retire the cases that have taken all their steps and whose stream drained;
+  // a stream that reports an error fails its case
   std::vector<hc_server::Case*> still;
   for (auto* c : s->active) {
-    if (c->left == 0 && hc_busy(c->ctx) == 0) {
-      if ((st = retire(s, *c)) != HC_OK) return st;
+    const int busy = c->inject == HC_SERVER_INJECT_STREAM_ERROR ? -1 : hc_busy(c->ctx);
+    if (busy < 0) {
+      const std::string m = c->inject == HC_SERVER_INJECT_STREAM_ERROR ? "injected stream error" : hc_last_error(c->ctx);
+      fail_case(c, HC_DEVICE, m);
+    } else if (c->left == 0 && busy == 0) {
+      if ((st = retire(s, *c)) != HC_OK) fail_case(c, st, s->err);
     } else {
       still.push_back(c);
     }
@@ -219,54 +240,49 @@ hc_status hc_server_poll(hc_server* s, int chunk, int* n_running) {
     hc_server::Case* c = s->queued.front();
     s->queued.pop_front();
     if ((st = build(s, *c)) != HC_OK) {
-      const std::string m = c->ctx ? hc_last_error(c->ctx) : s->err;
-      if (c->ctx) hc_destroy(c->ctx);
-      c->ctx = nullptr;
-      c->status = HC_CASE_FAILED;
-      return serr(s, st, "case " + std::to_string(c->id) + ": " + m);
+      fail_case(c, st, c->ctx ? hc_last_error(c->ctx) : s->err);
+      continue;
     }
     c->status = HC_CASE_RUNNING;
     used += c->bytes;
     s->active.push_back(c);
   }
-  // one chunk for every case with steps left, one batch per precision
-  int64_t n = chunk;
+  // every case with steps left takes min(chunk, its steps left): one batch per
+  // precision and step count (a case shorter than the chunk ends in its own
+  // batch instead of shortening everybody's)
+  std::map<std::pair<int, int64_t>, std::vector<hc_server::Case*>> groups;
   for (auto* c : s->active)
-    if (c->left > 0) n = std::min(n, c->left);
-  hc_status first = HC_OK;
-  for (int prec : {(int)HC_FP64, (int)HC_FP32_SHIFTED}) {
-    std::vector<hc_server::Case*> group;
+    if (c->left > 0) groups[{(int)c->spec.precision, std::min<int64_t>(chunk, c->left)}].push_back(c);
+  for (auto& kv : groups) {
+    const int64_t n = kv.first.second;
     std::vector<hc_ctx*> ctxs;
-    for (auto* c : s->active)
-      if (c->left > 0 && c->spec.precision == prec) {
-        group.push_back(c);
-        ctxs.push_back(c->ctx);
-      }
-    if (ctxs.empty()) continue;
+    for (auto* c : kv.second) ctxs.push_back(c->ctx);
     if ((st = hc_step_batch(ctxs.data(), (int)ctxs.size(), (int)n)) == HC_OK) {
-      for (auto* c : group) c->left -= n;
+      for (auto* c : kv.second) c->left -= n;
       continue;
     }
     // a batch that failed leaves its cases in an unknown state: they fail, the
-    // cases of the other precision keep their steps, and the error is reported
+    // other batches keep their steps
     std::string m;
     for (hc_ctx* x : ctxs)
       if (*hc_last_error(x)) m = hc_last_error(x);
-    if (first == HC_OK) {
-      first = st;
-      s->err = m;
-    }
-    for (auto* c : group) {
-      hc_destroy(c->ctx);
-      c->ctx = nullptr;
-      c->status = HC_CASE_FAILED;
-    }
-    s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
-                                   [](hc_server::Case* c) { return c->status == HC_CASE_FAILED; }),
-                    s->active.end());
+    for (auto* c : kv.second) fail_case(c, st, m);
   }
+  s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
+                                 [](hc_server::Case* c) { return c->status == HC_CASE_FAILED; }),
+                  s->active.end());
   if (n_running) *n_running = (int)(s->active.size() + s->queued.size());
   return first;
+}
+
+hc_status hc_server_inject_failure(hc_server* s, int64_t case_id, int what) {
+  if (!s || (what != HC_SERVER_INJECT_NONE && what != HC_SERVER_INJECT_RETIRE &&
+             what != HC_SERVER_INJECT_STREAM_ERROR))
+    return HC_INVALID_ARG;
+  auto it = s->cases.find(case_id);
+  if (it == s->cases.end()) return serr(s, HC_INVALID_ARG, "unknown case");
+  it->second->inject = what;
+  return HC_OK;
 }
 
 int hc_case_status(const hc_server* s, int64_t case_id) {

@@ -282,6 +282,7 @@ struct MeshMetrics {
   T volume = 0, area = 0, mean_edge = 0;
   T getVolume() const { return volume; }
   T getArea() const { return area; }
+  T getSurface() const { return area; }  // meshMetrics.h:90, the total area
 };
 
 class HemoCellFields;

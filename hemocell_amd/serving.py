@@ -38,6 +38,7 @@ class Case:
     started: float = math.nan
     finished: float = math.nan
     result: dict = field(default_factory=dict)
+    error: str = ""          # why the library failed the case (serve_native)
 
     @property
     def latency(self):
@@ -163,12 +164,15 @@ def serve(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SHIFTED, 
         idle = True
         stepping = [a for a in active if a[2] > 0]
         if batched and stepping:
-            # every active case advances by the same chunk, ending exactly at the
-            # first case's last step
-            n = min(chunk, min(a[2] for a in stepping))
-            hc.step_batch([a[1] for a in stepping], n)
+            # every active case advances by min(chunk, its steps left): one batch
+            # per step count, so a case about to finish does not shorten the others
+            by_n = {}
             for a in stepping:
-                a[2] -= n
+                by_n.setdefault(min(chunk, a[2]), []).append(a)
+            for n, group in by_n.items():
+                hc.step_batch([a[1] for a in group], n)
+                for a in group:
+                    a[2] -= n
             idle = False
         for a in ([] if batched else active):
             c, sim, left = a
@@ -215,33 +219,50 @@ def spec(case, units):
 
 
 def serve_native(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SHIFTED, keep_vertices=False,
-                 hbm_budget=64 << 30):
+                 hbm_budget=64 << 30, spec_of=None):
     """The same stream through the library's own request loop (hc_server_*):
-    arrivals are submitted as they come, every poll retires, admits and steps."""
+    arrivals are submitted as they come, every poll retires, admits and steps.
+    A case the library fails (set-up, batch, stream or read-back error) ends
+    with c.error set and no result; the other cases go on.  Returns the
+    finished cases (failed ones included) and the wall time.  spec_of(case,
+    units) gives a case's submission arguments (default: spec)."""
+    spec_of = spec_of or spec
     server = hc.Server(0, max_active, hbm_budget)
     pending = sorted(cases, key=lambda c: c.arrival)
     ids = {}
     done = []
+    last_error = ""
     t0 = time.perf_counter()
     k = 0
     try:
-        while k < len(pending) or len(done) < len(pending):
+        while k < len(pending) or ids:
             now = time.perf_counter() - t0
             while k < len(pending) and pending[k].arrival <= now:
                 c = pending[k]
                 k += 1
                 c.started = now
-                ids[c.ident] = (c, server.submit(precision=precision, **spec(c, units)))
-            server.poll(chunk)
+                ids[c.ident] = (c, server.submit(precision=precision, **spec_of(c, units)))
+            try:
+                server.poll(chunk)
+            except hc.HcError as e:
+                # the failed cases read CASE_FAILED below; the others keep their steps
+                last_error = str(e)
             for ident, (c, cid) in list(ids.items()):
-                if server.status(cid) == hc.CASE_DONE:
+                st = server.status(cid)
+                if st == hc.CASE_DONE:
                     c.finished = time.perf_counter() - t0
-                    x, st = server.result(cid)
-                    c.result = dict(volume=float(st["volume"]), stretch=float(st["stretch"]))
+                    x, cs = server.result(cid)
+                    c.result = dict(volume=float(cs["volume"]), stretch=float(cs["stretch"]))
                     if keep_vertices:
                         c.result["x"] = x
-                    done.append(c)
-                    del ids[ident]
+                elif st == hc.CASE_FAILED:
+                    c.finished = time.perf_counter() - t0
+                    c.result = None
+                    c.error = last_error
+                else:
+                    continue
+                done.append(c)
+                del ids[ident]
             if not ids and k < len(pending):
                 time.sleep(max(0.0, min(0.01, pending[k].arrival - (time.perf_counter() - t0))))
     finally:
@@ -250,11 +271,14 @@ def serve_native(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SH
 
 
 def latency_stats(done, wall):
-    lat = np.array([c.latency for c in done])
-    return dict(cases=len(done), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                mean_s=float(lat.mean()), cases_per_s=len(done) / wall,
-                steps_per_s=float(sum(c.steps for c in done) / wall),
-                lups_per_s=float(sum(c.steps * c.edge ** 3 for c in done) / wall))
+    """p50/p99/mean latency and throughput of the cases that finished; failed
+    cases (serve_native, result None) are counted apart."""
+    ok = [c for c in done if c.result is not None]
+    lat = np.array([c.latency for c in ok])
+    return dict(cases=len(ok), failed=len(done) - len(ok), p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()), cases_per_s=len(ok) / wall,
+                steps_per_s=float(sum(c.steps for c in ok) / wall),
+                lups_per_s=float(sum(c.steps * c.edge ** 3 for c in ok) / wall))
 
 
 def main(argv=None):

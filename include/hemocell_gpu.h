@@ -231,11 +231,19 @@ hc_status hc_server_create(int device, int max_active, double hbm_budget_bytes, 
 void hc_server_destroy(hc_server* s);
 const char* hc_server_last_error(const hc_server* s);
 hc_status hc_case_submit(hc_server* s, const hc_case_spec* spec, int64_t* case_id);
-/* retire drained cases, admit, step every running case by up to `chunk` steps
- * (asynchronous); n_running (may be NULL) receives the cases not yet done.  A case
- * whose set-up or batch fails is marked HC_CASE_FAILED and its context released; the
- * error is returned (hc_server_last_error) and the other cases go on at the next poll. */
+/* retire drained cases, admit, step every running case by min(chunk, its steps left)
+ * (asynchronous; one batch per precision and step count); n_running (may be NULL)
+ * receives the cases neither done nor failed.  A case whose set-up, batch, stream or
+ * result read-back fails is marked HC_CASE_FAILED and its context released; the other
+ * cases go on in the same pass, and the first error of the pass is returned
+ * (hc_server_last_error). */
 hc_status hc_server_poll(hc_server* s, int chunk, int* n_running);
+/* Test hook, no counterpart in the reference: makes a case fail from the host at its
+ * next poll, as a failed result read-back at retirement (HC_SERVER_INJECT_RETIRE) or as
+ * an error reported by its stream (HC_SERVER_INJECT_STREAM_ERROR), so the failure paths
+ * of hc_server_poll can be exercised without a device error. */
+enum { HC_SERVER_INJECT_NONE = 0, HC_SERVER_INJECT_RETIRE = 1, HC_SERVER_INJECT_STREAM_ERROR = 2 };
+hc_status hc_server_inject_failure(hc_server* s, int64_t case_id, int what);
 /* HC_CASE_QUEUED / RUNNING / DONE / FAILED, -1 for an unknown id */
 int hc_case_status(const hc_server* s, int64_t case_id);
 /* a finished case's final vertex positions (nv*3) and stats {volume, area, largest

@@ -274,6 +274,14 @@ void adjacent_triangles(const CellType& ct, const std::vector<std::vector<int>>&
   if (n != 2) throw std::runtime_error("mesh is not a closed 2-manifold (edge not shared by exactly 2 triangles)");
 }
 
+// Whether triangle t holds the directed edge a->b in its cyclic vertex order.
+bool directed_edge(const CellType& ct, int t, int a, int b) {
+  const int* T = &ct.tri[3 * t];
+  for (int k = 0; k < 3; ++k)
+    if (T[k] == a && T[(k + 1) % 3] == b) return true;
+  return false;
+}
+
 void build_tables(CellType& ct) {
   const double* X = ct.rest.data();
   auto P = [&](int v) { return X + 3 * v; };
@@ -296,6 +304,12 @@ void build_tables(CellType& ct) {
     ct.edge_len_eq[e] = norm(sub(P(b), P(a)));
     int adj[2];
     adjacent_triangles(ct, vt, a, b, adj);
+    // the pair is ordered by orientation: the first triangle holds the directed
+    // edge b->a, the second a->b (the one the edge list took the edge from).
+    // With outward triangles this makes the platelet's dihedral bending force
+    // (pltSimpleModel.cpp:160-178) oppose a fold of the edge; Palabos's
+    // getAdjacentTriangleIds order is not available to check against.
+    if (directed_edge(ct, adj[0], a, b)) std::swap(adj[0], adj[1]);
     ct.edge_tris[2 * e] = adj[0];
     ct.edge_tris[2 * e + 1] = adj[1];
     const int* T0 = &ct.tri[3 * adj[0]];

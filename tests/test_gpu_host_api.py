@@ -10,6 +10,20 @@
   zero-velocity walls; the axial and transverse diameters must fall in the
   bands of tests/validation/stretch_cell/test_stretch_cell.cpp:162-165 and the
   volume stay within 2% (:145-146).
+* stretch_cell_ci: the case the reference's continuous integration runs,
+  examples/stretchCell/stretchCell.cpp with scripts/ci/config-stretchCell.xml
+  (137 pN, dt 0.5e-7 s, 1000 steps, measured at iteration 1 and every 100),
+  checked against the bounds of scripts/ci/stretchCell_sanity.sh:7-34 at every
+  measurement of the stretched cell (iterations 100-1000): largest diameter
+  <= 9.6 um, volume in [81.12, 81.19] um^3 and in [100, 100.1] % of the rest
+  mesh, surface in [129.34, 133.04] um^2.  These bands fix the size of the area
+  and volume forces (a wrong constant or area weight moves the volume or the
+  surface out of them).  At iteration 1 the cell is still the rest mesh
+  (volume and surface within 1e-4 of the rest values): this build's rest RBC
+  has 129.21 um^2 (volume 81.116 um^3, RBC_template.xml:16), below the
+  script's surface floor, which the reference's Palabos-built mesh meets
+  before any stretching; the script's volume lines cut field 3 of a line with
+  one colon and so never fail there.
 """
 import json
 import os
@@ -89,6 +103,36 @@ def test_stretch_cell_reference_bands(force, transverse, axial):
     assert transverse[0] <= res["transverse_um"] <= transverse[1]
     assert axial[0] <= res["axial_um"] <= axial[1]
     assert 0.98 < res["volume_ratio"] <= 1.02
+
+
+def test_stretch_cell_ci_bounds(tmp_path):
+    import shutil
+    shutil.copy(os.path.join(GOLDEN, "stretch_cell_ci", "config-stretchCell.xml"), tmp_path / "config.xml")
+    # examples/stretchCell/RBC.xml and RBC.pos are the validation's stretch_RBC files
+    shutil.copy(os.path.join(GOLDEN, "stretch_cell", "stretch_RBC.xml"), tmp_path / "RBC.xml")
+    shutil.copy(os.path.join(GOLDEN, "stretch_cell", "stretch_RBC.pos"), tmp_path / "RBC.pos")
+    args = [os.path.join(BIN, "stretch_cell_ci"), "config.xml"]
+    print("[gpu-host]", " ".join(args), flush=True)
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    head, meas = lines[0], lines[1:]
+    print(head)
+    for m in meas:
+        print(m)
+    assert head["lattice"] == [52, 26, 26] and head["force_pN"] == 137
+    assert [m["iteration"] for m in meas] == [1] + list(range(100, 1001, 100))
+    for m in meas:
+        assert m["finite"] == 1 and m["wall_hits"] == 0
+        assert m["largest_diameter_um"] <= 9.6, m
+    first = meas[0]
+    assert abs(first["volume_pct"] - 100.0) < 1e-2 and abs(first["surface_pct"] - 100.0) < 1e-2
+    for m in meas[1:]:
+        assert 100.0 <= m["volume_pct"] <= 100.1, m
+        assert 81.12 <= m["volume_um3"] <= 81.19, m
+        assert 129.34 <= m["surface_um2"] <= 133.04, m
+    # the pull has stretched the cell along x by the end
+    assert meas[-1]["diameters_um"][0] > meas[0]["diameters_um"][0] + 0.1
 
 
 def _h5(path, what):

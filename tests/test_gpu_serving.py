@@ -98,3 +98,63 @@ def test_native_server_failed_case_does_not_stop_the_others():
     x, st = s.result(good)
     assert np.isfinite(x).all() and st["volume"] > 0
     s.close()
+
+
+def test_native_server_retire_and_stream_failures():
+    """Failures forced from the host (no device error): a case whose result
+    read-back fails at retirement and a case whose stream reports an error are
+    marked FAILED and release their slots; poll reports each error once, the
+    third case finishes with the result it has alone, and the loop drains."""
+    print("[serving] native injected failures", flush=True)
+    u = hc.lbm_base_parameters(5e-7, 1e-7)
+    stream = serving.make_cases(3, rate=1.0, seed=13, edge=(24, 28), steps=(60, 90))
+    s = hc.Server(0, 3, 1 << 30)
+    ids = [s.submit(**serving.spec(c, u)) for c in stream]
+    assert s.poll(20) == 3  # all three admitted and stepping
+    s.inject_failure(ids[1], hc.INJECT_RETIRE)
+    s.inject_failure(ids[2], hc.INJECT_STREAM_ERROR)
+    errors = []
+    running, t0 = 3, time.time()
+    while running and time.time() - t0 < 60:
+        try:
+            running = s.poll(20)
+        except hc.HcError as e:
+            errors.append(str(e))
+            running = sum(s.status(i) in (hc.CASE_QUEUED, hc.CASE_RUNNING) for i in ids)
+        time.sleep(1e-3)
+    assert running == 0
+    assert [s.status(i) for i in ids] == [hc.CASE_DONE, hc.CASE_FAILED, hc.CASE_FAILED]
+    assert len(errors) == 2 and any("stream" in e for e in errors) and any("retire" in e for e in errors)
+    with pytest.raises(hc.HcError):
+        s.result(ids[1])
+    x, st = s.result(ids[0])
+    alone, xa = serving.run_alone(stream[0], u)
+    assert cases.rel_l2(x, xa) < 1e-9
+    s.close()
+
+
+def test_serve_native_keeps_serving_past_a_failed_case():
+    """serve_native with one case whose mesh the library rejects at admission:
+    that case ends failed with its error, every other case of the stream
+    finishes, and the latency statistics count it apart."""
+    print("[serving] serve_native with a failed case", flush=True)
+    u = hc.lbm_base_parameters(5e-7, 1e-7)
+    stream = serving.make_cases(4, rate=500.0, seed=17, edge=(24, 30), steps=(80, 160))
+
+    def spec_of(c, units):
+        kw = serving.spec(c, units)
+        if c.ident == 1:
+            tris = np.array(kw["triangles"], dtype=np.int32).copy()
+            tris.flat[0] = len(kw["vertices"]) + 7  # a vertex index past the mesh
+            kw["triangles"] = tris
+        return kw
+
+    done, wall = serving.serve_native(stream, u, max_active=4, chunk=40, spec_of=spec_of)
+    assert sorted(c.ident for c in done) == [0, 1, 2, 3]
+    bad = [c for c in done if c.ident == 1][0]
+    assert bad.result is None and bad.error
+    for c in done:
+        if c.ident != 1:
+            assert c.result and c.result["volume"] > 0 and not c.error
+    st = serving.latency_stats(done, wall)
+    assert st["cases"] == 3 and st["failed"] == 1
