@@ -121,6 +121,7 @@ SIGNATURES = {
     "hc_case_status": (_INT, [_P, ctypes.c_int64]),
     "hc_case_result": (_ST, [_P, ctypes.c_int64, _D, _D]),
     "hc_server_inject_failure": (_ST, [_P, ctypes.c_int64, _INT]),
+    "hc_get_cell_stats_noalloc": (_ST, [_P, _D, ctypes.c_int64]),
     "hc_set_periodicity": (_ST, [_P, ctypes.POINTER(_INT)]),
     "hc_set_lees_edwards": (_ST, [_P, ctypes.c_double, ctypes.c_double]),
     "hc_stage_lees_edwards": (_ST, [_P]),
@@ -201,6 +202,7 @@ def build_mesh(shape, radius_lu, min_triangles, aspect_ratio=0.3):
 CASE_SHEAR, CASE_STRETCH = 0, 1
 CASE_QUEUED, CASE_RUNNING, CASE_DONE, CASE_FAILED = 0, 1, 2, 3
 INJECT_NONE, INJECT_RETIRE, INJECT_STREAM_ERROR = 0, 1, 2
+CELL_STATS_QUANTITIES, CELL_STATS_NOALLOC_MAX = 15, 262144
 
 
 class Server:
@@ -504,6 +506,19 @@ class Simulation:
                    bbox=np.zeros((nc, 6)), stretch=np.zeros(nc))
         self._chk(self.lib.hc_get_cell_stats(self.h, *[_dp(out[k]) for k in
                                                        ("volume", "area", "centroid", "velocity", "bbox", "stretch")]))
+        return out
+
+    def cell_stats_noalloc(self, out=None):
+        """hc_get_cell_stats_noalloc: the statistics of cell_stats as one (ncells, 15)
+        array (volume, area, centroid[3], velocity[3], bbox[6], stretch), staged on the
+        device without allocating; `out` may be a preallocated C-contiguous array."""
+        nc = self.lib.hc_num_cells(self.h)
+        if out is None:
+            out = np.zeros((nc, CELL_STATS_QUANTITIES))
+        if out.dtype != np.float64 or not out.flags["C_CONTIGUOUS"] or out.ndim != 2 or \
+                out.shape[1] != CELL_STATS_QUANTITIES:
+            raise ValueError("out must be a C-contiguous float64 array of shape (cells, 15)")
+        self._chk(self.lib.hc_get_cell_stats_noalloc(self.h, _dp(out), out.shape[0]))
         return out
 
     def force_terms(self):

@@ -47,12 +47,14 @@ struct Prof {
 
 struct hc_ctx {
   int device = 0;
+  bool noalloc_ok = false;  // the statistics staging array of this device was found
   hipStream_t stream = nullptr;
   // the cell kernels of different types run side by side: two more streams that
   // fork from and join back into the context stream within each step
   hipStream_t side[2] = {nullptr, nullptr};
   hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
   std::string err;
+  const char* err_fixed = nullptr;  // a static message set without allocating (takes precedence)
   // lattice
   bool lattice_ready = false;
   LatticeDev L{};
@@ -153,7 +155,15 @@ struct hc_ctx {
 namespace {
 
 hc_status fail(hc_ctx* c, hc_status s, const std::string& m) {
-  if (c) c->err = m;
+  if (c) {
+    c->err = m;
+    c->err_fixed = nullptr;
+  }
+  return s;
+}
+// an error with a static message: sets nothing that allocates
+hc_status fail_fixed(hc_ctx* c, hc_status s, const char* m) {
+  if (c) c->err_fixed = m;
   return s;
 }
 hc_status hipfail(hc_ctx* c, hipError_t e, const char* where) {
@@ -1168,6 +1178,9 @@ hc_status hc_create(int device, hc_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return HC_DEVICE;
   hc_ctx* ctx = new hc_ctx();
   ctx->device = device;
+  // the statistics staging array of hc_get_cell_stats_noalloc on this device
+  // (looked up once per device; a failure disables only that entry point)
+  ctx->noalloc_ok = noalloc_scratch_setup(device) == hipSuccess;
   if (hipStreamCreate(&ctx->stream) != hipSuccess) {
     delete ctx;
     return HC_DEVICE;
@@ -1258,7 +1271,10 @@ void hc_destroy(hc_ctx* ctx) {
   delete ctx;
 }
 
-const char* hc_last_error(const hc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* hc_last_error(const hc_ctx* ctx) {
+  if (!ctx) return "null context";
+  return ctx->err_fixed ? ctx->err_fixed : ctx->err.c_str();
+}
 
 hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* node_type, double tau,
                           const int periodic[3], int precision) {
@@ -2012,6 +2028,38 @@ hc_status hc_get_cell_stats(hc_ctx* ctx, double* volume, double* area, double* c
       for (int k = 0; k < 6; ++k) bbox[6 * c + k] = q[8 + k];
     if (stretch) stretch[c] = q[14];
   }
+  return HC_OK;
+}
+
+hc_status hc_get_cell_stats_noalloc(hc_ctx* ctx, double* stats, int64_t max_cells) {
+  // no allocation on any path: the staging array is static, errors carry
+  // static messages
+  if (!ctx) return HC_INVALID_ARG;
+  if (!stats || max_cells < 0) return fail_fixed(ctx, HC_INVALID_ARG, "hc_get_cell_stats_noalloc: null output");
+  if (ctx->cells_dirty)
+    return fail_fixed(ctx, HC_STATE,
+                      "hc_get_cell_stats_noalloc: cells were added since the last step or statistics call and their "
+                      "device tables are not built (building them allocates): call hc_step or hc_get_cell_stats "
+                      "first");
+  const int64_t nc = (int64_t)ctx->cell_base.size();
+  if (nc > (int64_t)HC_CELL_STATS_NOALLOC_MAX)
+    return fail_fixed(ctx, HC_INVALID_ARG,
+                      "hc_get_cell_stats_noalloc: more cells than HC_CELL_STATS_NOALLOC_MAX (262144); use "
+                      "hc_get_cell_stats");
+  if (nc > max_cells)
+    return fail_fixed(ctx, HC_INVALID_ARG, "hc_get_cell_stats_noalloc: the output holds fewer cells than the context");
+  if (nc == 0) return HC_OK;
+  double* d = ctx->noalloc_ok ? noalloc_scratch(ctx->device) : nullptr;
+  if (!d) return fail_fixed(ctx, HC_DEVICE, "hc_get_cell_stats_noalloc: no staging array on this device");
+  hipError_t e = hipSuccess;
+  for (auto& tp : ctx->types)
+    if (e == hipSuccess && tp->ncells)
+      e = launch_cell_stats(ctx->stream, vdev(ctx), tp->dev, tp->d_cells, tp->ncells, ctx->d_cell_base, d);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(stats, d, sizeof(double) * HC_CELL_STATS_QUANTITIES * (size_t)nc, hipMemcpyDeviceToHost,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail_fixed(ctx, HC_DEVICE, "hc_get_cell_stats_noalloc: a HIP call failed");
   return HC_OK;
 }
 

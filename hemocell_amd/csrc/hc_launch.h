@@ -163,6 +163,10 @@ hipError_t launch_mechanics(hipStream_t s, const VertexDev& V, const TypeDev& T,
                             const int64_t* cell_base, double* bend_scratch, int lds_limit, int term = 63, double* out = nullptr);
 hipError_t launch_cell_stats(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                              const int64_t* cell_base, double* out);
+// the device address of the statistics staging array of hc_get_cell_stats_noalloc:
+// looked up once per device (that device current) at context creation
+hipError_t noalloc_scratch_setup(int device);
+double* noalloc_scratch(int device);
 hipError_t launch_check_finite(hipStream_t s, const double* p, int64_t n, unsigned int* bad);
 hipError_t launch_check_finite_f(hipStream_t s, const float* p, int64_t n, unsigned int* bad);
 

@@ -18,6 +18,7 @@
 // scratch array and read back after a barrier.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "hc_mech_dev.h"
 
@@ -270,6 +271,41 @@ hipError_t launch_check_finite_f(hipStream_t s, const float* p, int64_t n, unsig
   if (b > 8192) b = 8192;
   hipLaunchKernelGGL(k_check_finite_f, dim3((unsigned)b), dim3(256), 0, s, p, n, bad);
   return hipGetLastError();
+}
+
+
+// ---- statistics staging without allocation -----------------------------------
+// hc_get_cell_stats_noalloc stages its per-cell statistics (15 doubles per cell,
+// the layout k_cell_stats writes) in this file-scope array instead of a buffer
+// allocated per call: HC_CELL_STATS_NOALLOC_MAX cells, 31.5 MB on each device
+// the library runs on, held until the process ends.  In host code the array's
+// name is not a device address: its address on each device is looked up once
+// (noalloc_scratch_setup, with that device current) and only that pointer is
+// used.  Every call on one device shares the array, so calls that overlap on a
+// device race; each call overwrites the slots of all its cells before reading.
+static __device__ __align__(256) double scratch[HC_CELL_STATS_QUANTITIES * (size_t)HC_CELL_STATS_NOALLOC_MAX];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return hipSuccess;
+}
+
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 }  // namespace hcg

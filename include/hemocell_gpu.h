@@ -315,6 +315,21 @@ hc_status hc_get_cell_volume_area(hc_ctx* ctx, double* volume, double* area);
  * distance ("stretch"); lattice units; any pointer may be NULL. */
 hc_status hc_get_cell_stats(hc_ctx* ctx, double* volume, double* area, double* centroid,
                             double* velocity, double* bbox, double* stretch);
+/* No counterpart in the reference: hc_get_cell_stats for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  Allocates nothing, on the device or the host: the statistics are staged
+ * in a static device array sized for HC_CELL_STATS_NOALLOC_MAX cells (31.5 MB on each
+ * device the library runs on, held until the process ends) and copied into
+ * stats[HC_CELL_STATS_QUANTITIES * c + q], q = 0 volume, 1 area, 2-4 centroid, 5-7 mean
+ * velocity, 8-13 bounding box (xmin, xmax, ymin, ymax, zmin, zmax), 14 largest
+ * vertex-vertex distance, the values hc_get_cell_stats returns.  max_cells: the cells
+ * stats holds.  HC_INVALID_ARG for more cells than HC_CELL_STATS_NOALLOC_MAX or than
+ * max_cells; HC_STATE when cells were added since the last step or statistics call
+ * (their device tables are built by those calls, which allocate).  Calls on contexts of
+ * one device share the staging array: calls that overlap on a device race. */
+#define HC_CELL_STATS_QUANTITIES 15
+#define HC_CELL_STATS_NOALLOC_MAX 262144
+hc_status hc_get_cell_stats_noalloc(hc_ctx* ctx, double* stats, int64_t max_cells);
 /* Replaces: the per-term force vectors of HemoCellParticleField::separateForceVectors
  * (core/hemoCellParticleField.cpp:590-614) that writeOutput recomputes with
  * applyConstitutiveModel(true): the area, volume, bending, link, membrane-viscosity and
