@@ -2,16 +2,21 @@
 
 Workload (the headline "Ht 30%" configuration): a 256x256x512 periodic D3Q19 box,
 body force along z, 30% hematocrit of 642-vertex RBCs, 66-vertex platelets
-(1 per 10 RBC) and 2562-vertex WBCs (1 per 500 RBC), seeded non-overlapping
-placement; dx = 0.5 um, dt = 1e-7 s (tau = 1.82), reference default stiffness.
+(1 per 10 RBC) and 2562-vertex WBCs (1 per 500 RBC) at seeded uniform random
+positions and orientations, no two cells overlapping (the committed packing
+hemocell_amd/data/suspension_256x256x512_ht30.npz, written by
+`python -m hemocell_amd.packing`); dx = 0.5 um, dt = 1e-7 s (tau = 1.82),
+reference default stiffness.
 
 One step = HemoCell::iterate(): spread, collide-stream, interpolate + advance,
 mechanics, force reset.  The timed region runs K steps back to back with inputs
-resident in HBM; the device stream is synchronised on both sides.  `value` is
-MLUPS (fluid nodes x steps / s) of the FP32-shifted production form; the FP64
-form is measured in the same run.  The collide-stream kernel's mean duration is
-taken from HIP events recorded on the library's stream around every launch in
-the timed region.
+resident in HBM, through the production path (steady steps replayed from the
+captured HIP graph); the device stream is synchronised on both sides.  `value`
+is MLUPS (fluid nodes x steps / s) of the FP32-shifted production form; the
+FP64 form is measured in the same run.  A second pass of K steps, launched
+eagerly with HIP events recorded on the library's stream around every kernel
+launch, gives the per-stage times and the collide-stream kernel's mean duration
+(the roofline figure).
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision both|fp32|fp64] [--no-cpu]
 """
@@ -28,9 +33,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS & % HBM roofline, FP32-shifted vs FP64 at Ht 30%; p50/p99 case latency"
-# algorithmic bytes per fluid-node update of the collide-stream kernel:
-# 19 population reads + 19 writes + 3 IBM-force reads + 3 writes zeroing the other force buffer
-BYTES_PER_LUP = {"fp32": 44 * 4, "fp64": 44 * 8}
+# algorithmic bytes per fluid-node update of the collide-stream kernel: 19 population
+# reads + 19 writes + 3 IBM-force reads (the force reset counted with the read)
+BYTES_PER_LUP = {"fp32": 41 * 4, "fp64": 41 * 8}
+# with the 3 writes that clear the other force buffer, which the kernel also does
+BYTES_PER_LUP_WITH_CLEAR = {"fp32": 44 * 4, "fp64": 44 * 8}
+PACKED = os.path.join(ROOT, "hemocell_amd", "data", "suspension_256x256x512_ht30.npz")
+CHANNEL_TILE = os.path.join(ROOT, "hemocell_amd", "data", "channel_tile_128x512x128_ht35.npz")
+TUBE = os.path.join(ROOT, "hemocell_amd", "data", "tube_256x64x64_ht20.npz")
 
 
 PROFILE = os.path.join(ROOT, "profiles", "step_suspension_256x256x512_ht30.md")
@@ -57,19 +67,45 @@ def profile_traffic(label):
 WORKLOADS = {
     "suspension": dict(shape="256,256,512", hematocrit=0.30, wbc_per_rbc=1.0 / 500, walls=False,
                        text="periodic %dx%dx%d body-force suspension, Ht 30%%: RBC 642v, PLT 66v 1:10, "
-                            "WBC 2562v 1:500, seeded non-overlapping"),
+                            "WBC 2562v 1:500, %s"),
     "channel": dict(shape="512,512,1024", hematocrit=0.35, wbc_per_rbc=0.0, walls=True,
                     text="%dx%dx%d channel, bounce-back walls y = 0 and y = ny-1, periodic x and z, body force "
-                         "along z, Ht 35%%: RBC 642v, PLT 66v 1:10, seeded non-overlapping"),
+                         "along z, Ht 35%%: RBC 642v, PLT 66v 1:10, %s (a 128x512x128 tile, 4 x 8)"),
+}
+PLACEMENT = {
+    "random": "seeded uniform random positions and orientations, non-overlapping (force-biased packing)",
+    "aligned": "seeded non-overlapping slot lattice, RBC disc axes along z (aligned variant)",
 }
 
 
-def build_case(hc, susp, shape, precision, seed=20261202, workload="suspension"):
+def place_cells(susp, types, shape, w, placement, seed=20261202):
+    """{name: [n, nv, 3]} for the workload: the committed random packing of the
+    headline box, a random packing made now for other shapes, or the aligned slot
+    lattice."""
+    from hemocell_amd import packing
+    if placement == "aligned":
+        return susp.pack(shape, types, hematocrit=w["hematocrit"], wbc_per_rbc=w["wbc_per_rbc"], seed=seed,
+                         wall_axis=1 if w["walls"] else None)
+    if not w["walls"] and shape == (256, 256, 512) and w["hematocrit"] == 0.30 and os.path.exists(PACKED):
+        pk, _ = packing.load(PACKED)
+    elif w["walls"] and shape == (512, 512, 1024) and os.path.exists(CHANNEL_TILE):
+        # the channel tile (128 x 512 x 128, walls y = 0 and 511, periodic x and z) repeated 4 x 8
+        pk = packing.tile(packing.load(CHANNEL_TILE)[0], (128, 512, 128), (4, 1, 8))
+    else:
+        fluid = float(np.prod(shape)) * ((shape[1] - 2) / shape[1] if w["walls"] else 1.0)
+        counts = susp.cell_counts(types, fluid, w["hematocrit"], 0.1, w["wbc_per_rbc"])
+        print("[bench] packing %s at random positions and orientations (seed %d)" % (counts, seed),
+              file=sys.stderr, flush=True)
+        pk = packing.pack(shape, types, counts, seed=seed, walls=1 if w["walls"] else None,
+                          log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
+    return packing.instances(types, pk)
+
+
+def build_case(hc, susp, shape, precision, seed=20261202, workload="suspension", placement="random"):
     w = WORKLOADS[workload]
     units = hc.lbm_base_parameters(5e-7, 1e-7)
     types = susp.cell_types(units)
-    cells = susp.pack(shape, types, hematocrit=w["hematocrit"], wbc_per_rbc=w["wbc_per_rbc"], seed=seed,
-                      wall_axis=1 if w["walls"] else None)
+    cells = place_cells(susp, types, shape, w, placement, seed)
     sim = hc.Simulation(0)
     if w["walls"]:
         mask = np.zeros(shape, dtype=np.uint8)
@@ -90,11 +126,11 @@ def build_case(hc, susp, shape, precision, seed=20261202, workload="suspension")
     return sim, units, types, cells
 
 
-def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension", autotune=True):
+def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension", autotune=True, placement="random"):
     label = "fp32" if precision == hc.HC_FP32_SHIFTED else "fp64"
     print("[bench] building %s %s case %s" % (label, workload, shape), file=sys.stderr, flush=True)
     t0 = time.time()
-    sim, units, types, cells = build_case(hc, susp, shape, precision, workload=workload)
+    sim, units, types, cells = build_case(hc, susp, shape, precision, workload=workload, placement=placement)
     nv = sim.num_vertices
     x0 = sim.vertices()[0]
     print("[bench] %d cells, %d vertices, Ht %.3f, setup %.1fs" % (
@@ -106,15 +142,23 @@ def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension", au
     if tuning:
         print("[bench] autotune: %s" % json.dumps({k: v["chosen"] for k, v in tuning.items()}), file=sys.stderr,
               flush=True)
-    done_before = sim.iteration
-    sim.profile_enable(True)
+    sim.step(2)  # the step graph is captured here, outside the timed region
     sim.synchronize()
+    done_before = sim.iteration
     t1 = time.perf_counter()
     sim.step(steps)
     sim.synchronize()
     dt = time.perf_counter() - t1
-    # per-step mean of each stage (fused path: "interp" is the velocity field and
+    x1 = sim.vertices()[0]
+    # the same number of steps again, launched eagerly with HIP events around every
+    # kernel launch: per-stage times (fused path: "interp" is the velocity field and
     # "mechanics" the per-cell kernel doing interpolation, mechanics and next spread)
+    sim.profile_enable(True)
+    sim.synchronize()
+    tp = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    dtp = time.perf_counter() - tp
     stages = {}
     for name, k in (("spread", hc.PROF_SPREAD), ("collide_stream", hc.PROF_COLLIDE),
                     ("velocity_field_or_interp", hc.PROF_INTERP), ("cell_step_or_mechanics", hc.PROF_MECH)):
@@ -125,21 +169,40 @@ def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension", au
     sim.profile_enable(False)
     sim.check_finite()
     wall_hits = sim.wall_hits()
-    x1 = sim.vertices()[0]
     N = int(np.prod(shape))
     res = dict(label=label, ms_per_step=1e3 * dt / steps, mlups=N * steps / dt / 1e6,
-               vertex_updates_per_s=nv * steps / dt, stage_ms=stages, vertices=nv, nodes=N,
-               cells={k: int(len(v)) for k, v in cells.items()})
+               ms_per_step_eager=1e3 * dtp / steps, vertex_updates_per_s=nv * steps / dt, stage_ms=stages,
+               vertices=nv, nodes=N, cells={k: int(len(v)) for k, v in cells.items()})
     res["collide_ms"] = collide_ms
     res["tuning"] = {k: v["chosen"] for k, v in tuning.items()} if tuning else None
     res["steps_before"] = done_before
     res["wall_hits"] = int(wall_hits)
     res["x0"], res["x1"] = x0, x1
-    res["collide_gbs"] = BYTES_PER_LUP[label] * N / (stages["collide_stream"] * 1e-3) / 1e9
-    print("[bench] %s: %.3f ms/step, %.1f MLUPS, collide %.3f ms = %.0f GB/s" % (
-        label, res["ms_per_step"], res["mlups"], res["collide_ms"], res["collide_gbs"]), file=sys.stderr, flush=True)
+    res["collide_gbs"] = BYTES_PER_LUP[label] * N / (collide_ms * 1e-3) / 1e9
+    res["collide_gbs_with_clear"] = BYTES_PER_LUP_WITH_CLEAR[label] * N / (collide_ms * 1e-3) / 1e9
+    print("[bench] %s: %.3f ms/step (graph), %.3f eager, %.1f MLUPS, collide %.3f ms = %.0f GB/s" % (
+        label, res["ms_per_step"], res["ms_per_step_eager"], res["mlups"], collide_ms, res["collide_gbs"]),
+        file=sys.stderr, flush=True)
     sim.close()
     return res
+
+
+def copy_ceiling(mib=2048, reps=10):
+    """GB/s of a plain device-to-device copy (read + write) of mib MiB, HIP events."""
+    import torch
+    a = torch.ones(mib * 2 ** 18, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * a.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def serve_stream(n, rate, label="serving"):
@@ -163,7 +226,7 @@ def serve_stream(n, rate, label="serving"):
 
 
 def whole_step(r):
-    """Algorithmic HBM bytes of one step over the step time: collide-stream (176/352 B per
+    """Algorithmic HBM bytes of one step over the step time: collide-stream (164/328 B per
     node), velocity field (22 reads + 4 writes per node) and the vertex state of the cell
     kernel (x read, x, v and f written: 96 B per vertex); the spread's atomic updates are
     not counted."""
@@ -174,36 +237,86 @@ def whole_step(r):
             "cell_kernel_ms": round(r["stage_ms"]["cell_step_or_mechanics"], 4)}
 
 
-def cpu_baseline(hc, susp, seconds=12.0):
-    """FP64 CPU oracle (a restatement of the reference step, OpenMP over nodes and
-    cells) on a 64x64x128 crop of the workload at the same hematocrit."""
+def _oracle_rate(o, nodes, warmup, reps, steps):
+    """Oracle MLUPS: `warmup` untimed steps, then `reps` timed repetitions of `steps`."""
+    o.step(warmup)
+    rates = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o.step(steps)
+        rates.append(nodes * steps / (time.perf_counter() - t0) / 1e6)
+    return rates
+
+
+def cpu_baseline(hc, susp):
+    """The FP64 CPU oracle (a restatement of the reference step, not the reference
+    binary, which cannot be built here: Palabos is not available; OpenMP over nodes
+    and cells on the host's cores) on the single-cell shear box (50^3, one RBC), the
+    Ht 20% tube (256x64x64) and a 128x128x256 crop of the Ht 30% suspension (the
+    cells of the headline packing that lie wholly inside it): 10 warm-up steps, then
+    3 timed repetitions each.  `value` is the crop's median MLUPS."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import OracleSim, library
-    shape = (64, 64, 128)
+    from hemocell_amd import packing
     units = hc.lbm_base_parameters(5e-7, 1e-7)
     types = susp.cell_types(units)
-    cells = susp.pack(shape, types, seed=7)
+    out = {}
+
+    def add(o, cells):
+        o.init_equilibrium(1.0, (0, 0, 0))
+        o.set_force_limit(units.f_limit)
+        for name in ("RBC", "PLT", "WBC"):
+            if name in cells and len(cells[name]):
+                t = types[name]
+                o.add_cells(o.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner), cells[name])
+
+    # C1: 50^3 Couette between velocity walls z = 0, 49 at 100 1/s, one RBC
+    n = 50
+    mask = np.zeros((n, n, n), dtype=np.uint8)
+    mask[:, :, 0] = hc.NODE_VELOCITY + 4
+    mask[:, :, n - 1] = hc.NODE_VELOCITY + 5
+    o = OracleSim(n, n, n, units.tau, periodic=(1, 1, 0), node_type=mask.reshape(-1))
+    uw = (n - 1) * 100.0 * units.dt * 0.5
+    o.set_wall_velocity(4, (uw, 0, 0))
+    o.set_wall_velocity(5, (-uw, 0, 0))
+    v = types["RBC"].verts
+    add(o, {"RBC": (v - v.mean(axis=0) + (n - 1) / 2.0)[None]})
+    out["single_cell_shear_50"] = _oracle_rate(o, n ** 3, 10, 3, 20)
+    del o
+    # C2: the Ht 20% tube
+    L, D = 256, 64
+    fluid = susp.tube_fluid(L, D)
+    o = OracleSim(L, D, D, units.tau, periodic=(1, 0, 0),
+                  node_type=np.where(fluid, 0, hc.NODE_BOUNCEBACK).astype(np.uint8).reshape(-1))
+    R = D / 2.0 - 0.5
+    o.set_body_force((4.0 * units.nu_lbm * (1e-3 * units.dt / units.dx) / (R * R), 0.0, 0.0))
+    add(o, packing.instances(types, packing.load(TUBE)[0]))
+    out["tube_ht20_256x64x64"] = _oracle_rate(o, int(fluid.sum()), 10, 3, 5)
+    del o
+    # 128x128x256 crop of the headline packing
+    shape = (128, 128, 256)
+    full = packing.instances(types, packing.load(PACKED)[0])
+    crop = {}
+    for name, x in full.items():
+        lo, hi = x.min(axis=1), x.max(axis=1)
+        inside = np.all((lo >= 1.0) & (hi <= np.array(shape) - 2.0), axis=1)
+        crop[name] = x[inside]
     o = OracleSim(*shape, units.tau)
-    o.init_equilibrium(1.0, (0, 0, 0))
-    o.set_body_force((0, 0, 1e-6))
-    o.set_force_limit(units.f_limit)
-    for name in ("RBC", "PLT", "WBC"):
-        if name in cells:
-            t = types[name]
-            tid = o.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner)
-            o.add_cells(tid, cells[name])
-    o.step(1)
-    n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds and n < 200:
-        o.step(1)
-        n += 1
-    el = time.perf_counter() - t0
-    threads = library().ho_num_threads()
-    N = int(np.prod(shape))
-    return dict(value=N * n / el / 1e6, unit="MLUPS", cores=int(threads), kind="port",
-                sample="64x64x128 periodic crop at Ht %.2f (%d RBC, %d PLT, %d WBC), %d steps FP64 after 1 warm-up" % (
-                    susp.hematocrit(types, cells, shape), len(cells["RBC"]), len(cells.get("PLT", [])),
-                    len(cells.get("WBC", [])), n))
+    o.set_body_force((0.0, 0.0, 1e-6))
+    add(o, crop)
+    rates = _oracle_rate(o, int(np.prod(shape)), 10, 3, 5)
+    ht = susp.hematocrit(types, crop, shape)
+    del o
+    threads = int(library().ho_num_threads())
+    return dict(value=float(np.median(rates)), unit="MLUPS", cores=threads, kind="port",
+                sample="the FP64 oracle (restatement, not the reference binary) on a 128x128x256 crop of the Ht 30%% "
+                       "suspension (%d RBC, %d PLT, %d WBC wholly inside, Ht %.3f), 10 warm-up steps then 3 x 5 "
+                       "timed steps; median MLUPS" % (len(crop["RBC"]), len(crop["PLT"]), len(crop.get("WBC", [])),
+                                                      ht),
+                repetitions_mlups={"crop_128x128x256": [round(r, 3) for r in rates],
+                                   **{k: [round(r, 3) for r in v] for k, v in out.items()}},
+                host={"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+                      "omp_threads": threads})
 
 
 def main():
@@ -214,11 +327,16 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp32", "fp64"])
     ap.add_argument("--workload", default="suspension", choices=sorted(WORKLOADS),
                     help="suspension: the Ht 30%% periodic box (the headline line); channel: the largest domain")
+    ap.add_argument("--placement", default="random", choices=sorted(PLACEMENT),
+                    help="random: random positions and orientations (the workload as specified); aligned: the "
+                         "slot-lattice variant with RBC disc axes along z")
     ap.add_argument("--shape", default=None, help="nx,ny,nz (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve", type=int, default=32, metavar="N",
-                    help="cases of the serving stream (0 skips the latency measurement)")
-    ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate, cases/s")
+    ap.add_argument("--serve", type=int, default=200, metavar="N",
+                    help="cases of the steady serving stream (0 skips the latency measurement)")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="Poisson arrival rate of the steady stream, cases/s (default: 0.9 x the sustained rate "
+                         "measured on a saturated burst first)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a device-to-device copy of MIB MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -239,13 +357,16 @@ def main():
         print("[bench] calibration copy: %d MiB read + %d MiB written" % (args.copy_calibration, args.copy_calibration),
               file=sys.stderr, flush=True)
         del a, b
+    # the copy ceiling first: torch's runtime initialises the device before the library does
+    ceiling = copy_ceiling()
+    print("[bench] device-to-device copy: %.0f GB/s" % ceiling, file=sys.stderr, flush=True)
     res = {}
     if args.precision in ("both", "fp32"):
         res["fp32"] = run_gpu(hc, susp, shape, hc.HC_FP32_SHIFTED, args.steps, args.warmup, args.workload,
-                              not args.no_autotune)
+                              not args.no_autotune, args.placement)
     if args.precision in ("both", "fp64"):
         res["fp64"] = run_gpu(hc, susp, shape, hc.HC_FP64, args.steps, args.warmup, args.workload,
-                              not args.no_autotune)
+                              not args.no_autotune, args.placement)
     head = res.get("fp32") or res["fp64"]
     traffic = profile_traffic(head["label"])
     if traffic is not None and (shape != (256, 256, 512) or args.workload != "suspension"):
@@ -263,17 +384,22 @@ def main():
         "vs_baseline": None,
         "dtype": head["label"],
         "data": "synthetic",
-        "config": {"workload": wl["text"] % shape,
-                   "cells": head["cells"], "vertices": head["vertices"], "precision": head["label"],
-                   "wall_hits": head["wall_hits"]},
+        "config": {"workload": wl["text"] % (shape + (PLACEMENT[args.placement],)),
+                   "placement": args.placement, "cells": head["cells"], "vertices": head["vertices"],
+                   "precision": head["label"], "wall_hits": head["wall_hits"], "timed_path": "hip graph replay"},
         "roofline": {"bound": "hbm", "achieved": round(head["collide_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["collide_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_collide_stream", "mean_ms": round(head["collide_ms"], 4),
                      "bytes_per_lup": BYTES_PER_LUP[head["label"]],
-                     "algorithmic_bytes": BYTES_PER_LUP[head["label"]] * head["nodes"]},
+                     "algorithmic_bytes": BYTES_PER_LUP[head["label"]] * head["nodes"],
+                     "achieved_with_force_clear": round(head["collide_gbs_with_clear"], 1),
+                     "bytes_per_lup_with_force_clear": BYTES_PER_LUP_WITH_CLEAR[head["label"]],
+                     "copy_ceiling_gbs": round(ceiling, 1),
+                     "frac_of_copy_ceiling": round(head["collide_gbs_with_clear"] / ceiling, 4)},
         "whole_step": whole_step(head),
         "vertex_updates_per_s": round(head["vertex_updates_per_s"], 1),
         "stage_ms": {k: round(v, 4) for k, v in head["stage_ms"].items()},
+        "ms_per_step_eager_profiled": round(head["ms_per_step_eager"], 4),
         "tuning": head["tuning"],
     }
     if "fp64" in res and "fp32" in res:
@@ -288,14 +414,17 @@ def main():
         line["fp64"] = {"value": round(f64["mlups"], 2), "ms_per_step": round(f64["ms_per_step"], 4),
                         "collide_gbs": round(f64["collide_gbs"], 1),
                         "collide_frac": round(f64["collide_gbs"] / HBM_PEAK_GBS, 4),
-                        "stage_ms": {k: round(v, 4) for k, v in f64["stage_ms"].items()}}
+                        "stage_ms": {k: round(v, 4) for k, v in f64["stage_ms"].items()},
+                        "tuning": f64["tuning"]}
         line["fp32_over_fp64"] = round(res["fp32"]["mlups"] / f64["mlups"], 3)
     # the serving stream and the CPU baseline belong to the headline workload
     if args.serve > 0 and args.workload == "suspension":
-        line["serving"] = serve_stream(args.serve, args.serve_rate)
-        # the same kind of stream arriving faster than it can be served: its
-        # cases/s is the sustained rate, with the latencies at that rate
-        line["serving_saturated"] = serve_stream(max(8, (3 * args.serve) // 4), 50.0, "serving (saturated)")
+        # a burst arriving faster than it can be served gives the sustained rate;
+        # the steady stream then arrives at 90% of it
+        sat = serve_stream(24, 50.0, "serving (saturated burst)")
+        line["serving_saturated"] = sat
+        rate = args.serve_rate or round(0.9 * sat["cases_per_s"], 3)
+        line["serving"] = serve_stream(args.serve, rate)
     if not args.no_cpu and args.workload == "suspension":
         line["cpu_baseline"] = cpu_baseline(hc, susp)
     print(json.dumps(line), flush=True)

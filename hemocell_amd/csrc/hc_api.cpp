@@ -92,6 +92,15 @@ struct hc_ctx {
   bool cell_block_env = false;  // the RBC block was set explicitly
   int n_cu = 256;               // compute units of the device
   bool fuse_vel = true;         // velocity-wall nodes inside the collide-stream kernel
+  // every cell type in one persistent launch (instead of one kernel per type side
+  // by side): the queue of (cell, type) entries, most vertices first, the types'
+  // device descriptors, the queue head, and workgroups per compute unit
+  bool mixed = false;
+  int mixed_per_cu = 8;
+  int2* d_queue = nullptr;
+  TypeDev* d_types = nullptr;
+  unsigned int* d_head = nullptr;
+  int nqueue = 0;
   // extra (stretch) forces
   int64_t* d_extra_ids = nullptr;
   double* d_extra_f = nullptr;
@@ -294,6 +303,32 @@ hc_status sync_cells(hc_ctx* ctx) {
       HC_TRY(hipMemcpy(tb.d_cells, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice), "hipMemcpy");
     tb.ncells = (int)list.size();
   }
+  // the persistent launch's queue: types by vertex count, descending
+  if (ctx->d_queue) hipFree(ctx->d_queue);
+  if (ctx->d_types) hipFree(ctx->d_types);
+  ctx->d_queue = nullptr;
+  ctx->d_types = nullptr;
+  ctx->nqueue = 0;
+  if (nc && ctx->types.size() <= (size_t)kMaxMixedTypes) {
+    std::vector<int> order(ctx->types.size());
+    for (size_t t = 0; t < order.size(); ++t) order[t] = (int)t;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return ctx->types[a]->host.nv > ctx->types[b]->host.nv; });
+    std::vector<int2> q;
+    q.reserve(nc);
+    for (int t : order)
+      for (size_t c = 0; c < nc; ++c)
+        if (ctx->cell_type[c] == t) q.push_back(make_int2((int)c, t));
+    std::vector<TypeDev> td(ctx->types.size());
+    for (size_t t = 0; t < td.size(); ++t) td[t] = ctx->types[t]->dev;
+    if ((s = dalloc(ctx, (void**)&ctx->d_queue, sizeof(int2) * q.size(), "cell queue")) != HC_OK) return s;
+    if ((s = dalloc(ctx, (void**)&ctx->d_types, sizeof(TypeDev) * td.size(), "cell type descriptors")) != HC_OK)
+      return s;
+    HC_TRY(hipMemcpy(ctx->d_queue, q.data(), sizeof(int2) * q.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    HC_TRY(hipMemcpy(ctx->d_types, td.data(), sizeof(TypeDev) * td.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!ctx->d_head && (s = dalloc(ctx, (void**)&ctx->d_head, 256, "cell queue head")) != HC_OK) return s;
+    ctx->nqueue = (int)q.size();
+  }
   ctx->cells_dirty = false;
   return HC_OK;
 }
@@ -486,10 +521,45 @@ hc_status do_visc_membrane(hc_ctx* ctx, const double* xold) {
   return HC_OK;
 }
 
+// every type in one persistent launch on the context stream; false when the
+// launch does not apply (no queue, or the layout does not fit the LDS)
+template <typename R>
+bool mixed_cell_step(hc_ctx* ctx, int do_interp, hc_status& st) {
+  st = HC_OK;
+  if (!ctx->mixed || !ctx->d_queue || ctx->nqueue == 0) return false;
+  std::vector<TypeDev> ht(ctx->types.size());
+  std::vector<int> flags(ctx->types.size());
+  for (size_t t = 0; t < ht.size(); ++t) {
+    const TypeBufs& tb = *ctx->types[t];
+    ht[t] = tb.dev;
+    int f = CELL_FLAG_SPREAD;
+    if (do_interp) f |= CELL_FLAG_INTERP;
+    if ((ctx->iter % tb.host.timescale) == 0) f |= CELL_FLAG_MECH;
+    if (ctx->repulsion) f |= CELL_FLAG_REPULSION;
+    flags[t] = f;
+  }
+  const int box = ctx->spread_lds > 0 ? ctx->spread_lds : 12288;
+  const hipError_t e = launch_cell_step_mixed<R>(
+      ctx->stream, vdev(ctx), ctx->d_types, ht.data(), (int)ht.size(), flags.data(), ctx->d_queue, ctx->nqueue,
+      ctx->d_head, ctx->mixed_per_cu * ctx->n_cu, ctx->d_cell_base, (const R*)ctx->ufield, ctx->type, ctx->L,
+      (R*)ctx->Fibm[1 - ctx->cur], ctx->d_extra_dense, ctx->bend, ctx->f_limit, box, ctx->lds_device,
+      ctx->d_wall_hits);
+  if (e == hipErrorInvalidConfiguration) return false;
+  if (e != hipSuccess) st = hipfail(ctx, e, "cell_step (all types)");
+  return true;
+}
+
 template <typename R>
 hc_status do_cell_step(hc_ctx* ctx, int do_interp) {
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_MECH, &ev);
+  {
+    hc_status st;
+    if (mixed_cell_step<R>(ctx, do_interp, st)) {
+      prof_end(ctx, HC_PROF_MECH, ev);
+      return st;
+    }
+  }
   // the type with the most vertices runs on the context stream, the others beside it
   std::vector<TypeBufs*> act;
   for (auto& tp : ctx->types)
@@ -728,6 +798,12 @@ std::vector<unsigned char> launch_signature(const hc_ctx* c) {
   put(v, c->n_cu);
   put(v, c->fuse_vel);
   put(v, c->side_gpos);
+  put(v, c->mixed);
+  put(v, c->mixed_per_cu);
+  put(v, c->d_queue);
+  put(v, c->d_types);
+  put(v, c->d_head);
+  put(v, c->nqueue);
   put(v, c->pv_timescale);
   for (const auto& tp : c->types) {
     put(v, tp->dev);
@@ -1070,19 +1146,27 @@ hc_status autotune_impl(hc_ctx* ctx, int steps, std::string& report) {
     const char* name;
     std::vector<int> values;
     std::function<void(int)> set;
-    bool on;
+    std::function<bool()> on;
   };
+  // the per-type launches side by side first, then the one persistent launch of
+  // all types against them, and its workgroups per compute unit when it wins
+  const bool queue = ctx->d_queue && ntypes > 1;
   std::vector<Param> params = {
-      {"spread_box_bytes", {8192, 12288, 16384, 24576}, [&](int v) { ctx->spread_lds = v; }, true},
-      {"rbc_threads", {256, 512, 1024}, [&](int v) { ctx->cell_block = v; ctx->cell_block_env = true; }, has_rbc},
-      {"side_positions_global", {1, 0}, [&](int v) { ctx->side_gpos = v != 0; }, ntypes > 1 && big_side},
-      {"wbc_threads", {256, 512, 1024}, [&](int v) { ctx->wbc_block = v; }, has_wbc},
+      {"spread_box_bytes", {8192, 12288, 16384, 24576}, [&](int v) { ctx->spread_lds = v; }, [] { return true; }},
+      {"rbc_threads", {256, 512, 1024}, [&](int v) { ctx->cell_block = v; ctx->cell_block_env = true; },
+       [&] { return has_rbc; }},
+      {"side_positions_global", {1, 0}, [&](int v) { ctx->side_gpos = v != 0; },
+       [&] { return ntypes > 1 && big_side; }},
+      {"wbc_threads", {256, 512, 1024}, [&](int v) { ctx->wbc_block = v; }, [&] { return has_wbc; }},
+      {"persistent_launch", {0, 1}, [&](int v) { ctx->mixed = v != 0; }, [&] { return queue; }},
+      {"persistent_workgroups_per_cu", {4, 8, 16}, [&](int v) { ctx->mixed_per_cu = v; },
+       [&] { return ctx->mixed; }},
   };
   std::ostringstream out;
   out << "{";
   bool first_param = true;
   for (auto& p : params) {
-    if (!p.on || s != HC_OK) continue;
+    if (!p.on() || s != HC_OK) continue;
     int best = p.values[0];
     double best_ms = 1e300;
     out << (first_param ? "" : ", ") << "\"" << p.name << "\": {";
@@ -1236,6 +1320,10 @@ hc_status hc_create(int device, hc_ctx** out) {
   if (sl) ctx->spread_lds = std::atoi(sl);
   const char* sg = std::getenv("HC_SIDE_GPOS");  // tuning: 0 stages every type's positions in LDS
   if (sg) ctx->side_gpos = std::atoi(sg) != 0;
+  const char* mx = std::getenv("HC_MIXED");  // tuning: 1 runs every cell type in one persistent launch
+  if (mx) ctx->mixed = std::atoi(mx) != 0;
+  const char* mp = std::getenv("HC_MIXED_PER_CU");  // tuning: its workgroups per compute unit
+  if (mp && std::atoi(mp) > 0) ctx->mixed_per_cu = std::atoi(mp);
   *out = ctx;
   return HC_OK;
 }
@@ -1244,7 +1332,8 @@ void hc_destroy(hc_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  void* ps[] = {ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->d_extra_dense, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
+  void* ps[] = {ctx->d_queue, ctx->d_types, ctx->d_head,
+                ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->d_extra_dense, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
                 ctx->fbr, ctx->d_bflag, ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag,
                 ctx->d_visc, ctx->d_stamp, ctx->d_xold, ctx->d_le, ctx->d_batch[0], ctx->d_batch[1]};
   for (void* p : ps)

@@ -1,4 +1,4 @@
-// hc_cell.hip -- the vertex half of the IB-LBM step, fused per cell.
+// hc_cell_body.h -- the vertex half of the IB-LBM step, fused per cell.
 //
 // One workgroup per cell runs, in the order of HemoCell::iterate()
 // (core/hemoCell.cpp:299-376):
@@ -16,7 +16,13 @@
 // Step t+1's spread needs exactly x_t and F(x_t), which steps 1-2 produce, so
 // doing it here saves reloading the cell.  Positions stay in LDS for the whole
 // kernel; per-vertex bending forces and velocities too when they fit.
+//
+// Instantiated by hc_cell_f64.hip (FP64, compiled without contracting
+// multiply-adds so the FP64 form follows the oracle's operation order) and
+// hc_cell_f32.hip (the FP32-shifted form, which may fuse them).
+#pragma once
 #include <algorithm>
+#include <type_traits>
 
 #include "hc_ibm_dev.h"
 #include "hc_mech_dev.h"
@@ -27,6 +33,19 @@ namespace {
 enum { CELL_INTERP = 1, CELL_MECH = 2, CELL_SPREAD = 4, CELL_REPULSION = 8 };
 // x-planes of a cell's support box up to which the spread buckets its vertices
 constexpr int kMaxPlanes = 64;
+
+// Scalar type of the positions in LDS and of the membrane mechanics: the FP32-
+// shifted form's staged kernels keep float offsets from the cell's first vertex
+// (the mechanics only takes differences of positions; the volume stays FP64);
+// everything else keeps absolute double positions.
+template <typename R, bool STAGED, bool GPOS>
+using MechT = typename std::conditional<std::is_same<R, float>::value && STAGED && !GPOS, float, double>::type;
+
+// LDS bytes before the box region: positions and (STAGED, viscous) velocities of
+// P scalars, rounded up to 16 bytes
+__host__ __device__ constexpr size_t cell_region_offset(int nv, bool visc, size_t psize) {
+  return ((visc ? 6 : 3) * (size_t)nv * psize + 15) & ~(size_t)15;
+}
 
 // STAGED: positions, velocities and bending forces in LDS.  Otherwise positions
 // in LDS and the rest in global memory, or (GPOS) the positions too: then the
@@ -39,6 +58,8 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
                                                const double* __restrict__ extra, double* __restrict__ scratch,
                                                double f_limit, int flags, int box_nodes,
                                                unsigned long long* wall_hits) {
+  using P = MechT<R, STAGED, GPOS>;
+  constexpr bool REL = !std::is_same<P, double>::value;  // LDS holds offsets from the cell origin
   extern __shared__ __align__(16) double sm[];
   __shared__ double red[BS];
   __shared__ double vforce_sh;
@@ -48,30 +69,44 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   const int nv = T.nv;
   const int64_t vs = V.vs;
   const bool visc = (MODEL != HC_MODEL_RBC) || (T.eta_m != 0.0);
+  // origin of the offsets: the first vertex before this step's advance, read by
+  // every thread before any thread moves it
+  double org0 = 0.0, org1 = 0.0, org2 = 0.0;
+  if (REL) {
+    org0 = V.x[base];
+    org1 = V.x[vs + base];
+    org2 = V.x[2 * vs + base];
+    __syncthreads();
+  }
   // LDS: positions, then (STAGED) velocities, then a region shared by the bending
   // forces (phase 2) and the spread box (phase 3)
-  double* px = GPOS ? V.x + base : sm;
-  double* py = GPOS ? V.x + vs + base : sm + nv;
-  double* pz = GPOS ? V.x + 2 * vs + base : sm + 2 * nv;
-  double *vx, *vy, *vz, *bx, *by, *bz;
+  P* const sp_ = reinterpret_cast<P*>(sm);
+  P* px = GPOS ? reinterpret_cast<P*>(V.x + base) : sp_;
+  P* py = GPOS ? reinterpret_cast<P*>(V.x + vs + base) : sp_ + nv;
+  P* pz = GPOS ? reinterpret_cast<P*>(V.x + 2 * vs + base) : sp_ + 2 * nv;
+  P *vx, *vy, *vz, *bx, *by, *bz;
   double* region;
   if (STAGED) {
-    vx = sm + 3 * nv;
-    vy = sm + 4 * nv;
-    vz = sm + 5 * nv;
-    region = sm + (visc ? 6 : 3) * nv;
-    bx = region;
-    by = region + nv;
-    bz = region + 2 * nv;
+    vx = sp_ + 3 * nv;
+    vy = sp_ + 4 * nv;
+    vz = sp_ + 5 * nv;
+    region = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + cell_region_offset(nv, visc, sizeof(P)));
+    bx = reinterpret_cast<P*>(region);
+    by = bx + nv;
+    bz = bx + 2 * nv;
   } else {
-    vx = V.v + base;
-    vy = V.v + vs + base;
-    vz = V.v + 2 * vs + base;
+    vx = reinterpret_cast<P*>(V.v + base);
+    vy = reinterpret_cast<P*>(V.v + vs + base);
+    vz = reinterpret_cast<P*>(V.v + 2 * vs + base);
     region = GPOS ? sm : sm + 3 * nv;
-    bx = scratch + base;
-    by = scratch + vs + base;
-    bz = scratch + 2 * vs + base;
+    bx = reinterpret_cast<P*>(scratch + base);
+    by = reinterpret_cast<P*>(scratch + vs + base);
+    bz = reinterpret_cast<P*>(scratch + 2 * vs + base);
   }
+  // absolute position of vertex k for the spread: LDS (double) or the vertex planes
+  auto pos_x = [&](int k) -> double { return REL ? V.x[base + k] : (double)px[k]; };
+  auto pos_y = [&](int k) -> double { return REL ? V.x[vs + base + k] : (double)py[k]; };
+  auto pos_z = [&](int k) -> double { return REL ? V.x[2 * vs + base + k] : (double)pz[k]; };
   double* box = region;
   // after the box: the spread's vertex list, bucketed by x-plane
   uint16_t* vlist = reinterpret_cast<uint16_t*>(region + 3 * (int64_t)box_nodes);
@@ -133,14 +168,14 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       V.x[vs + i] = x1;
       V.x[2 * vs + i] = x2;
       if (!GPOS) {
-        px[k] = x0;
-        py[k] = x1;
-        pz[k] = x2;
+        px[k] = (P)(x0 - org0);
+        py[k] = (P)(x1 - org1);
+        pz[k] = (P)(x2 - org2);
       }
       if (STAGED && visc) {
-        vx[k] = v0;
-        vy[k] = v1;
-        vz[k] = v2;
+        vx[k] = (P)v0;
+        vy[k] = (P)v1;
+        vz[k] = (P)v2;
       }
       if (type && isfinite(x0) && isfinite(x1) && isfinite(x2)) {
         const double p[3] = {x0, x1, x2};
@@ -159,7 +194,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     }
   }
   __syncthreads();
-  const CellView c{px, py, pz};
+  const CellViewT<P> c{px, py, pz};
 
   // ---- 2. mechanics ------------------------------------------------------------
   if (flags & CELL_MECH) {
@@ -176,20 +211,20 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     }
     if (MODEL != HC_MODEL_PLT) {
       for (int k = tid; k < nv; k += BS) {
-        const D3 b = bending_force(c, T, k);
+        const V3T<P> b = bending_force(c, T, k);
         bx[k] = b.x;
         by[k] = b.y;
         bz[k] = b.z;
       }
     }
     __syncthreads();
-    const double vforce = vforce_sh;
+    const P vforce = (P)vforce_sh;
     for (int k = tid; k < nv; k += BS) {
-      const D3 F = vertex_force<MODEL>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
+      const V3T<P> F = vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
       const int64_t i = base + k;
-      V.f[i] = F.x;
-      V.f[vs + i] = F.y;
-      V.f[2 * vs + i] = F.z;
+      V.f[i] = (double)F.x;
+      V.f[vs + i] = (double)F.y;
+      V.f[2 * vs + i] = (double)F.z;
     }
     __syncthreads();  // forces visible to the whole workgroup; the region is free again
   }
@@ -211,7 +246,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     int lmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, lmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
     bool lbad = false;
     for (int k = tid; k < nv; k += BS) {
-      const double p[3] = {px[k], py[k], pz[k]};
+      const double p[3] = {pos_x(k), pos_y(k), pos_z(k)};
       for (int a = 0; a < 3; ++a) {
         if (!(fabs(p[a]) < 1.0e9)) {
           lbad = true;
@@ -274,7 +309,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       double f0, f1, f2;
       spread_force(i, f0, f1, f2);
       Support sp;
-      ibm_support(px[k], py[k], pz[k], L, type, sp);
+      ibm_support(pos_x(k), pos_y(k), pos_z(k), L, type, sp);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         if (sp.w[q] == 0.0) continue;
@@ -294,7 +329,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   if (bucketed) {
     for (int b = tid; b <= s0; b += BS) pcount[b] = 0;
     __syncthreads();
-    for (int k = tid; k < nv; k += BS) atomicAdd(&pcount[(int)floor(px[k]) - o0], 1);
+    for (int k = tid; k < nv; k += BS) atomicAdd(&pcount[(int)floor(pos_x(k)) - o0], 1);
     __syncthreads();
     if (tid == 0) {
       int acc = 0;
@@ -306,7 +341,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       }
     }
     __syncthreads();
-    for (int k = tid; k < nv; k += BS) vlist[atomicAdd(&pcount[(int)floor(px[k]) - o0], 1)] = (uint16_t)k;
+    for (int k = tid; k < nv; k += BS) vlist[atomicAdd(&pcount[(int)floor(pos_x(k)) - o0], 1)] = (uint16_t)k;
     __syncthreads();
   }
   const int dims[3] = {L.nx, L.ny, L.nz};
@@ -328,7 +363,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       double f0, f1, f2;
       spread_force(base + vtx, f0, f1, f2);
       Support sp;
-      ibm_support(px[vtx], py[vtx], pz[vtx], L, type, sp);
+      ibm_support(pos_x(vtx), pos_y(vtx), pos_z(vtx), L, type, sp);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int lx = sp.lo[0] + (q >> 2) - o0 - xs;
@@ -379,16 +414,50 @@ __global__ __launch_bounds__(BS) void k_cell_step_multi(const CellCase* __restri
                                              c.scratch, c.f_limit, c.flags, box_nodes, c.wall_hits);
 }
 
+// Cells of every type in one persistent launch: each workgroup takes the next
+// entry of the queue (cell, type index; the types with the most vertices first)
+// until the queue is empty.  RBC and platelet cells run the staged body, WBCs
+// the one with positions in global memory; all share the launch's dynamic LDS.
+template <typename R>
+__global__ __launch_bounds__(kMixedBlock) void k_cell_step_mixed(
+    VertexDev V, const TypeDev* __restrict__ types, MixedArgs A, const int2* __restrict__ queue, int nqueue,
+    unsigned int* __restrict__ head, const int64_t* __restrict__ cell_base, const R* __restrict__ uf,
+    const uint8_t* __restrict__ type, LatticeDev L, R* __restrict__ Fnext, const double* __restrict__ extra,
+    double* __restrict__ scratch, double f_limit, unsigned long long* wall_hits) {
+  __shared__ int slot;
+  for (;;) {
+    if (threadIdx.x == 0) slot = (int)atomicAdd(head, 1u);
+    __syncthreads();
+    const int q = slot;
+    __syncthreads();  // every thread has read the slot before thread 0 takes the next one
+    if (q >= nqueue) break;
+    const int2 e = queue[q];
+    const TypeDev& T = types[e.y];
+    const int64_t base = cell_base[e.x];
+    if (T.model == HC_MODEL_RBC)
+      cell_step_body<R, HC_MODEL_RBC, kMixedBlock, true, false>(V, T, base, uf, type, L, Fnext, extra, scratch,
+                                                                 f_limit, A.flags[e.y], A.box_nodes[e.y], wall_hits);
+    else if (T.model == HC_MODEL_PLT)
+      cell_step_body<R, HC_MODEL_PLT, kMixedBlock, true, false>(V, T, base, uf, type, L, Fnext, extra, scratch,
+                                                                 f_limit, A.flags[e.y], A.box_nodes[e.y], wall_hits);
+    else
+      cell_step_body<R, HC_MODEL_WBC, kMixedBlock, false, true>(V, T, base, uf, type, L, Fnext, extra, scratch,
+                                                                 f_limit, A.flags[e.y], A.box_nodes[e.y], wall_hits);
+    __syncthreads();  // the next cell reuses the LDS
+  }
+}
+
 }  // namespace
 
-// Fixed LDS of the kernel (bytes) for a type: positions, plus velocities and bending
-// forces when staged; nothing with the positions in global memory.
-static size_t cell_lds_fixed(const TypeDev& T, bool staged, bool gpos) {
+// Fixed LDS of the kernel (bytes) for a type: positions, plus velocities when
+// staged (psize bytes each: float offsets in the FP32-shifted staged kernels);
+// nothing with the positions in global memory.
+static size_t cell_lds_fixed(const TypeDev& T, bool staged, bool gpos, size_t psize) {
   if (gpos) return 0;
   const bool visc = (T.model != HC_MODEL_RBC) || (T.eta_m != 0.0);
   const size_t nv = (size_t)T.nv;
   if (!staged) return 3 * nv * sizeof(double);
-  return ((visc ? 6 : 3) * nv) * sizeof(double);
+  return cell_region_offset(T.nv, visc, psize);
 }
 
 // Static LDS of one kernel instantiation (the reduction array grows with the block).
@@ -416,14 +485,15 @@ static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& 
                                           static_lds(k_cell_step_multi<R, M, B, false, false>));
   static const size_t st_gpos = static_lds(k_cell_step<R, M, B, false, true>);
   const size_t margin = 256, dev = (size_t)lds_device;
-  const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * sizeof(double) : 0;
+  const size_t psize = sizeof(MechT<R, true, false>);  // the staged kernel's position scalar
+  const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * psize : 0;
   const size_t lists = (2 * (size_t)T.nv + 15) & ~(size_t)15;  // the spread's bucketed vertex list
   bool staged = !gpos;
-  size_t fixed = cell_lds_fixed(T, staged, gpos);
+  size_t fixed = cell_lds_fixed(T, staged, gpos, psize);
   size_t region = staged ? std::max(bend, (size_t)box_bytes) : (size_t)box_bytes;
   if (staged && fixed + region + lists + st_staged + margin > dev) {
     staged = false;
-    fixed = cell_lds_fixed(T, false, false);
+    fixed = cell_lds_fixed(T, false, false, sizeof(double));
     const size_t used = fixed + lists + st_plain + margin;
     region = std::min((size_t)box_bytes, dev > used ? dev - used : 0);
   }
@@ -486,6 +556,44 @@ hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T,
 #undef HC_CELL
 }
 
+// The persistent mixed launch: dynamic LDS for the largest need of the types
+// present (staged RBC and platelet cells, WBCs with positions in global memory),
+// each type's box filling what its layout leaves; `grid` workgroups.
+template <typename R>
+hipError_t launch_cell_step_mixed(hipStream_t s, const VertexDev& V, const TypeDev* d_types, const TypeDev* h_types,
+                                  int ntypes, const int* flags, const int2* queue, int nqueue, unsigned int* head,
+                                  int grid, const int64_t* cell_base, const R* uf, const uint8_t* type,
+                                  const LatticeDev& L, R* Fnext, const double* extra, double* scratch, double f_limit,
+                                  int box_bytes, int lds_device, unsigned long long* wall_hits) {
+  if (nqueue <= 0) return hipSuccess;
+  if (ntypes > kMaxMixedTypes || grid <= 0) return hipErrorInvalidValue;
+  static const size_t st = static_lds(k_cell_step_mixed<R>);
+  const size_t margin = 256, dev = (size_t)lds_device;
+  const size_t psize = sizeof(MechT<R, true, false>);
+  size_t fixed[kMaxMixedTypes], lists[kMaxMixedTypes], dyn = 0;
+  for (int t = 0; t < ntypes; ++t) {
+    const TypeDev& T = h_types[t];
+    const bool gpos = T.model == HC_MODEL_WBC;
+    const size_t bend = (!gpos && T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * psize : 0;
+    fixed[t] = cell_lds_fixed(T, !gpos, gpos, psize);
+    lists[t] = (2 * (size_t)T.nv + 15) & ~(size_t)15;
+    dyn = std::max(dyn, fixed[t] + std::max(bend, (size_t)box_bytes) + lists[t]);
+  }
+  dyn = (dyn + 15) & ~(size_t)15;
+  if (dyn + st + margin > dev) return hipErrorInvalidConfiguration;
+  MixedArgs A{};
+  for (int t = 0; t < ntypes; ++t) {
+    A.flags[t] = flags[t];
+    A.box_nodes[t] = (int)((dyn - fixed[t] - lists[t]) / (3 * sizeof(double)));
+  }
+  hipError_t e = launch_fill_zero<float>(s, reinterpret_cast<float*>(head), 1);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_cell_step_mixed<R>), dim3((unsigned)std::min(grid, nqueue)), dim3(kMixedBlock), dyn, s, V,
+                     d_types, A, queue, nqueue, head, cell_base, uf, type, L, Fnext, extra, scratch, f_limit,
+                     wall_hits);
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_cell_step_multi(hipStream_t s, const TypeDev& T, const CellCase* cases, int ncells, int box_bytes,
                                   int lds_device, int block) {
@@ -519,16 +627,5 @@ hipError_t launch_cell_step_multi(hipStream_t s, const TypeDev& T, const CellCas
   }
 #undef HC_CELL
 }
-
-template hipError_t launch_cell_step_multi<float>(hipStream_t, const TypeDev&, const CellCase*, int, int, int, int);
-template hipError_t launch_cell_step_multi<double>(hipStream_t, const TypeDev&, const CellCase*, int, int, int, int);
-template hipError_t launch_cell_step<float>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
-                                            const int64_t*, const float*, const uint8_t*, const LatticeDev&, float*,
-                                            const double*, double*, double, int, int, int, unsigned long long*, int,
-                                            bool);
-template hipError_t launch_cell_step<double>(hipStream_t, const VertexDev&, const TypeDev&, const int32_t*, int,
-                                             const int64_t*, const double*, const uint8_t*, const LatticeDev&, double*,
-                                             const double*, double*, double, int, int, int, unsigned long long*, int,
-                                             bool);
 
 }  // namespace hcg

@@ -66,13 +66,28 @@ hipError_t launch_interp_advance(hipStream_t s, const VertexDev& V, const R* uf,
                                  const LatticeDev& L, int do_interp, int do_advance, unsigned long long* wall_hits);
 hipError_t launch_add_extra_force(hipStream_t s, const VertexDev& V, const int64_t* ids, const double* forces, int n);
 
-// fused per-cell step (hc_cell.hip): interpolation + advance, mechanics, next spread
+// fused per-cell step (hc_cell_body.h): interpolation + advance, mechanics, next spread
 enum { CELL_FLAG_INTERP = 1, CELL_FLAG_MECH = 2, CELL_FLAG_SPREAD = 4, CELL_FLAG_REPULSION = 8 };
 template <typename R>
 hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                             const int64_t* cell_base, const R* uf, const uint8_t* type, const LatticeDev& L,
                             R* Fnext, const double* extra, double* scratch, double f_limit, int flags, int box_bytes,
                             int lds_device, unsigned long long* wall_hits, int block, bool gpos = false);
+
+// all cell types in one persistent launch (hc_cell_*.hip): a queue of (cell,
+// type index) entries, the types with most vertices first; flags per type
+constexpr int kMaxMixedTypes = 8;
+constexpr int kMixedBlock = 256;
+struct MixedArgs {
+  int flags[kMaxMixedTypes];
+  int box_nodes[kMaxMixedTypes];
+};
+template <typename R>
+hipError_t launch_cell_step_mixed(hipStream_t s, const VertexDev& V, const TypeDev* d_types, const TypeDev* h_types,
+                                  int ntypes, const int* flags, const int2* queue, int nqueue, unsigned int* head,
+                                  int grid, const int64_t* cell_base, const R* uf, const uint8_t* type,
+                                  const LatticeDev& L, R* Fnext, const double* extra, double* scratch, double f_limit,
+                                  int box_bytes, int lds_device, unsigned long long* wall_hits);
 
 // binning and cell-cell repulsion (hc_repulsion.hip)
 struct RepulsionBufs {

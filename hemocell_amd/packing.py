@@ -16,9 +16,17 @@ Lattice units throughout.  Result: {name: (centres [n, 3], rotations [n, 3, 3])}
 instances(types, packing) gives the vertex positions {name: [n, nv, 3]}.
 """
 import math
+import os
 
 import numpy as np
 from scipy.spatial import cKDTree
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def data_path(name):
+    """A committed packing of hemocell_amd/data."""
+    return os.path.join(DATA, name)
 
 
 def _quat_to_matrix(q):
@@ -72,11 +80,12 @@ def _random_quats(rng, n):
     return q / np.linalg.norm(q, axis=1, keepdims=True)
 
 
-def pack(shape, types, counts, seed=20261202, gap=0.5, walls=None, tube=None, max_iter=20000, log=None, tol=1e-3):
+def pack(shape, types, counts, seed=20261202, gap=0.5, walls=None, tube=None, max_iter=20000, log=None, tol=1e-3,
+         lam0=0.5, wall_margin=1.5):
     """Random positions and orientations of counts[name] cells of each type in a
     box of `shape` nodes.  walls: an axis (0, 1 or 2) bounded by wall planes at
-    0 and n-1; tube: (axis, centre, radius) of a cylinder the cells stay inside;
-    other axes periodic.  Cells keep `gap` (less `tol`) from each other and from
+    0 and n-1, which the cells keep wall_margin + gap away from; tube: (axis,
+    centre, radius) of a cylinder the cells stay inside; other axes periodic.  Cells keep `gap` (less `tol`) from each other and from
     the walls.  Returns {name: (centres, rotations)}."""
     rng = np.random.default_rng(seed)
     shape = np.asarray(shape, dtype=np.float64)
@@ -104,7 +113,7 @@ def pack(shape, types, counts, seed=20261202, gap=0.5, walls=None, tube=None, ma
     hi = shape.copy()
     if walls is not None:
         periodic[walls] = False
-        lo[walls], hi[walls] = gap, shape[walls] - 1.0 - gap
+        lo[walls], hi[walls] = wall_margin + gap, shape[walls] - 1.0 - wall_margin - gap
     if tube is not None:
         ax, tc, tr = tube
         others = [a for a in range(3) if a != ax]
@@ -122,31 +131,42 @@ def pack(shape, types, counts, seed=20261202, gap=0.5, walls=None, tube=None, ma
         c[:, o1] = tc[1] + r * np.sin(a)
     q = _random_quats(rng, nb)
     box = np.where(periodic, shape, 8.0 * shape.max())
-    lam = 0.5
+    lam = lam0
     it = 0
     rsmall = rad[small].max() if small.any() else 0.0
+    idx_small = np.flatnonzero(small)
+    ib = np.flatnonzero(big)
+    gi, gj = np.meshgrid(ib, ib, indexing="ij")
+    bb = gi < gj
+    skin = 1.0
+    S_ref, lam_ref, cand = None, None, None
     while it < max_iter:
         it += 1
         R = _quat_to_matrix(q)
         arm = lam * np.einsum("mij,mj->mi", R[sb], loc)
         S = c[sb] + arm
         r = lam * rad
-        Sw = np.where(periodic, np.mod(S, shape), S + 4.0 * shape.max())
-        tree = cKDTree(Sw[small], boxsize=box)
-        idx_small = np.flatnonzero(small)
-        pr = tree.query_pairs(2 * lam * rsmall + gap, output_type="ndarray")
-        i, j = idx_small[pr[:, 0]], idx_small[pr[:, 1]]
-        if big.any():
-            ib = np.flatnonzero(big)
-            hits = tree.query_ball_point(Sw[ib], lam * (rad[ib] + rsmall) + gap)
-            bi = np.concatenate([np.full(len(h), b) for b, h in zip(ib, hits)]) if len(ib) else np.zeros(0, int)
-            bj = idx_small[np.concatenate(hits).astype(int)] if len(ib) else np.zeros(0, int)
-            gi, gj = np.meshgrid(ib, ib, indexing="ij")
-            m = gi < gj
-            i = np.concatenate([i, bi, gi[m]])
-            j = np.concatenate([j, bj, gj[m]])
-        keep = sb[i] != sb[j]
-        i, j = i[keep], j[keep]
+        # candidate pairs within the reach plus a skin, rebuilt when a sphere has
+        # moved (or all have grown) by more than the skin allows
+        if S_ref is not None:
+            dS = S - S_ref
+            dS -= np.where(periodic, np.round(dS / shape) * shape, 0.0)
+            drift = 2 * np.sqrt(np.max(np.sum(dS * dS, axis=1))) + 2 * (lam - lam_ref) * rad.max()
+        if S_ref is None or drift > skin:
+            Sw = np.where(periodic, np.mod(S, shape), S + 4.0 * shape.max())
+            tree = cKDTree(Sw[small], boxsize=box)
+            pr = tree.query_pairs(2 * lam * rsmall + gap + skin, output_type="ndarray")
+            ci, cj = idx_small[pr[:, 0]], idx_small[pr[:, 1]]
+            if len(ib):
+                hits = tree.query_ball_point(Sw[ib], lam * (rad[ib] + rsmall) + gap + skin)
+                bi = np.concatenate([np.full(len(h), b) for b, h in zip(ib, hits)])
+                bj = idx_small[np.concatenate(hits).astype(int)]
+                ci = np.concatenate([ci, bi, gi[bb]])
+                cj = np.concatenate([cj, bj, gj[bb]])
+            other = sb[ci] != sb[cj]
+            cand = (ci[other], cj[other])
+            S_ref, lam_ref = S.copy(), lam
+        i, j = cand
         d = S[j] - S[i]
         d -= np.where(periodic, np.round(d / shape) * shape, 0.0)
         dist = np.linalg.norm(d, axis=1)
@@ -155,9 +175,8 @@ def pack(shape, types, counts, seed=20261202, gap=0.5, walls=None, tube=None, ma
         i, j, d, dist, over = i[hit], j[hit], d[hit], dist[hit], over[hit]
         n = d / np.maximum(dist, 1e-12)[:, None]
         push = 0.5 * (over + 2 * tol)[:, None] * n  # moves sphere j's cell by +push, sphere i's by -push
-        F = np.zeros((len(S), 3))
-        np.add.at(F, i, -push)
-        np.add.at(F, j, push)
+        F = np.stack([np.bincount(j, weights=push[:, k], minlength=len(S)) -
+                      np.bincount(i, weights=push[:, k], minlength=len(S)) for k in range(3)], -1).astype(np.float64)
         # walls and tube: push the sphere back inside
         wall_over = 0.0
         if walls is not None:
@@ -182,10 +201,16 @@ def pack(shape, types, counts, seed=20261202, gap=0.5, walls=None, tube=None, ma
                 break
             lam = min(1.0, lam + (0.01 if lam < 0.9 else 0.004))
             continue
-        # rigid-body response: mean push per cell (as many spheres as push it)
-        moved = np.bincount(sb, weights=np.any(F != 0, axis=1), minlength=nb)
+        # rigid-body response: the mean push over the cell's contacts (a cell
+        # pressed from many sides by one big sphere's neighbours moves by their
+        # mean, not their sum), at most a lattice spacing per iteration
+        contacts = np.bincount(sb[i], minlength=nb) + np.bincount(sb[j], minlength=nb)
+        if walls is not None or tube is not None:
+            contacts = contacts + np.bincount(sb, weights=np.any(F != 0, axis=1), minlength=nb)
         dc = np.stack([np.bincount(sb, weights=F[:, k], minlength=nb) for k in range(3)], -1)
-        dc /= np.maximum(moved, 1.0)[:, None]
+        dc /= np.maximum(contacts, 1.0)[:, None]
+        step = np.linalg.norm(dc, axis=1)
+        dc *= np.minimum(1.0, 1.0 / np.maximum(step, 1e-30))[:, None]
         tq = np.cross(arm, F)
         w = np.stack([np.bincount(sb, weights=tq[:, k], minlength=nb) for k in range(3)], -1)
         w /= (lam * lam * inertia)[:, None]
@@ -210,6 +235,19 @@ def pack(shape, types, counts, seed=20261202, gap=0.5, walls=None, tube=None, ma
     for ti, name in enumerate(names):
         m = body_type == ti
         out[name] = (c[m], R[m])
+    return out
+
+
+def tile(packing, tile_shape, reps):
+    """A packing of a box periodic along the tiled axes, repeated reps[a] times
+    along each axis a (the tiles join without overlap because the packing is
+    periodic across those faces)."""
+    offs = np.stack(np.meshgrid(*[np.arange(r) for r in reps], indexing="ij"), -1).reshape(-1, 3)
+    offs = offs * np.asarray(tile_shape, dtype=np.float64)
+    out = {}
+    for name, (cen, rot) in packing.items():
+        out[name] = ((cen[None, :, :] + offs[:, None, :]).reshape(-1, 3),
+                     np.broadcast_to(rot[None], (len(offs),) + rot.shape).reshape(-1, 3, 3).copy())
     return out
 
 
@@ -246,3 +284,104 @@ def overlap_free(types, packing, shape, gap, periodic=(True, True, True)):
     d = S[j] - S[i]
     d -= np.where(per, np.round(d / shape) * shape, 0.0)
     return bool(np.all(np.linalg.norm(d, axis=1) >= r[i] + r[j] + gap - 1e-9))
+
+
+def save(path, packing, **meta):
+    """A packing as an .npz file: per type its centres and unit quaternions
+    (float32; rounding moves a vertex by well under 1e-3 lattice units), plus
+    the metadata given (shape, seed, hematocrit, ...)."""
+    arrays = {}
+    for name, (cen, rot) in packing.items():
+        arrays[name + "_centres"] = cen.astype(np.float32)
+        arrays[name + "_quats"] = _matrix_to_quat(rot).astype(np.float32)
+    for k, v in meta.items():
+        arrays["meta_" + k] = np.asarray(v)
+    np.savez_compressed(path, **arrays)
+
+
+def load(path):
+    """(packing, meta) of a file written by save()."""
+    d = np.load(path, allow_pickle=False)
+    out, meta = {}, {}
+    for k in d.files:
+        if k.startswith("meta_"):
+            meta[k[5:]] = d[k]
+        elif k.endswith("_centres"):
+            name = k[:-8]
+            q = d[name + "_quats"].astype(np.float64)
+            q /= np.linalg.norm(q, axis=1, keepdims=True)
+            out[name] = (d[k].astype(np.float64), _quat_to_matrix(q))
+    return out, meta
+
+
+def _matrix_to_quat(R):
+    """Unit quaternions (w, x, y, z) of rotation matrices [n, 3, 3]."""
+    q = np.zeros((len(R), 4))
+    tr = np.trace(R, axis1=1, axis2=2)
+    for k in range(len(R)):
+        m = R[k]
+        if tr[k] > 0:
+            s = 2.0 * math.sqrt(1.0 + tr[k])
+            q[k] = (0.25 * s, (m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s)
+        else:
+            a = int(np.argmax(np.diag(m)))
+            b, c = (a + 1) % 3, (a + 2) % 3
+            s = 2.0 * math.sqrt(max(1.0 + m[a, a] - m[b, b] - m[c, c], 1e-300))
+            q[k, 0] = (m[c, b] - m[b, c]) / s
+            q[k, 1 + a] = 0.25 * s
+            q[k, 1 + b] = (m[b, a] + m[a, b]) / s
+            q[k, 1 + c] = (m[c, a] + m[a, c]) / s
+    return q / np.linalg.norm(q, axis=1, keepdims=True)
+
+
+def main(argv=None):
+    """python -m hemocell_amd.packing OUT.npz --shape nx,ny,nz --hematocrit H
+    [--walls AXIS] [--tube] [--plt-per-rbc R] [--wbc-per-rbc R] [--seed S]"""
+    import argparse
+    import sys
+    import time
+
+    from . import lbm_base_parameters
+    from . import suspension as susp
+    ap = argparse.ArgumentParser(prog="python -m hemocell_amd.packing")
+    ap.add_argument("out")
+    ap.add_argument("--shape", required=True)
+    ap.add_argument("--hematocrit", type=float, required=True)
+    ap.add_argument("--plt-per-rbc", type=float, default=0.1)
+    ap.add_argument("--wbc-per-rbc", type=float, default=1.0 / 500)
+    ap.add_argument("--walls", type=int, default=None, help="axis bounded by wall planes at 0 and n-1")
+    ap.add_argument("--tube", action="store_true", help="cells inside the tube along x of tube_fluid()")
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--max-iter", type=int, default=20000)
+    a = ap.parse_args(argv)
+    shape = tuple(int(v) for v in a.shape.split(","))
+    types = susp.cell_types(lbm_base_parameters(5e-7, 1e-7))
+    tube = None
+    if a.tube:
+        if shape[1] != shape[2]:
+            raise SystemExit("--tube needs a square y-z cross-section")
+        fluid = float(susp.tube_fluid(1, shape[1]).sum()) * shape[0]
+        c = (shape[1] - 1) / 2.0
+        # the supports (one node each way) stay off the wall nodes
+        tube = (0, (c, c), shape[1] / 2.0 - 0.5 - 1.5)
+    else:
+        fluid = float(np.prod(shape)) if a.walls is None else float(np.prod(shape)) * (shape[a.walls] - 2) / shape[a.walls]
+    counts = susp.cell_counts(types, fluid, a.hematocrit, a.plt_per_rbc, a.wbc_per_rbc)
+    t0 = time.time()
+    log = lambda m: print("%s (%.0f s)" % (m, time.time() - t0), file=sys.stderr, flush=True)  # noqa: E731
+    pk = pack(shape, types, counts, seed=a.seed, walls=a.walls, tube=tube, log=log, max_iter=a.max_iter)
+    periodic = [True, True, True]
+    if a.walls is not None:
+        periodic[a.walls] = False
+    if a.tube:
+        periodic[1] = periodic[2] = False
+    if not overlap_free(types, pk, shape, 0.49, periodic):
+        raise SystemExit("packing check failed")
+    save(a.out, pk, shape=shape, hematocrit=a.hematocrit, seed=a.seed, counts=[counts.get(n, 0) for n in
+                                                                             ("RBC", "PLT", "WBC")],
+         walls=-1 if a.walls is None else a.walls, tube=int(a.tube))
+    log("wrote %s: %s" % (a.out, counts))
+
+
+if __name__ == "__main__":
+    main()

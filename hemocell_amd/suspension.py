@@ -142,6 +142,13 @@ def pack(shape, types, hematocrit=0.30, plt_per_rbc=0.1, wbc_per_rbc=1.0 / 500, 
     return out
 
 
+def cell_counts(types, fluid_nodes, hematocrit, plt_per_rbc=0.1, wbc_per_rbc=1.0 / 500):
+    """Cells of each type for a hematocrit (RBC volume over the fluid volume)."""
+    rbc = types["RBC"]
+    n_rbc = int(round(hematocrit * fluid_nodes / _volume(rbc.verts, rbc.tris)))
+    return {"RBC": n_rbc, "PLT": int(round(plt_per_rbc * n_rbc)), "WBC": int(round(wbc_per_rbc * n_rbc))}
+
+
 def tube_fluid(length, diameter):
     """Fluid mask of a tube along x in a length x diameter x diameter box: fluid
     iff (y - c)^2 + (z - c)^2 < r^2 with c = (diameter - 1) / 2, r = diameter / 2 - 0.5;

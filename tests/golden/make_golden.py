@@ -11,6 +11,9 @@ purpose, and say why in the commit.
   rbc_forces.npz    RBC mechanics forces for a seeded +-5% perturbation of the rest mesh
   ibm_weights.npz   phi2 support and weights of 16 seeded positions, some next to a wall
   rbc_shear32.npz   one RBC in a 32^3 Couette box at 1000 1/s (velocity walls z = 0, 31), 100 steps
+  c3_crop_64x64x128.npz  an input, not an oracle output: a 64x64x128 periodic crop of the
+                    Ht 30% suspension packed by hemocell_amd.packing (random positions and
+                    orientations): 242 RBC, 24 PLT and one 2562-vertex WBC
 """
 import os
 import sys
@@ -106,10 +109,28 @@ def shear32():
                         u_slice_y16=uslice, tau=u.tau, steps=100)
 
 
+def c3_crop():
+    import hemocell_amd as hc
+    from hemocell_amd import packing
+    from hemocell_amd import suspension as susp
+    types = susp.cell_types(hc.lbm_base_parameters(5e-7, 1e-7))
+    shape = (64, 64, 128)
+    counts = susp.cell_counts(types, float(np.prod(shape)), 0.30)
+    counts["WBC"] = 1  # 1:500 rounds to none at this size; the crop keeps one
+    pk = packing.pack(shape, types, counts, seed=SEED + 5)
+    assert packing.overlap_free(types, pk, shape, 0.49)
+    packing.save(os.path.join(HERE, "c3_crop_64x64x128.npz"), pk, shape=shape, hematocrit=0.30, seed=SEED + 5,
+                 counts=[counts["RBC"], counts["PLT"], counts["WBC"]], walls=-1, tube=0)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["c3_crop"]:
+        c3_crop()
+        sys.exit(0)
     collide()
     periodic8()
     rbc()
     ibm()
     shear32()
+    c3_crop()
     print("fixtures written to", HERE)

@@ -11,6 +11,13 @@
   checkpoint.xml (loadCheckPoint(), the reference examples' restart path) ends
   where the uninterrupted run ends.
 
+* Against the oracle: with the material timescale 2 (every type), the particle
+  velocity timescale 2 and the repulsion timescale 3, a GPU run checkpointed at
+  iteration 8 and resumed in a fresh context through the C ABI ends, after 17
+  steps, where the uninterrupted FP64 oracle ends: positions, displacements and
+  populations within 1e-10 relative L2 (the resumed timescale phases line up
+  with the oracle's only if the restored iteration is right).
+
 The resumed run spreads its first step's forces with the stand-alone spread
 kernel where the uninterrupted one used the fused kernel's spread, and float
 atomics commit in no fixed order, so the runs agree to round-off, not bit for
@@ -29,6 +36,7 @@ import pytest
 
 import cases
 import hemocell_amd as hc
+from oracle import OracleSim
 
 pytestmark = pytest.mark.gpu
 
@@ -128,6 +136,39 @@ def test_state_round_trip_through_the_c_abi_resumes_the_run(variant, precision):
     assert cases.rel_l2(xa, xb) <= 100 * max(cases.rel_l2(xa, x2), floor)
     assert cases.rel_l2(fa, fb) <= 100 * max(cases.rel_l2(fa, f2), floor)
     assert cases.rel_l2(va, vb) <= 100 * max(cases.rel_l2(va, v2), floor)
+
+
+def _oracle_timescales(u, types):
+    o = OracleSim(*SHAPE, u.tau)
+    o.set_body_force([0, 0, 2e-6])
+    o.set_force_limit(u.f_limit)
+    tids = [o.add_cell_type(model, v, t, p, inner, timescale=2) for model, v, t, p, inner in types]
+    o.set_velocity_timescale(2)
+    o.set_repulsion(2e-3, 1.2, 3)
+    _load_cells(o, tids, types)
+    return o
+
+
+def test_resumed_run_with_timescales_matches_the_oracle():
+    print("[checkpoint] resumed vs oracle", flush=True)
+    u = cases.units()
+    types = _types(u)
+    o = _oracle_timescales(u, types)
+    first, tids = _fresh(u, types, "timescales", hc.HC_FP64)
+    _load_cells(first, tids, types)
+    x0 = first.vertices()[0]
+    first.step(8)
+    st = _save(first)
+    first.close()
+    resumed, tids = _fresh(u, types, "timescales", hc.HC_FP64)
+    _restore(resumed, tids, types, st)
+    resumed.step(9)
+    o.step(17)
+    xg, xo = resumed.vertices()[0], o.vertices()[0]
+    e = dict(x=cases.rel_l2(xg, xo), disp=cases.rel_l2(xg - x0, xo - x0),
+             pop=cases.rel_l2(resumed.populations(), o.populations()))
+    print("[checkpoint] resumed GPU vs uninterrupted oracle after 17 steps:", e)
+    assert e["x"] < 1e-10 and e["disp"] < 1e-10 and e["pop"] < 1e-10
 
 
 def test_iteration_and_repulsion_force_setters_validate():

@@ -89,7 +89,7 @@ def test_rbc_shear32_golden():
     g.step(int(d["steps"]))
     x, vel, _ = g.vertices()
     assert cases.rel_l2(x, d["x"]) < 1e-10
-    assert cases.rel_l2(x - d["x0"], d["x"] - d["x0"]) < 1e-8
+    assert cases.rel_l2(x - d["x0"], d["x"] - d["x0"]) < 1e-10
     assert cases.rel_l2(vel, d["v"]) < 1e-9
     _, uf = g.fluid()
     assert cases.rel_l2(uf.reshape(32, 32, 32, 3)[:, 16, :, :], d["u_slice_y16"]) < 1e-10

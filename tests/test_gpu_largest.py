@@ -3,7 +3,9 @@ do not depend on size (no oracle run is feasible at 268M nodes).
 
 512x512x1024 channel between bounce-back planes y = 0 and y = 511, periodic in x
 and z, body force along z, Ht 35%: 144780 RBCs and 14478 platelets (93.9M
-vertices), seeded non-overlapping placement clear of the walls.  After 20 steps:
+vertices), at seeded random positions and orientations clear of the walls (the
+128x512x128 tile hemocell_amd/data/channel_tile_128x512x128_ht35.npz, periodic in
+x and z, repeated 4 x 8).  After 20 steps:
 
 * FP64 total mass sum(rho) is conserved to 1e-12 relative: bounce-back planes
   and periodic faces neither add nor remove mass, and the IBM force adds
@@ -18,6 +20,7 @@ import numpy as np
 import pytest
 
 import hemocell_amd as hc
+from hemocell_amd import packing
 from hemocell_amd import suspension as susp
 
 pytestmark = pytest.mark.gpu
@@ -30,7 +33,8 @@ def _run(precision):
     print("[largest] building", "fp64" if precision == hc.HC_FP64 else "fp32", flush=True)
     units = hc.lbm_base_parameters(5e-7, 1e-7)
     types = susp.cell_types(units)
-    cells = susp.pack(SHAPE, types, hematocrit=0.35, wbc_per_rbc=0.0, wall_axis=1)
+    tile = packing.load(packing.data_path("channel_tile_128x512x128_ht35.npz"))[0]
+    cells = packing.instances(types, packing.tile(tile, (128, 512, 128), (4, 1, 8)))
     mask = np.zeros(SHAPE, dtype=np.uint8)
     mask[:, 0, :] = mask[:, SHAPE[1] - 1, :] = hc.NODE_BOUNCEBACK
     sim = hc.Simulation(0)

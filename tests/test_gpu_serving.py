@@ -12,6 +12,7 @@ import pytest
 import cases
 import hemocell_amd as hc
 from hemocell_amd import serving
+from oracle import OracleSim
 
 pytestmark = pytest.mark.gpu
 
@@ -158,3 +159,87 @@ def test_serve_native_keeps_serving_past_a_failed_case():
             assert c.result and c.result["volume"] > 0 and not c.error
     st = serving.latency_stats(done, wall)
     assert st["cases"] == 3 and st["failed"] == 1
+
+
+def test_stream_at_full_ranges_matches_alone():
+    """The bench's stream generator at its full ranges (edge 32-96, 2000-20000
+    steps, shear 10-1000 1/s or stretch 0-200 pN, RBC/PLT/WBC): every case of an
+    8-case stream served by the batched loop ends as close to the case run alone
+    as two lone runs are to each other."""
+    print("[serving] full ranges", flush=True)
+    u = hc.lbm_base_parameters(5e-7, 1e-7)
+    stream = serving.make_cases(8, rate=100.0, seed=23)
+    assert max(c.edge for c in stream) > 64 and max(c.steps for c in stream) > 10000
+    done, wall = serving.serve(stream, u, max_active=8, chunk=250, keep_vertices=True)
+    assert sorted(c.ident for c in done) == list(range(8))
+    for c in done:
+        alone, x = serving.run_alone(c, u)
+        _, x2 = serving.run_alone(c, u)
+        noise = cases.rel_l2(x2, x)
+        mix = cases.rel_l2(c.result["x"], x)
+        print("[serving] case %d (%s %s, edge %d, %d steps): mix %.2e, lone-vs-lone %.2e" % (
+            c.ident, c.kind, c.cell, c.edge, c.steps, mix, noise))
+        assert mix <= 100 * max(noise, 1e-12), (c, mix, noise)
+
+
+def _oracle_case(case, units):
+    """The oracle set up as serving.build sets up the case (FP64)."""
+    E = case.edge
+    t = serving._cell_types(units)[case.cell]
+    mask = np.zeros((E, E, E), dtype=np.uint8)
+    if case.kind == "shear":
+        mask[:, :, 0] = hc.NODE_VELOCITY + 4
+        mask[:, :, E - 1] = hc.NODE_VELOCITY + 5
+        o = OracleSim(E, E, E, units.tau, periodic=(1, 1, 0), node_type=mask.reshape(-1))
+        vh = (E - 1) * case.shear_rate * units.dt * 0.5
+        o.set_wall_velocity(4, (vh, 0.0, 0.0))
+        o.set_wall_velocity(5, (-vh, 0.0, 0.0))
+    else:
+        for a in range(3):
+            for side, face in ((0, 2 * a), (E - 1, 2 * a + 1)):
+                sl = [slice(None)] * 3
+                sl[a] = side
+                fm = mask[tuple(sl)]
+                fm[fm == 0] = hc.NODE_VELOCITY + face
+        o = OracleSim(E, E, E, units.tau, periodic=(0, 0, 0), node_type=mask.reshape(-1))
+    o.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    o.set_force_limit(units.f_limit)
+    tid = o.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner)
+    x = t.verts - 0.5 * (t.verts.min(axis=0) + t.verts.max(axis=0)) + (E - 1) / 2.0
+    o.add_cells(tid, x[None])
+    extra = None
+    if case.kind == "stretch":
+        order = np.argsort(x[:, 0], kind="stable")
+        n = 7
+        f = case.force_pN * 1e-12 / units.df / n
+        ids = np.concatenate([order[:n], order[::-1][:n]])
+        forces = np.zeros((2 * n, 3))
+        forces[:n, 0] = -f
+        forces[n:, 0] = f
+        extra = (ids, forces)
+    return o, extra, x
+
+
+def test_stream_cases_match_the_oracle():
+    """Three cases of the stream (shear and stretch; edge and steps cut to 32-40
+    and 300-500 so the oracle finishes), served together in FP64 by the batched
+    loop, each against the FP64 oracle: positions and displacements within 1e-10."""
+    print("[serving] stream cases vs oracle", flush=True)
+    u = hc.lbm_base_parameters(5e-7, 1e-7)
+    stream = serving.make_cases(3, rate=100.0, seed=31, edge=(32, 40), steps=(300, 500))
+    assert {c.kind for c in stream} == {"shear", "stretch"}
+    done, _ = serving.serve(stream, u, max_active=3, chunk=100, keep_vertices=True, precision=hc.HC_FP64)
+    for c in done:
+        o, extra, x0 = _oracle_case(c, u)
+        for _ in range(c.steps):
+            if extra is not None:
+                _, _, fo = o.vertices()
+                fo[extra[0]] += extra[1]
+                o.set_vertices(None, None, fo)
+            o.step(1)
+        xo = o.vertices()[0]
+        xg = c.result["x"]
+        e = (cases.rel_l2(xg, xo), cases.rel_l2(xg - x0, xo - x0))
+        print("[serving] case %d (%s %s, edge %d, %d steps) vs oracle: x %.2e, displacement %.2e" % (
+            (c.ident, c.kind, c.cell, c.edge, c.steps) + e))
+        assert e[0] < 1e-10 and e[1] < 1e-10
