@@ -175,6 +175,9 @@ def run_gpu(hc, susp, shape, precision, steps, warmup, workload="suspension", au
                vertices=nv, nodes=N, cells={k: int(len(v)) for k, v in cells.items()})
     res["collide_ms"] = collide_ms
     res["tuning"] = {k: v["chosen"] for k, v in tuning.items()} if tuning else None
+    # every trial's ms per step (eager steps, one parameter varied at a time)
+    res["tuning_ms"] = ({k: {c: round(t, 4) for c, t in v.items() if c != "chosen"} for k, v in tuning.items()}
+                        if tuning else None)
     res["steps_before"] = done_before
     res["wall_hits"] = int(wall_hits)
     res["x0"], res["x1"] = x0, x1
@@ -401,6 +404,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in head["stage_ms"].items()},
         "ms_per_step_eager_profiled": round(head["ms_per_step_eager"], 4),
         "tuning": head["tuning"],
+        "tuning_ms": head["tuning_ms"],
     }
     if "fp64" in res and "fp32" in res:
         f64 = res["fp64"]
@@ -415,7 +419,7 @@ def main():
                         "collide_gbs": round(f64["collide_gbs"], 1),
                         "collide_frac": round(f64["collide_gbs"] / HBM_PEAK_GBS, 4),
                         "stage_ms": {k: round(v, 4) for k, v in f64["stage_ms"].items()},
-                        "tuning": f64["tuning"]}
+                        "tuning": f64["tuning"], "tuning_ms": f64["tuning_ms"]}
         line["fp32_over_fp64"] = round(res["fp32"]["mlups"] / f64["mlups"], 3)
     # the serving stream and the CPU baseline belong to the headline workload
     if args.serve > 0 and args.workload == "suspension":

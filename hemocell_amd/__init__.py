@@ -83,6 +83,7 @@ SIGNATURES = {
     "hc_set_velocity_timescale": (_ST, [_P, _INT]),
     "hc_set_material_timescale": (_ST, [_P, _INT, _INT]),
     "hc_set_force_limit": (_ST, [_P, ctypes.c_double]),
+    "hc_set_exact_spread": (_ST, [_P, ctypes.c_int]),
     "hc_set_repulsion": (_ST, [_P, ctypes.c_double, ctypes.c_double, _INT]),
     "hc_set_extra_vertex_force": (_ST, [_P, _INT, _I64, _D]),
     "hc_step": (_ST, [_P, _INT]),
@@ -360,6 +361,11 @@ class Simulation:
 
     def set_force_limit(self, fl):
         self._chk(self.lib.hc_set_force_limit(self.h, fl))
+
+    def set_exact_spread(self, on=True):
+        """FP64 only, after lattice_init and set_force_limit: spread contributions
+        rounded to a power-of-two quantum so repeated runs agree bit for bit."""
+        self._chk(self.lib.hc_set_exact_spread(self.h, 1 if on else 0))
 
     def set_repulsion(self, k_rep, cutoff_lu, timescale=1):
         self._chk(self.lib.hc_set_repulsion(self.h, k_rep, cutoff_lu, timescale))

@@ -121,6 +121,8 @@ struct hc_ctx {
   int64_t iter = 0;
   int pv_timescale = 1;
   double f_limit = 1e300;
+  bool f_limit_set = false;
+  bool exact_spread = false;  // FP64: spread contributions rounded to L.spread_q
   bool repulsion = false;
   double rep_k = 0, rep_cutoff = 0;
   int rep_timescale = 1;
@@ -1157,7 +1159,9 @@ hc_status autotune_impl(hc_ctx* ctx, int steps, std::string& report) {
        [&] { return has_rbc; }},
       {"side_positions_global", {1, 0}, [&](int v) { ctx->side_gpos = v != 0; },
        [&] { return ntypes > 1 && big_side; }},
-      {"wbc_threads", {256, 512, 1024}, [&](int v) { ctx->wbc_block = v; }, [&] { return has_wbc; }},
+      // a side WBC with positions in global memory always runs 256 threads
+      {"wbc_threads", {256, 512, 1024}, [&](int v) { ctx->wbc_block = v; },
+       [&] { return has_wbc && !(ctx->side_gpos && ntypes > 1 && big_side); }},
       {"persistent_launch", {0, 1}, [&](int v) { ctx->mixed = v != 0; }, [&] { return queue; }},
       {"persistent_workgroups_per_cu", {4, 8, 16}, [&](int v) { ctx->mixed_per_cu = v; },
        [&] { return ctx->mixed; }},
@@ -1387,6 +1391,7 @@ hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* no
   }
   ctx->L.N = N;
   ctx->L.stride = (N + 255) / 256 * 256;
+  ctx->L.spread_q = ctx->L.spread_inv_q = 0.0;  // plain adds until hc_set_exact_spread
   bool has_walls = false;
   std::vector<uint32_t> vel;
   if (node_type) {
@@ -1584,13 +1589,41 @@ hc_status hc_set_material_timescale(hc_ctx* ctx, int type_id, int ts) {
   ctx->types[type_id]->host.timescale = ts;
   return HC_OK;
 }
+// The exact spread's quantum: 2^(e-49) for a force limit in [2^e, 2^(e+1)).  A
+// node's force sum stays exact up to 2^53 quanta = 2^(e+4) >= 8 force limits,
+// and a vertex force at the limit keeps 49 bits.
+static void set_spread_quantum(hc_ctx* ctx) {
+  if (ctx->exact_spread) {
+    const int e = std::ilogb(ctx->f_limit);
+    ctx->L.spread_q = std::ldexp(1.0, e - 49);
+    ctx->L.spread_inv_q = std::ldexp(1.0, 49 - e);
+  } else {
+    ctx->L.spread_q = 0.0;
+    ctx->L.spread_inv_q = 0.0;
+  }
+}
+
 hc_status hc_set_force_limit(hc_ctx* ctx, double f_limit) {
-  if (!ctx || !(f_limit > 0)) return fail(ctx, HC_INVALID_ARG, "force limit must be > 0");
+  if (!ctx || !(f_limit > 0) || !std::isfinite(f_limit))
+    return fail(ctx, HC_INVALID_ARG, "force limit must be finite and > 0");
   if (ctx->lattice_ready) {
     hc_status s = invalidate_spread(ctx);
     if (s != HC_OK) return s;
   }
   ctx->f_limit = f_limit;
+  ctx->f_limit_set = true;
+  set_spread_quantum(ctx);
+  return HC_OK;
+}
+hc_status hc_set_exact_spread(hc_ctx* ctx, int on) {
+  if (!ctx) return HC_INVALID_ARG;
+  if (!ctx->lattice_ready) return fail(ctx, HC_STATE, "lattice not initialised");
+  if (on && ctx->precision != HC_FP64) return fail(ctx, HC_INVALID_ARG, "the exact spread needs the FP64 form");
+  if (on && !ctx->f_limit_set) return fail(ctx, HC_STATE, "the exact spread needs a force limit (hc_set_force_limit)");
+  hc_status s = invalidate_spread(ctx);
+  if (s != HC_OK) return s;
+  ctx->exact_spread = on != 0;
+  set_spread_quantum(ctx);
   return HC_OK;
 }
 hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale) {

@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "hc_ibm_dev.h"
+#include "hc_lds_plan.h"
 #include "hc_mech_dev.h"
 
 namespace hcg {
@@ -40,12 +41,6 @@ constexpr int kMaxPlanes = 64;
 // everything else keeps absolute double positions.
 template <typename R, bool STAGED, bool GPOS>
 using MechT = typename std::conditional<std::is_same<R, float>::value && STAGED && !GPOS, float, double>::type;
-
-// LDS bytes before the box region: positions and (STAGED, viscous) velocities of
-// P scalars, rounded up to 16 bytes
-__host__ __device__ constexpr size_t cell_region_offset(int nv, bool visc, size_t psize) {
-  return ((visc ? 6 : 3) * (size_t)nv * psize + 15) & ~(size_t)15;
-}
 
 // STAGED: positions, velocities and bending forces in LDS.  Otherwise positions
 // in LDS and the rest in global memory, or (GPOS) the positions too: then the
@@ -85,12 +80,13 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   P* py = GPOS ? reinterpret_cast<P*>(V.x + vs + base) : sp_ + nv;
   P* pz = GPOS ? reinterpret_cast<P*>(V.x + 2 * vs + base) : sp_ + 2 * nv;
   P *vx, *vy, *vz, *bx, *by, *bz;
-  double* region;
+  // the region (bending forces, then the box) after the fixed part (hc_lds_plan.h)
+  double* const region =
+      reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + cell_fixed_bytes(nv, visc, sizeof(P), STAGED, GPOS));
   if (STAGED) {
     vx = sp_ + 3 * nv;
     vy = sp_ + 4 * nv;
     vz = sp_ + 5 * nv;
-    region = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + cell_region_offset(nv, visc, sizeof(P)));
     bx = reinterpret_cast<P*>(region);
     by = bx + nv;
     bz = bx + 2 * nv;
@@ -98,7 +94,6 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     vx = reinterpret_cast<P*>(V.v + base);
     vy = reinterpret_cast<P*>(V.v + vs + base);
     vz = reinterpret_cast<P*>(V.v + 2 * vs + base);
-    region = GPOS ? sm : sm + 3 * nv;
     bx = reinterpret_cast<P*>(scratch + base);
     by = reinterpret_cast<P*>(scratch + vs + base);
     bz = reinterpret_cast<P*>(scratch + 2 * vs + base);
@@ -314,9 +309,9 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       for (int q = 0; q < 8; ++q) {
         if (sp.w[q] == 0.0) continue;
         const int64_t n = sp.idx[q];
-        atomic_add(&Fnext[n], (R)(f0 * sp.w[q]));
-        atomic_add(&Fnext[S + n], (R)(f1 * sp.w[q]));
-        atomic_add(&Fnext[2 * S + n], (R)(f2 * sp.w[q]));
+        atomic_add(&Fnext[n], (R)spread_term(f0 * sp.w[q], L));
+        atomic_add(&Fnext[S + n], (R)spread_term(f1 * sp.w[q], L));
+        atomic_add(&Fnext[2 * S + n], (R)spread_term(f2 * sp.w[q], L));
       }
     }
     return;
@@ -369,9 +364,9 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
         const int lx = sp.lo[0] + (q >> 2) - o0 - xs;
         if (sp.w[q] == 0.0 || lx < 0 || lx >= nxp) continue;
         const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * lx);
-        lds_add(&box[e], f0 * sp.w[q]);
-        lds_add(&box[total + e], f1 * sp.w[q]);
-        lds_add(&box[2 * total + e], f2 * sp.w[q]);
+        lds_add(&box[e], spread_term(f0 * sp.w[q], L));
+        lds_add(&box[total + e], spread_term(f1 * sp.w[q], L));
+        lds_add(&box[2 * total + e], spread_term(f2 * sp.w[q], L));
       }
     }
     __syncthreads();
@@ -449,17 +444,6 @@ __global__ __launch_bounds__(kMixedBlock) void k_cell_step_mixed(
 
 }  // namespace
 
-// Fixed LDS of the kernel (bytes) for a type: positions, plus velocities when
-// staged (psize bytes each: float offsets in the FP32-shifted staged kernels);
-// nothing with the positions in global memory.
-static size_t cell_lds_fixed(const TypeDev& T, bool staged, bool gpos, size_t psize) {
-  if (gpos) return 0;
-  const bool visc = (T.model != HC_MODEL_RBC) || (T.eta_m != 0.0);
-  const size_t nv = (size_t)T.nv;
-  if (!staged) return 3 * nv * sizeof(double);
-  return cell_region_offset(T.nv, visc, psize);
-}
-
 // Static LDS of one kernel instantiation (the reduction array grows with the block).
 template <typename K>
 static size_t static_lds(K kernel) {
@@ -484,23 +468,14 @@ static hipError_t launch_cell(hipStream_t s, const VertexDev& V, const TypeDev& 
   static const size_t st_plain = std::max(static_lds(k_cell_step<R, M, B, false, false>),
                                           static_lds(k_cell_step_multi<R, M, B, false, false>));
   static const size_t st_gpos = static_lds(k_cell_step<R, M, B, false, true>);
-  const size_t margin = 256, dev = (size_t)lds_device;
-  const size_t psize = sizeof(MechT<R, true, false>);  // the staged kernel's position scalar
-  const size_t bend = (T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * psize : 0;
-  const size_t lists = (2 * (size_t)T.nv + 15) & ~(size_t)15;  // the spread's bucketed vertex list
-  bool staged = !gpos;
-  size_t fixed = cell_lds_fixed(T, staged, gpos, psize);
-  size_t region = staged ? std::max(bend, (size_t)box_bytes) : (size_t)box_bytes;
-  if (staged && fixed + region + lists + st_staged + margin > dev) {
-    staged = false;
-    fixed = cell_lds_fixed(T, false, false, sizeof(double));
-    const size_t used = fixed + lists + st_plain + margin;
-    region = std::min((size_t)box_bytes, dev > used ? dev - used : 0);
-  }
-  const size_t st = staged ? st_staged : (gpos ? st_gpos : st_plain);
-  if (fixed + region + lists + st + margin > dev) return hipErrorInvalidConfiguration;
-  const int box_nodes = (int)(region / (3 * sizeof(double)));
-  const size_t dyn = fixed + region + lists;
+  const bool visc = (T.model != HC_MODEL_RBC) || (T.eta_m != 0.0);
+  // the staged kernel's position scalar: float offsets in the FP32-shifted form
+  const CellLdsPlan plan = cell_lds_plan(T.model, T.nv, visc, sizeof(MechT<R, true, false>), (size_t)box_bytes,
+                                         (size_t)lds_device, st_staged, st_plain, st_gpos, gpos);
+  if (!plan.ok) return hipErrorInvalidConfiguration;
+  const bool staged = plan.staged;
+  const int box_nodes = plan.box_nodes;
+  const size_t dyn = plan.dyn;
   if (multi) {
     if (gpos) return hipErrorInvalidValue;
     if (staged)
@@ -541,19 +516,25 @@ hipError_t launch_cell_step(hipStream_t s, const VertexDev& V, const TypeDev& T,
       if (block >= 1024) HC_CELL(HC_MODEL_RBC, 1024)
       if (block >= 512) HC_CELL(HC_MODEL_RBC, 512)
       HC_CELL(HC_MODEL_RBC, 256)
-      return e;
+      break;
     case HC_MODEL_PLT:
       HC_CELL(HC_MODEL_PLT, 64)
-      return e;
+      break;
     case HC_MODEL_WBC:
       if (block >= 1024) HC_CELL(HC_MODEL_WBC, 1024)
       if (block >= 512) HC_CELL(HC_MODEL_WBC, 512)
       HC_CELL(HC_MODEL_WBC, 256)
-      return e;
+      break;
     default:
       return hipErrorInvalidValue;
   }
 #undef HC_CELL
+  // a mesh whose positions do not fit the workgroup's LDS at any block (a
+  // 2562-vertex WBC on a 64 KB device) keeps them in global memory
+  if (e == hipErrorInvalidConfiguration && !gpos)
+    return launch_cell_step<R>(s, V, T, cells, ncells, cell_base, uf, type, L, Fnext, extra, scratch, f_limit, flags,
+                               box_bytes, lds_device, wall_hits, 256, true);
+  return e;
 }
 
 // The persistent mixed launch: dynamic LDS for the largest need of the types
@@ -568,24 +549,18 @@ hipError_t launch_cell_step_mixed(hipStream_t s, const VertexDev& V, const TypeD
   if (nqueue <= 0) return hipSuccess;
   if (ntypes > kMaxMixedTypes || grid <= 0) return hipErrorInvalidValue;
   static const size_t st = static_lds(k_cell_step_mixed<R>);
-  const size_t margin = 256, dev = (size_t)lds_device;
-  const size_t psize = sizeof(MechT<R, true, false>);
-  size_t fixed[kMaxMixedTypes], lists[kMaxMixedTypes], dyn = 0;
+  int model[kMaxMixedTypes], nv[kMaxMixedTypes];
+  bool visc[kMaxMixedTypes];
   for (int t = 0; t < ntypes; ++t) {
-    const TypeDev& T = h_types[t];
-    const bool gpos = T.model == HC_MODEL_WBC;
-    const size_t bend = (!gpos && T.model != HC_MODEL_PLT) ? 3 * (size_t)T.nv * psize : 0;
-    fixed[t] = cell_lds_fixed(T, !gpos, gpos, psize);
-    lists[t] = (2 * (size_t)T.nv + 15) & ~(size_t)15;
-    dyn = std::max(dyn, fixed[t] + std::max(bend, (size_t)box_bytes) + lists[t]);
+    model[t] = h_types[t].model;
+    nv[t] = h_types[t].nv;
+    visc[t] = (h_types[t].model != HC_MODEL_RBC) || (h_types[t].eta_m != 0.0);
   }
-  dyn = (dyn + 15) & ~(size_t)15;
-  if (dyn + st + margin > dev) return hipErrorInvalidConfiguration;
   MixedArgs A{};
-  for (int t = 0; t < ntypes; ++t) {
-    A.flags[t] = flags[t];
-    A.box_nodes[t] = (int)((dyn - fixed[t] - lists[t]) / (3 * sizeof(double)));
-  }
+  const size_t dyn = mixed_lds_plan(ntypes, model, nv, visc, sizeof(MechT<R, true, false>), (size_t)box_bytes,
+                                    (size_t)lds_device, st, A.box_nodes);
+  if (dyn == 0) return hipErrorInvalidConfiguration;
+  for (int t = 0; t < ntypes; ++t) A.flags[t] = flags[t];
   hipError_t e = launch_fill_zero<float>(s, reinterpret_cast<float*>(head), 1);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_cell_step_mixed<R>), dim3((unsigned)std::min(grid, nqueue)), dim3(kMixedBlock), dyn, s, V,

@@ -79,9 +79,9 @@ __global__ __launch_bounds__(BLOCK) void k_spread(VertexDev V, R* __restrict__ F
   for (int k = 0; k < 8; ++k) {
     if (sp.w[k] == 0.0) continue;
     const int64_t n = sp.idx[k];
-    atomic_add(&Fibm[n], (R)(t0 * sp.w[k]));
-    atomic_add(&Fibm[S + n], (R)(t1 * sp.w[k]));
-    atomic_add(&Fibm[2 * S + n], (R)(t2 * sp.w[k]));
+    atomic_add(&Fibm[n], (R)spread_term(t0 * sp.w[k], L));
+    atomic_add(&Fibm[S + n], (R)spread_term(t1 * sp.w[k], L));
+    atomic_add(&Fibm[2 * S + n], (R)spread_term(t2 * sp.w[k], L));
   }
 }
 
@@ -288,14 +288,14 @@ __global__ __launch_bounds__(BS) void k_spread_cells(VertexDev V, R* __restrict_
       if (sp.w[q] == 0.0) continue;
       if (direct) {
         const int64_t n = sp.idx[q];
-        atomic_add(&Fibm[n], (R)(t0 * sp.w[q]));
-        atomic_add(&Fibm[S + n], (R)(t1 * sp.w[q]));
-        atomic_add(&Fibm[2 * S + n], (R)(t2 * sp.w[q]));
+        atomic_add(&Fibm[n], (R)spread_term(t0 * sp.w[q], L));
+        atomic_add(&Fibm[S + n], (R)spread_term(t1 * sp.w[q], L));
+        atomic_add(&Fibm[2 * S + n], (R)spread_term(t2 * sp.w[q], L));
       } else {
         const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * (sp.lo[0] + (q >> 2) - o0));
-        lds_add(&box[e], t0 * sp.w[q]);
-        lds_add(&box[total + e], t1 * sp.w[q]);
-        lds_add(&box[2 * total + e], t2 * sp.w[q]);
+        lds_add(&box[e], spread_term(t0 * sp.w[q], L));
+        lds_add(&box[total + e], spread_term(t1 * sp.w[q], L));
+        lds_add(&box[2 * total + e], spread_term(t2 * sp.w[q], L));
       }
     }
   }

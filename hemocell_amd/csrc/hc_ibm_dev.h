@@ -84,6 +84,15 @@ __device__ __forceinline__ void ibm_support(double p0, double p1, double p2, con
   for (int k = 0; k < 8; ++k) s.w[k] = s.w[k] != 0.0 ? s.w[k] * coeff : 0.0;
 }
 
+// One force contribution to the spread: rounded to a multiple of L.spread_q when
+// the exact spread is on.  spread_q is a power of two, so both scalings are
+// exact and rint is the only rounding; the sums of such multiples stay exact
+// (hence independent of the order of the adds) while they are below 2^53
+// spread_q in magnitude.
+__device__ __forceinline__ double spread_term(double v, const LatticeDev& L) {
+  return L.spread_q > 0.0 ? rint(v * L.spread_inv_q) * L.spread_q : v;
+}
+
 __device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 

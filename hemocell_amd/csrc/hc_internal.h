@@ -72,6 +72,10 @@ struct LatticeDev {
   int periodic[3];
   int64_t N;       // nodes
   int64_t stride;  // population / force component stride (>= N, padded)
+  // Exact spread (FP64 only): every force contribution is rounded to a multiple
+  // of the power of two spread_q before it is added, so the sums are exact and
+  // do not depend on the order of the atomic adds; 0 = plain adds.
+  double spread_q, spread_inv_q;
 };
 
 }  // namespace hcg

@@ -137,6 +137,14 @@ hc_status hc_add_cells(hc_ctx* ctx, int type_id, int ncells, const double* xyz,
 hc_status hc_set_velocity_timescale(hc_ctx* ctx, int timescale);
 hc_status hc_set_material_timescale(hc_ctx* ctx, int type_id, int timescale);
 hc_status hc_set_force_limit(hc_ctx* ctx, double f_limit);
+/* The reference spreads in particle order (core/hemoCellParticleField.cpp:841-863), so
+ * its runs repeat bit for bit; the GPU spread adds with atomics in no fixed order.  With
+ * on != 0 (FP64 form only; needs a force limit, and the lattice), every force contribution
+ * is rounded to a multiple of 2^(e-49), 2^e <= f_limit < 2^(e+1), before it is added: the
+ * sums are then exact up to 8 force limits per node, so repeated runs, a run alone or in a
+ * batch, and a run resumed from a checkpoint agree bit for bit.  A vertex force at the
+ * limit keeps 49 bits; smaller ones lose relatively more (absolute rounding 2^(e-50)). */
+hc_status hc_set_exact_spread(hc_ctx* ctx, int on);
 hc_status hc_set_repulsion(hc_ctx* ctx, double k_rep, double cutoff_lu, int timescale);
 /* Replaces: HemoCell::enableBoundaryParticles (core/hemoCell.cpp:428-436) with
  * populateBoundaryParticles / applyBoundaryRepulsionForce
