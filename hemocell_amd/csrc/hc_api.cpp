@@ -1154,7 +1154,8 @@ hc_status autotune_impl(hc_ctx* ctx, int steps, std::string& report) {
   // all types against them, and its workgroups per compute unit when it wins
   const bool queue = ctx->d_queue && ntypes > 1;
   std::vector<Param> params = {
-      {"spread_box_bytes", {8192, 12288, 16384, 24576}, [&](int v) { ctx->spread_lds = v; }, [] { return true; }},
+      {"spread_box_bytes", {8192, 12288, 16384, 20480, 24576}, [&](int v) { ctx->spread_lds = v; },
+       [] { return true; }},
       {"rbc_threads", {256, 512, 1024}, [&](int v) { ctx->cell_block = v; ctx->cell_block_env = true; },
        [&] { return has_rbc; }},
       {"side_positions_global", {1, 0}, [&](int v) { ctx->side_gpos = v != 0; },

@@ -55,6 +55,10 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
                                                unsigned long long* wall_hits) {
   using P = MechT<R, STAGED, GPOS>;
   constexpr bool REL = !std::is_same<P, double>::value;  // LDS holds offsets from the cell origin
+  // scalar of the support weights, the interpolated velocity and the spread's
+  // contributions: FP32 in the FP32-shifted form (whose velocity field and
+  // force field are FP32), FP64 otherwise
+  using SW = typename std::conditional<std::is_same<R, float>::value, float, double>::type;
   extern __shared__ __align__(16) double sm[];
   __shared__ double red[BS];
   __shared__ double vforce_sh;
@@ -127,20 +131,20 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       if (k >= nv) continue;
       const int64_t i = base + k;
       if (flags & CELL_INTERP) {
-        Support sp;
+        SupportT<SW> sp;
         ibm_support(xa[u][0], xa[u][1], xa[u][2], L, type, sp);
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        SW v0 = SW(0), v1 = SW(0), v2 = SW(0);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          if (sp.w[q] == 0.0) continue;
+          if (sp.w[q] == SW(0)) continue;
           const R* un = uf + 4 * (int64_t)sp.idx[q];
-          v0 += (double)un[0] * sp.w[q];
-          v1 += (double)un[1] * sp.w[q];
-          v2 += (double)un[2] * sp.w[q];
+          v0 += (SW)un[0] * sp.w[q];
+          v1 += (SW)un[1] * sp.w[q];
+          v2 += (SW)un[2] * sp.w[q];
         }
-        va[u][0] = v0;
-        va[u][1] = v1;
-        va[u][2] = v2;
+        va[u][0] = (double)v0;
+        va[u][1] = (double)v1;
+        va[u][2] = (double)v2;
       } else {
         va[u][0] = V.v[i];
         va[u][1] = V.v[vs + i];
@@ -303,15 +307,16 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       const int64_t i = base + k;
       double f0, f1, f2;
       spread_force(i, f0, f1, f2);
-      Support sp;
+      const SW g0 = (SW)f0, g1 = (SW)f1, g2 = (SW)f2;
+      SupportT<SW> sp;
       ibm_support(pos_x(k), pos_y(k), pos_z(k), L, type, sp);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        if (sp.w[q] == 0.0) continue;
+        if (sp.w[q] == SW(0)) continue;
         const int64_t n = sp.idx[q];
-        atomic_add(&Fnext[n], (R)spread_term(f0 * sp.w[q], L));
-        atomic_add(&Fnext[S + n], (R)spread_term(f1 * sp.w[q], L));
-        atomic_add(&Fnext[2 * S + n], (R)spread_term(f2 * sp.w[q], L));
+        atomic_add(&Fnext[n], (R)spread_term((double)(g0 * sp.w[q]), L));
+        atomic_add(&Fnext[S + n], (R)spread_term((double)(g1 * sp.w[q]), L));
+        atomic_add(&Fnext[2 * S + n], (R)spread_term((double)(g2 * sp.w[q]), L));
       }
     }
     return;
@@ -357,16 +362,17 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       const int vtx = bucketed ? (int)vlist[first + j] : j;
       double f0, f1, f2;
       spread_force(base + vtx, f0, f1, f2);
-      Support sp;
+      const SW g0 = (SW)f0, g1 = (SW)f1, g2 = (SW)f2;
+      SupportT<SW> sp;
       ibm_support(pos_x(vtx), pos_y(vtx), pos_z(vtx), L, type, sp);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int lx = sp.lo[0] + (q >> 2) - o0 - xs;
-        if (sp.w[q] == 0.0 || lx < 0 || lx >= nxp) continue;
+        if (sp.w[q] == SW(0) || lx < 0 || lx >= nxp) continue;
         const int e = (sp.lo[2] + (q & 1) - o2) + s2 * ((sp.lo[1] + ((q >> 1) & 1) - o1) + s1 * lx);
-        lds_add(&box[e], spread_term(f0 * sp.w[q], L));
-        lds_add(&box[total + e], spread_term(f1 * sp.w[q], L));
-        lds_add(&box[2 * total + e], spread_term(f2 * sp.w[q], L));
+        lds_add(&box[e], spread_term((double)(g0 * sp.w[q]), L));
+        lds_add(&box[total + e], spread_term((double)(g1 * sp.w[q]), L));
+        lds_add(&box[2 * total + e], spread_term((double)(g2 * sp.w[q]), L));
       }
     }
     __syncthreads();

@@ -2,6 +2,7 @@
 // IBM kernels and the fused per-cell step kernel.
 #pragma once
 #include <cmath>
+#include <type_traits>
 
 #include "hc_d3q19.h"
 #include "hc_launch.h"
@@ -21,23 +22,29 @@ __device__ __forceinline__ int64_t wrap_mod(int64_t c, int n) {
 // node lies outside a non-periodic domain, has zero weight or is a wall node);
 // adding the zero weights to the total changes nothing, so the normalised
 // weights are those of immersedBoundaryMethod.h:62-138.  Fixed slots keep
-// everything in registers.
-struct Support {
+// everything in registers.  W = double evaluates the weights in FP64 as the
+// reference does; W = float (the FP32-shifted form's cell kernels) takes the
+// fractional offsets p - floor(p) in FP64 and the weights from them in FP32,
+// where 1 - d and d are the two hat values of an axis.
+template <typename W>
+struct SupportT {
   uint32_t idx[8];
-  double w[8];
+  W w[8];
   int lo[3];  // unwrapped coordinates of the slot-0 node
 };
+using Support = SupportT<double>;
 
 __device__ __forceinline__ int64_t wrap_far(int64_t c, int n) {
   const double q = floor((double)c / (double)n);
   return c - (int64_t)q * n;
 }
 
+template <typename W>
 __device__ __forceinline__ void ibm_support(double p0, double p1, double p2, const LatticeDev& L,
-                                            const uint8_t* __restrict__ type, Support& s) {
+                                            const uint8_t* __restrict__ type, SupportT<W>& s) {
   const double p[3] = {p0, p1, p2};
   const int dims[3] = {L.nx, L.ny, L.nz};
-  double q[3][2];
+  W q[3][2];
   int wi[3][2];
   bool ok[3][2];
   bool finite = true;
@@ -47,10 +54,15 @@ __device__ __forceinline__ void ibm_support(double p0, double p1, double p2, con
     const double fl = finite ? floor(p[a]) : 0.0;
     const int64_t n0 = (int64_t)fl;
     s.lo[a] = (int)n0;
+    if (std::is_same<W, float>::value) {
+      const W d = (W)(p[a] - fl);  // in [0, 1)
+      q[a][0] = W(1) - d;
+      q[a][1] = d;
+    }
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const int64_t node = n0 + k;
-      q[a][k] = fmax(1.0 - fabs(p[a] - (double)node), 0.0);
+      if (!std::is_same<W, float>::value) q[a][k] = (W)fmax(1.0 - fabs(p[a] - (double)node), 0.0);
       if (node >= 0 && node < dims[a]) {
         wi[a][k] = (int)node;
         ok[a][k] = true;
@@ -63,7 +75,7 @@ __device__ __forceinline__ void ibm_support(double p0, double p1, double p2, con
       }
     }
   }
-  double total = 0.0;
+  W total = W(0);
 #pragma unroll
   for (int ix = 0; ix < 2; ++ix)
 #pragma unroll
@@ -72,16 +84,16 @@ __device__ __forceinline__ void ibm_support(double p0, double p1, double p2, con
       for (int iz = 0; iz < 2; ++iz) {
         const int slot = ix * 4 + iy * 2 + iz;
         const uint32_t n = (uint32_t)wi[2][iz] + (uint32_t)L.nz * ((uint32_t)wi[1][iy] + (uint32_t)L.ny * (uint32_t)wi[0][ix]);
-        const double ww = q[0][ix] * q[1][iy] * q[2][iz];
-        bool keep = finite && ok[0][ix] && ok[1][iy] && ok[2][iz] && ww != 0.0;
+        const W ww = q[0][ix] * q[1][iy] * q[2][iz];
+        bool keep = finite && ok[0][ix] && ok[1][iy] && ok[2][iz] && ww != W(0);
         if (keep && type) keep = type[n] == HC_NODE_FLUID;
         s.idx[slot] = n;
-        s.w[slot] = keep ? ww : 0.0;
+        s.w[slot] = keep ? ww : W(0);
         total += s.w[slot];
       }
-  const double coeff = 1.0 / total;
+  const W coeff = W(1) / total;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) s.w[k] = s.w[k] != 0.0 ? s.w[k] * coeff : 0.0;
+  for (int k = 0; k < 8; ++k) s.w[k] = s.w[k] != W(0) ? s.w[k] * coeff : W(0);
 }
 
 // One force contribution to the spread: rounded to a multiple of L.spread_q when
