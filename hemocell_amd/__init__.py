@@ -84,6 +84,9 @@ SIGNATURES = {
     "hc_set_material_timescale": (_ST, [_P, _INT, _INT]),
     "hc_set_force_limit": (_ST, [_P, ctypes.c_double]),
     "hc_set_exact_spread": (_ST, [_P, ctypes.c_int]),
+    "hc_get_removed_vertices": (_ST, [_P, ctypes.c_void_p]),
+    "hc_set_removed_vertices": (_ST, [_P, ctypes.c_void_p]),
+    "hc_get_incomplete_cells": (_ST, [_P, ctypes.c_void_p]),
     "hc_set_repulsion": (_ST, [_P, ctypes.c_double, ctypes.c_double, _INT]),
     "hc_set_extra_vertex_force": (_ST, [_P, _INT, _I64, _D]),
     "hc_step": (_ST, [_P, _INT]),
@@ -485,6 +488,23 @@ class Simulation:
     def set_repulsion_force(self, fr):
         fr = np.ascontiguousarray(fr, dtype=np.float64)
         self._chk(self.lib.hc_set_repulsion_force(self.h, _dp(fr)))
+
+    def removed_vertices(self):
+        """One bool per vertex: removed by wall contact."""
+        r = np.zeros(self.num_vertices, dtype=np.uint8)
+        self._chk(self.lib.hc_get_removed_vertices(self.h, r.ctypes.data))
+        return r.astype(bool)
+
+    def set_removed_vertices(self, removed):
+        r = np.ascontiguousarray(np.asarray(removed, dtype=bool).astype(np.uint8))
+        assert r.shape == (self.num_vertices,)
+        self._chk(self.lib.hc_set_removed_vertices(self.h, r.ctypes.data))
+
+    def incomplete_cells(self):
+        """One bool per cell (cell order): a vertex of it was removed by wall contact."""
+        r = np.zeros(self.lib.hc_num_cells(self.h), dtype=np.uint8)
+        self._chk(self.lib.hc_get_incomplete_cells(self.h, r.ctypes.data))
+        return r.astype(bool)
 
     def cells(self):
         """(type id, cell id) of every cell, in cell order."""

@@ -79,6 +79,7 @@ struct hc_ctx {
   std::vector<int64_t> cell_base;
   int64_t nv = 0, vcap = 0;
   double *x = nullptr, *v = nullptr, *fv = nullptr, *frep = nullptr, *bend = nullptr, *fbr = nullptr;
+  uint8_t* vflag = nullptr;  // per vertex: VFLAG_REMOVED
   int64_t* d_cell_base = nullptr;
   int32_t* d_cell_nv = nullptr;
   bool cells_dirty = true;
@@ -199,7 +200,7 @@ hc_status dalloc(hc_ctx* ctx, void** p, size_t bytes, const char* what) {
   return HC_OK;
 }
 
-VertexDev vdev(hc_ctx* c) { return VertexDev{c->x, c->v, c->fv, c->frep, c->fbr, c->nv, c->vcap}; }
+VertexDev vdev(hc_ctx* c) { return VertexDev{c->x, c->v, c->fv, c->frep, c->fbr, c->nv, c->vcap, c->vflag}; }
 
 void prof_begin(hc_ctx* c, int stage, hipEvent_t* start) {
   *start = nullptr;
@@ -261,9 +262,20 @@ hc_status grow_vertices(hc_ctx* ctx, int64_t need) {
         HC_TRY(hipMemcpyAsync(nw[a] + c * cap, arrs[a] + c * ctx->vcap, sizeof(double) * ctx->nv,
                               hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
   }
+  uint8_t* nflag = nullptr;
+  hc_status fs = dalloc(ctx, (void**)&nflag, (size_t)cap, "vertex flags");
+  if (fs != HC_OK) {
+    for (int b = 0; b < 6; ++b) hipFree(nw[b]);
+    return fs;
+  }
+  HC_TRY(hipMemsetAsync(nflag, 0, (size_t)cap, ctx->stream), "hipMemsetAsync");
+  if (ctx->vflag && ctx->nv > 0)
+    HC_TRY(hipMemcpyAsync(nflag, ctx->vflag, (size_t)ctx->nv, hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
   HC_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   for (int a = 0; a < 6; ++a)
     if (arrs[a]) hipFree(arrs[a]);
+  if (ctx->vflag) hipFree(ctx->vflag);
+  ctx->vflag = nflag;
   ctx->x = nw[0];
   ctx->v = nw[1];
   ctx->fv = nw[2];
@@ -494,6 +506,9 @@ hc_status do_visc_grid(hc_ctx* ctx) {
   hipEvent_t ev;
   prof_begin(ctx, HC_PROF_MECH, &ev);
   HC_TRY(launch_visc_clear(ctx->stream, ctx->d_visc, ctx->L.N), "interior viscosity classes");
+  HC_TRY(launch_delete_incomplete(ctx->stream, vdev(ctx), ctx->d_cell_base, ctx->d_cell_nv,
+                                  (int)ctx->cell_base.size()),
+         "delete incomplete cells");
   for (size_t t = 0; t < ctx->types.size(); ++t) {
     TypeBufs& tb = *ctx->types[t];
     if (!(iv_tau_of(ctx, t) > 0.0) || tb.ncells == 0) continue;
@@ -786,6 +801,7 @@ std::vector<unsigned char> launch_signature(const hc_ctx* c) {
   put(v, c->v);
   put(v, c->fv);
   put(v, c->frep);
+  put(v, c->vflag);
   put(v, c->bend);
   put(v, c->nv);
   put(v, c->vcap);
@@ -1339,7 +1355,7 @@ void hc_destroy(hc_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   void* ps[] = {ctx->d_queue, ctx->d_types, ctx->d_head,
                 ctx->f, ctx->Fibm[0], ctx->Fibm[1], ctx->ufield, ctx->d_cell_nv, ctx->d_extra_dense, ctx->type, ctx->vel_nodes, ctx->x, ctx->v, ctx->fv, ctx->frep, ctx->bend,
-                ctx->fbr, ctx->d_bflag, ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag,
+                ctx->fbr, ctx->vflag, ctx->d_bflag, ctx->d_cell_base, ctx->d_extra_ids, ctx->d_extra_f, ctx->d_wall_hits, ctx->d_flag,
                 ctx->d_visc, ctx->d_stamp, ctx->d_xold, ctx->d_le, ctx->d_batch[0], ctx->d_batch[1]};
   for (void* p : ps)
     if (p) hipFree(p);
@@ -2089,6 +2105,38 @@ hc_status hc_set_vertices(hc_ctx* ctx, const double* x, const double* v, const d
   if (x && (s = set_planes(ctx, ctx->x, x)) != HC_OK) return s;
   if (v && (s = set_planes(ctx, ctx->v, v)) != HC_OK) return s;
   if (f && (s = set_planes(ctx, ctx->fv, f)) != HC_OK) return s;
+  return HC_OK;
+}
+hc_status hc_get_removed_vertices(hc_ctx* ctx, uint8_t* removed) {
+  if (!ctx || (!removed && ctx->nv > 0)) return HC_INVALID_ARG;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  if (ctx->nv > 0)
+    HC_TRY(hipMemcpy(removed, ctx->vflag, (size_t)ctx->nv, hipMemcpyDeviceToHost), "hipMemcpy");
+  for (int64_t k = 0; k < ctx->nv; ++k) removed[k] = (removed[k] & VFLAG_REMOVED) ? 1 : 0;
+  return HC_OK;
+}
+hc_status hc_set_removed_vertices(hc_ctx* ctx, const uint8_t* removed) {
+  if (!ctx || (!removed && ctx->nv > 0)) return HC_INVALID_ARG;
+  hc_status s;
+  if (ctx->lattice_ready && (s = invalidate_spread(ctx)) != HC_OK) return s;
+  HC_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  std::vector<uint8_t> h((size_t)ctx->nv);
+  for (int64_t k = 0; k < ctx->nv; ++k) h[k] = removed[k] ? VFLAG_REMOVED : 0;
+  if (ctx->nv > 0) HC_TRY(hipMemcpy(ctx->vflag, h.data(), (size_t)ctx->nv, hipMemcpyHostToDevice), "hipMemcpy");
+  return HC_OK;
+}
+hc_status hc_get_incomplete_cells(hc_ctx* ctx, uint8_t* incomplete) {
+  if (!ctx || (!incomplete && !ctx->cell_base.empty())) return HC_INVALID_ARG;
+  std::vector<uint8_t> rm((size_t)ctx->nv);
+  hc_status s = hc_get_removed_vertices(ctx, rm.data());
+  if (s != HC_OK) return s;
+  const size_t nc = ctx->cell_base.size();
+  for (size_t c = 0; c < nc; ++c) {
+    const int64_t b = ctx->cell_base[c], e = c + 1 < nc ? ctx->cell_base[c + 1] : ctx->nv;
+    uint8_t any = 0;
+    for (int64_t k = b; k < e; ++k) any |= rm[k];
+    incomplete[c] = any;
+  }
   return HC_OK;
 }
 hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep) {

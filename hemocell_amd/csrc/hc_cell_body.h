@@ -113,9 +113,15 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   // ---- 1. interpolation and advance ------------------------------------------
   // A thread takes up to U of its vertices at once, so their position loads and
   // support-node gathers are in flight together.
+  // A vertex removed by wall contact (flag) stays where it is and spreads
+  // nothing; its cell is incomplete from then on (hemoCellParticleField.cpp:
+  // 566-588 removes it; applyConstitutiveModel, :633-675, skips cells with a
+  // vertex missing and zeroes the rest's forces).
   constexpr int U = 3;
+  bool lrem = false;
   for (int k0 = tid; k0 < nv; k0 += U * BS) {
     double xa[U][3], va[U][3];
+    bool gone[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = k0 + u * BS;
@@ -124,12 +130,23 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       xa[u][0] = V.x[i];
       xa[u][1] = V.x[vs + i];
       xa[u][2] = V.x[2 * vs + i];
+      gone[u] = V.flag && (V.flag[i] & VFLAG_REMOVED);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = k0 + u * BS;
       if (k >= nv) continue;
       const int64_t i = base + k;
+      if (gone[u]) {
+        lrem = true;
+        if (!GPOS) {
+          px[k] = (P)(xa[u][0] - org0);
+          py[k] = (P)(xa[u][1] - org1);
+          pz[k] = (P)(xa[u][2] - org2);
+        }
+        if (STAGED && visc) vx[k] = vy[k] = vz[k] = P(0);
+        continue;
+      }
       if (flags & CELL_INTERP) {
         SupportT<SW> sp;
         ibm_support(xa[u][0], xa[u][1], xa[u][2], L, type, sp);
@@ -154,7 +171,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = k0 + u * BS;
-      if (k >= nv) continue;
+      if (k >= nv || gone[u]) continue;
       const int64_t i = base + k;
       const double v0 = va[u][0], v1 = va[u][1], v2 = va[u][2];
       if (flags & CELL_INTERP) {
@@ -187,16 +204,27 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
           else if (L.periodic[a]) wc[a] = wrap_mod(cc, dims[a]);
           else out = true;
         }
-        if (!out && type[wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0])] != HC_NODE_FLUID)
+        if (!out && type[wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0])] != HC_NODE_FLUID) {
           atomicAdd(wall_hits, 1ull);
+          if (V.flag) V.flag[i] = VFLAG_REMOVED;
+          lrem = true;
+        }
       }
     }
   }
-  __syncthreads();
+  const bool incomplete = __syncthreads_or(lrem) != 0;
   const CellViewT<P> c{px, py, pz};
 
   // ---- 2. mechanics ------------------------------------------------------------
-  if (flags & CELL_MECH) {
+  if ((flags & CELL_MECH) && incomplete) {
+    for (int k = tid; k < nv; k += BS) {
+      const int64_t i = base + k;
+      V.f[i] = 0.0;
+      V.f[vs + i] = 0.0;
+      V.f[2 * vs + i] = 0.0;
+    }
+    __syncthreads();
+  } else if (flags & CELL_MECH) {
     red[tid] = volume_partial<BS>(c, T, tid);
     __syncthreads();
     for (int h = BS / 2; h > 0; h >>= 1) {
@@ -256,6 +284,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
         lmax[a] = max(lmax[a], cc);
       }
       const int64_t i = base + k;
+      if (incomplete && (V.flag[i] & VFLAG_REMOVED)) continue;
       double f0 = V.f[i], f1 = V.f[vs + i], f2 = V.f[2 * vs + i];
       bool store = false;
       if (extra) {
@@ -293,6 +322,10 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   const bool direct = bad || s0 <= 0 || plane > box_nodes;
   const int64_t S = L.stride;
   auto spread_force = [&](int64_t i, double& f0, double& f1, double& f2) {
+    if (incomplete && (V.flag[i] & VFLAG_REMOVED)) {
+      f0 = f1 = f2 = 0.0;
+      return;
+    }
     f0 = V.f[i];
     f1 = V.f[vs + i];
     f2 = V.f[2 * vs + i];

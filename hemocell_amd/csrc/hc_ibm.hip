@@ -52,6 +52,7 @@ __global__ __launch_bounds__(BLOCK) void k_spread(VertexDev V, R* __restrict__ F
                                                   LatticeDev L, double f_limit, int use_rep) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= V.n) return;
+  if (V.flag && (V.flag[i] & VFLAG_REMOVED)) return;  // removed by wall contact
   const int64_t vs = V.vs;
   Support sp;
   ibm_support(V.x[i], V.x[vs + i], V.x[2 * vs + i], L, type, sp);
@@ -141,6 +142,7 @@ __global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* 
                                                           unsigned long long* wall_hits) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= V.n) return;
+  if (V.flag && (V.flag[i] & VFLAG_REMOVED)) return;  // removed by wall contact: stays put
   const int64_t vs = V.vs;
   double x0 = V.x[i], x1 = V.x[vs + i], x2 = V.x[2 * vs + i];
   double v0, v1, v2;
@@ -184,8 +186,11 @@ __global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* 
       else if (L.periodic[a]) wc[a] = wrap_mod(c, dims[a]);
       else out = true;
     }
-    if (!out && type[wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0])] != HC_NODE_FLUID)
+    if (!out && type[wc[2] + (int64_t)L.nz * (wc[1] + (int64_t)L.ny * wc[0])] != HC_NODE_FLUID) {
+      // on a wall node: the vertex is removed (hemoCellParticleField.cpp:566-588)
       atomicAdd(wall_hits, 1ull);
+      if (V.flag) V.flag[i] = VFLAG_REMOVED;
+    }
   }
 }
 
@@ -194,6 +199,10 @@ __global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* 
 __device__ __forceinline__ void spread_force(const VertexDev& V, int64_t i, double f_limit, int use_rep, double& t0,
                                              double& t1, double& t2) {
   const int64_t vs = V.vs;
+  if (V.flag && (V.flag[i] & VFLAG_REMOVED)) {  // removed by wall contact: spreads nothing
+    t0 = t1 = t2 = 0.0;
+    return;
+  }
   double f0 = V.f[i], f1 = V.f[vs + i], f2 = V.f[2 * vs + i];
   const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
   if (fm > f_limit) {

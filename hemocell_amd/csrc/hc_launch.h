@@ -50,7 +50,12 @@ struct VertexDev {
   double* fbr;    // wall (boundary particle) repulsion force
   int64_t n;      // vertices
   int64_t vs;     // component stride
+  // per vertex: bit 0 set once the vertex landed on a wall node and was removed
+  // (HemoCellParticleField::advanceParticles); a cell with a removed vertex is
+  // incomplete: no mechanics, zero force, its other vertices still drift
+  uint8_t* flag;
 };
+constexpr uint8_t VFLAG_REMOVED = 1;
 template <typename R>
 hipError_t launch_spread(hipStream_t s, const VertexDev& V, R* Fibm, const uint8_t* type, const LatticeDev& L,
                          double f_limit, int use_repulsion);
@@ -169,6 +174,8 @@ hipError_t launch_visc_clear(hipStream_t s, uint8_t* visc, int64_t n);
 hipError_t launch_visc_grid(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells, int ncells,
                             const int64_t* cell_base, const LatticeDev& L, const uint8_t* type, int cls,
                             uint8_t* visc);
+hipError_t launch_delete_incomplete(hipStream_t s, const VertexDev& V, const int64_t* cell_base, const int32_t* cell_nv,
+                                    int ncells);
 hipError_t launch_visc_membrane(hipStream_t s, const VertexDev& V, const TypeDev& T, const int32_t* cells,
                                 int ncells, const int64_t* cell_base, const double* xold, const LatticeDev& L,
                                 const uint8_t* type, int cls, uint32_t* stamp, uint8_t* visc, bool apply);

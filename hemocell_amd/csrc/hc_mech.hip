@@ -42,6 +42,20 @@ __global__ __launch_bounds__(BS) void k_mechanics(VertexDev V, TypeDev T, const 
   const int64_t vs = V.vs;
   const int nv = T.nv;
   const bool visc = (MODEL != HC_MODEL_RBC) || (T.eta_m != 0.0);
+  // a cell that lost a vertex to a wall gets no mechanics and zero force
+  // (hemoCellParticleField.cpp:633-675 leaves incomplete cells out)
+  bool lrem = false;
+  if (V.flag)
+    for (int k = tid; k < nv; k += BS) lrem |= (V.flag[base + k] & VFLAG_REMOVED) != 0;
+  if (__syncthreads_or(lrem)) {
+    double* dst = out ? out : V.f;
+    for (int k = tid; k < nv; k += BS) {
+      dst[base + k] = 0.0;
+      dst[vs + base + k] = 0.0;
+      dst[2 * vs + base + k] = 0.0;
+    }
+    return;
+  }
   const double *vx, *vy, *vz;
   double *bx, *by, *bz;
   CellView c;

@@ -48,10 +48,11 @@ __device__ __forceinline__ int64_t node_index(int x, int y, int z, const Lattice
   return (int64_t)z + (int64_t)L.nz * ((int64_t)y + (int64_t)L.ny * x);
 }
 
-// nearest-node key of vertex i (N when it is outside the domain)
+// nearest-node key of vertex i (N when it is outside the domain or was removed by
+// wall contact: such a vertex is in no bin and repels nothing)
 __device__ __forceinline__ uint32_t vertex_key(const VertexDev& V, int64_t i, const LatticeDev& L, int c[3]) {
   const int dims[3] = {L.nx, L.ny, L.nz};
-  bool inside = true;
+  bool inside = !(V.flag && (V.flag[i] & VFLAG_REMOVED));
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const double p = V.x[a * V.vs + i];
@@ -132,7 +133,8 @@ __global__ void k_boundary_repulsion(VertexDev V, LatticeDev L, const uint8_t* _
   const int64_t vs = V.vs;
   const double p[3] = {V.x[i], V.x[vs + i], V.x[2 * vs + i]};
   double F0 = 0.0, F1 = 0.0, F2 = 0.0;
-  if (fabs(p[0]) < 1.0e9 && fabs(p[1]) < 1.0e9 && fabs(p[2]) < 1.0e9) {
+  const bool removed = V.flag && (V.flag[i] & VFLAG_REMOVED);
+  if (!removed && fabs(p[0]) < 1.0e9 && fabs(p[1]) < 1.0e9 && fabs(p[2]) < 1.0e9) {
     const int64_t c0 = (int64_t)floor(p[0] + 0.5), c1 = (int64_t)floor(p[1] + 0.5), c2 = (int64_t)floor(p[2] + 0.5);
     for (int dx = -1; dx <= 1; ++dx) {
       const int xx = wrap_axis(c0 + dx, L.nx, L.periodic[0]);

@@ -138,6 +138,8 @@ __global__ __launch_bounds__(VB) void k_visc_grid(VertexDev V, TypeDev T, const 
   double* pz = sp + 2 * nv;
   float4* pb = reinterpret_cast<float4*>(sp + ((3 * nv + 1) & ~1));
   const int64_t b = cell_base[cells[blockIdx.x]];
+  // a cell removed by k_delete_incomplete has no inner nodes
+  if (V.flag && (V.flag[b] & VFLAG_REMOVED)) return;
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int k = threadIdx.x; k < nv; k += VB) {
     const double p[3] = {V.x[b + k], V.x[V.vs + b + k], V.x[2 * V.vs + b + k]};
@@ -250,6 +252,20 @@ __global__ __launch_bounds__(VB) void k_visc_grid(VertexDev V, TypeDev T, const 
   }
 }
 
+// HemoCell::iterate deletes the incomplete cells (a vertex lost to a wall) before
+// the entire-grid pass (core/hemoCell.cpp:347-351, deleteIncompleteCells,
+// hemoCellParticleField.cpp:510-553): every vertex of such a cell is removed.
+// One workgroup per cell.
+__global__ __launch_bounds__(VB) void k_delete_incomplete(VertexDev V, const int64_t* __restrict__ cell_base,
+                                                          const int32_t* __restrict__ cell_nv) {
+  const int64_t b = cell_base[blockIdx.x];
+  const int nv = cell_nv[blockIdx.x];
+  bool lrem = false;
+  for (int k = threadIdx.x; k < nv; k += VB) lrem |= (V.flag[b + k] & VFLAG_REMOVED) != 0;
+  if (__syncthreads_or(lrem))
+    for (int k = threadIdx.x; k < nv; k += VB) V.flag[b + k] = VFLAG_REMOVED;
+}
+
 template <bool APPLY>
 __global__ __launch_bounds__(VB) void k_visc_membrane(VertexDev V, TypeDev T, const int32_t* __restrict__ cells,
                                                       const int64_t* __restrict__ cell_base,
@@ -260,6 +276,7 @@ __global__ __launch_bounds__(VB) void k_visc_membrane(VertexDev V, TypeDev T, co
   const CellView cv{V.x + b, V.x + V.vs + b, V.x + 2 * V.vs + b};
   for (int k = threadIdx.x; k < T.nv; k += VB) {
     const int64_t i = b + k;
+    if (V.flag && (V.flag[i] & VFLAG_REMOVED)) continue;  // removed by wall contact
     // rbcHighOrderModel.cpp:114-121: sum of unit normal x area / area_mean_eq
     // over the vertex's triangles, ascending; zero for the other models
     double nx = 0.0, ny = 0.0, nz = 0.0;
@@ -341,6 +358,13 @@ hipError_t launch_visc_grid(hipStream_t s, const VertexDev& V, const TypeDev& T,
     hipLaunchKernelGGL((k_visc_grid<false>), dim3((unsigned)ncells), dim3(VB), pos, s, V, T, cells, cell_base, L,
                        type, (uint8_t)cls, visc);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_delete_incomplete(hipStream_t s, const VertexDev& V, const int64_t* cell_base, const int32_t* cell_nv,
+                                    int ncells) {
+  if (ncells <= 0 || !V.flag) return hipSuccess;
+  hipLaunchKernelGGL(k_delete_incomplete, dim3((unsigned)ncells), dim3(VB), 0, s, V, cell_base, cell_nv);
   return hipGetLastError();
 }
 

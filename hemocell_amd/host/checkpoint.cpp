@@ -10,7 +10,9 @@
 //                      cell its type and id, then per vertex position,
 //                      velocity, force and repulsion force (the fields of
 //                      HemoCellParticle::serializeValues_t,
-//                      core/hemoCellParticle.h:45-63)
+//                      core/hemoCellParticle.h:45-63), then one byte per
+//                      vertex, 1 for a vertex removed by wall contact (the
+//                      reference saves only the particles that exist)
 //   internalViscosity.dat  (with interior viscosity on) header + the tau inside
 //                      membranes per node, 0 elsewhere (InteriorViscosityHelper::
 //                      checkpoint / restore, helper/interiorViscosity.cpp:82-133)
@@ -180,6 +182,9 @@ void HemoCell::saveCheckPoint() {
     put_vec(o, v);
     put_vec(o, fv);
     put_vec(o, fr);
+    std::vector<uint8_t> removed((size_t)nv, 0);
+    if (nv > 0) check(hc_get_removed_vertices(ctx, removed.data()), "saveCheckPoint: removed vertices");
+    put_vec(o, removed);
     if (!o) throw HemoCellError("saveCheckPoint: cannot write " + dir + "particleField.dat");
   }
 
@@ -268,6 +273,8 @@ void HemoCell::loadCheckPoint() {
     get_vec(i, v, file);
     get_vec(i, fv, file);
     get_vec(i, fr, file);
+    std::vector<uint8_t> removed((size_t)nv, 0);
+    if (i.peek() != std::char_traits<char>::eof()) get_vec(i, removed, file);  // absent in older files
     // cells go back in their saved order, one hc_add_cells per run of one type
     int64_t off = 0;
     for (int c = 0; c < nc;) {
@@ -286,6 +293,7 @@ void HemoCell::loadCheckPoint() {
     if (nv > 0) {
       check(hc_set_vertices(ctx, x.data(), v.data(), fv.data()), "loadCheckPoint: vertices");
       check(hc_set_repulsion_force(ctx, fr.data()), "loadCheckPoint: repulsion force");
+      check(hc_set_removed_vertices(ctx, removed.data()), "loadCheckPoint: removed vertices");
     }
   }
   bool iv = false;

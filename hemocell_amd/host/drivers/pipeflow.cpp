@@ -63,14 +63,18 @@ static int run(int argc, char* argv[]) {
   for (int i = 0; i < 100; ++i) hemocell.lattice->collideAndStream();
   const unsigned cells0 = hemo::CellInformationFunctionals::getTotalNumberOfCells(&hemocell);
   double vmin = 1e300, vmax = -1e300, fmax = 0;
-  unsigned cmin = cells0, cmax = cells0, checks = 0;
+  unsigned cmin = cells0, cmax = cells0, checks = 0, cells_at_100 = cells0;
+  long last_loss = -1;
   while (hemocell.iter < tmax) {
     hemocell.iterate();
     hemo::setExternalVector(*hemocell.lattice, hemocell.lattice->getBoundingBox(), 0, driving);
-    if (hemocell.iter < 100 || hemocell.iter % every) continue;
+    // the number of cells after every iteration (test_pipeflow.cpp:84-88)
     const unsigned count = hemo::CellInformationFunctionals::getTotalNumberOfCells(&hemocell);
+    if (count < cmin) last_loss = (long)hemocell.iter;
     cmin = std::min(cmin, count);
     cmax = std::max(cmax, count);
+    if (hemocell.iter == 100) cells_at_100 = count;
+    if (hemocell.iter < 100 || hemocell.iter % every) continue;
     const hemo::FluidStatistics fi = hemo::FluidInfo::calculateVelocityStatistics(&hemocell);
     const double viscosity = 0.5 * hemo::param::u_lbm_max / fi.avg;
     vmin = std::min(vmin, viscosity);
@@ -79,10 +83,12 @@ static int run(int argc, char* argv[]) {
     fmax = std::max(fmax, pi.avg * hemo::param::df * 1.0e12);
     ++checks;
   }
-  std::printf("{\"cells_loaded\": %u, \"cells_min\": %u, \"cells_max\": %u, \"viscosity_min\": %.17g, "
+  std::printf("{\"cells_loaded\": %u, \"cells_min\": %u, \"cells_max\": %u, \"cells_at_100\": %u, "
+              "\"last_loss_iteration\": %ld, \"removed_vertices\": %lld, \"viscosity_min\": %.17g, "
               "\"viscosity_max\": %.17g, \"mean_force_pN_max\": %.17g, \"checks\": %u, \"pipe_radius\": %.17g, "
               "\"iteration\": %u}\n",
-              cells0, cmin, cmax, vmin, vmax, fmax, checks, hemo::param::pipe_radius, hemocell.iter);
+              cells0, cmin, cmax, cells_at_100, last_loss, (long long)hc_wall_hits(hemocell.context()), vmin, vmax,
+              fmax, checks, hemo::param::pipe_radius, hemocell.iter);
   return 0;
 }
 

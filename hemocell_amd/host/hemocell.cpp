@@ -714,7 +714,8 @@ void HemoCell::setSystemPeriodicity(unsigned int axis, bool bePeriodic) {
 // per cell its centre (um) and three rotation angles (degrees, applied about
 // x, then y, then z around the mesh's bounding-box centre).  Vertices whose
 // nearest node is a wall, or lies within the type's minimum distance from a
-// wall, are dropped together with their cell.  Cell ids run over all types in
+// wall, are dropped together with their cell (nodes beyond a periodic face by
+// their periodic image).  Cell ids run over all types in
 // registration order, one per line.
 void HemoCell::loadParticles() {
   if (!cellfields) throw HemoCellError("loadParticles: initializeCellfield and addCellType first");
@@ -788,10 +789,17 @@ void HemoCell::loadParticles() {
         }
         T v[3];
         for (int c = 0; c < 3; ++c) v[c] = p[c] * posRatio + (rq[c] + ctr[c]);
-        // nearest node, truncated as int(v + 0.5)
+        // nearest node, truncated as int(v + 0.5); on a periodic axis a node
+        // beyond the domain is its periodic image (the reference reads it from
+        // the lattice envelope, which holds that image)
         const plint nx = (plint)(v[0] + 0.5), ny = (plint)(v[1] + 0.5), nz = (plint)(v[2] + 0.5);
+        auto image = [&](plint c, int axis, plint n) -> plint {
+          if (c >= 0 && c < n) return c;
+          return L.periodicity().get(axis) ? ((c % n) + n) % n : -1;
+        };
         auto boundary_at = [&](plint x, plint y, plint z) {
-          return x >= 0 && y >= 0 && z >= 0 && x < L.getNx() && y < L.getNy() && z < L.getNz() && L.isBoundary(x, y, z);
+          const plint X = image(x, 0, L.getNx()), Y = image(y, 1, L.getNy()), Z = image(z, 2, L.getNz());
+          return X >= 0 && Y >= 0 && Z >= 0 && L.isBoundary(X, Y, Z);
         };
         if (boundary_at(nx, ny, nz)) complete = false;
         for (int dx = -deny; dx <= deny && complete; ++dx)
@@ -896,13 +904,39 @@ ParticleStatistics ParticleInfo::calculateVelocityStatistics(HemoCell* hemocell)
 // ============================================================== cellInfo ====
 std::map<int, CellInformation> CellInformationFunctionals::info_per_cell;
 
+// deleteIncompleteCells (core/hemoCellParticleField.cpp:510-553), which the
+// cellInfo functions call first (helper/cellInfo.cpp:271-319): every vertex of a
+// cell that lost one to a wall is removed.  Returns one flag per cell.
+static std::vector<uint8_t> delete_incomplete_cells(HemoCell* h) {
+  hc_ctx* ctx = h->context();
+  const int nc = hc_num_cells(ctx);
+  std::vector<uint8_t> inc((size_t)nc, 0);
+  if (nc == 0) return inc;
+  h->check(hc_get_incomplete_cells(ctx, inc.data()), "deleteIncompleteCells");
+  if (std::find(inc.begin(), inc.end(), 1) == inc.end()) return inc;
+  const int64_t nv = hc_num_vertices(ctx);
+  std::vector<uint8_t> rm((size_t)nv);
+  h->check(hc_get_removed_vertices(ctx, rm.data()), "deleteIncompleteCells");
+  int64_t b = 0;
+  for (int c = 0; c < nc; ++c) {
+    const int n = (*h->cellfields)[(pluint)h->cellfields->cell_types.at(c)]->numVertex;
+    if (inc[(size_t)c])
+      for (int k = 0; k < n; ++k) rm[(size_t)(b + k)] = 1;
+    b += n;
+  }
+  h->check(hc_set_removed_vertices(ctx, rm.data()), "deleteIncompleteCells");
+  return inc;
+}
+
 void CellInformationFunctionals::fill(HemoCell* h) {
   hc_ctx* ctx = h->context();
   const int nc = hc_num_cells(ctx);
+  const std::vector<uint8_t> gone = h->cellfields ? delete_incomplete_cells(h) : std::vector<uint8_t>((size_t)nc, 0);
   std::vector<T> vol(nc), area(nc), pos(3 * nc), vel(3 * nc), bbox(6 * nc), stretch(nc);
   h->check(hc_get_cell_stats(ctx, vol.data(), area.data(), pos.data(), vel.data(), bbox.data(), stretch.data()),
            "cell information");
   for (int c = 0; c < nc; ++c) {
+    if (gone[(size_t)c]) continue;
     const int id = h->cellfields ? h->cellfields->cell_ids.at(c) : c;
     CellInformation& ci = info_per_cell[id];
     ci.volume = vol[c];
@@ -918,12 +952,17 @@ void CellInformationFunctionals::fill(HemoCell* h) {
 }
 
 unsigned int CellInformationFunctionals::getTotalNumberOfCells(HemoCell* hemocell) {
-  return (unsigned)hc_num_cells(hemocell->context());
+  if (!hemocell->cellfields) return (unsigned)hc_num_cells(hemocell->context());
+  const std::vector<uint8_t> gone = delete_incomplete_cells(hemocell);
+  return (unsigned)std::count(gone.begin(), gone.end(), 0);
 }
 
 unsigned int CellInformationFunctionals::getNumberOfCellsFromType(HemoCell* hemocell, std::string type) {
   const int ct = (*hemocell->cellfields)[type]->ctype;
-  return (unsigned)std::count(hemocell->cellfields->cell_types.begin(), hemocell->cellfields->cell_types.end(), ct);
+  const std::vector<uint8_t> gone = delete_incomplete_cells(hemocell);
+  unsigned n = 0;
+  for (size_t c = 0; c < gone.size(); ++c) n += !gone[c] && hemocell->cellfields->cell_types[c] == ct;
+  return n;
 }
 
 // ========================================================== cell stretch ====

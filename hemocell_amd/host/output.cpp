@@ -152,12 +152,29 @@ void HemoCell::writeOutput() {
       base[c] = b;
       b += (*cellfields)[(pluint)cellfields->cell_types[c]]->numVertex;
     }
+    // Cells that lost a vertex to a wall are incomplete: the reference deletes
+    // them before it writes the positions of their type
+    // (hemoCellParticleFieldOutputFunctions.cpp:51-52, deleteIncompleteCells) and
+    // writes no incomplete cell.  Deleting = removing every vertex of the cell.
+    std::vector<uint8_t> incomplete(base.size(), 0);
+    if (!base.empty()) check(hc_get_incomplete_cells(ctx, incomplete.data()), "writeOutput");
+    bool drop = false;
+    std::vector<uint8_t> removed((size_t)nv, 0);
+    for (size_t c = 0; c < base.size(); ++c) {
+      if (!incomplete[c]) continue;
+      const HemoCellField& F = *(*cellfields)[(pluint)cellfields->cell_types[c]];
+      if (std::find(F.outputs.begin(), F.outputs.end(), (int)OUTPUT_POSITION) == F.outputs.end()) continue;
+      if (!drop) check(hc_get_removed_vertices(ctx, removed.data()), "writeOutput");
+      drop = true;
+      for (int k = 0; k < F.numVertex; ++k) removed[(size_t)(base[c] + k)] = 1;
+    }
+    if (drop) check(hc_set_removed_vertices(ctx, removed.data()), "writeOutput");
     for (auto& fp : cellfields->fields) {
       HemoCellField& F = *fp;
       if (F.outputs.empty()) continue;
       std::vector<size_t> cells;
       for (size_t c = 0; c < base.size(); ++c)
-        if (cellfields->cell_types[c] == F.ctype) cells.push_back(c);
+        if (cellfields->cell_types[c] == F.ctype && !incomplete[c]) cells.push_back(c);
       const hsize_t rows = (hsize_t)cells.size() * F.numVertex;
       hid_t file = open_file(out + "hdf5/" + zero_pad(it) + "/" + F.name + "." + zero_pad(it) + ".p.0.h5", dx, dt, it);
       long nproc = 1;

@@ -302,6 +302,18 @@ hc_status hc_get_fluid(hc_ctx* ctx, double* rho, double* u /* 3*N, n*3+c */);
 hc_status hc_get_vertices(hc_ctx* ctx, double* x, double* v, double* f);
 hc_status hc_set_vertices(hc_ctx* ctx, const double* x, const double* v, const double* f);
 hc_status hc_get_repulsion_force(hc_ctx* ctx, double* frep);
+/* Replaces: the particle removal of HemoCellParticleField::advanceParticles (core/
+ * hemoCellParticleField.cpp:566-588) and the incomplete-cell handling of
+ * applyConstitutiveModel (:633-675) and deleteIncompleteCells (:510-553).  A vertex whose
+ * nearest node is a wall node after its advance is removed: it stops moving and spreads
+ * nothing (and counts once in hc_wall_hits).  Its cell is incomplete from then on: no
+ * mechanics, zero force on its other vertices at each material step, which keep drifting
+ * with the fluid; at an interior-viscosity grid pass the whole cell is removed.
+ * removed: one byte per vertex, 1 if removed (get) / to remove (set, for a restart);
+ * incomplete: one byte per cell, 1 if any of its vertices was removed. */
+hc_status hc_get_removed_vertices(hc_ctx* ctx, uint8_t* removed);
+hc_status hc_set_removed_vertices(hc_ctx* ctx, const uint8_t* removed);
+hc_status hc_get_incomplete_cells(hc_ctx* ctx, uint8_t* incomplete);
 /* the wall repulsion force of every vertex, n*3+c */
 hc_status hc_get_boundary_repulsion_force(hc_ctx* ctx, double* fbr);
 /* Replaces: the force_repulsion member restored with the particle field

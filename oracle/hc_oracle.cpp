@@ -429,6 +429,10 @@ struct Oracle {
   // vertices (SoA over all cells, cells contiguous, types in registration order)
   std::vector<double> x, v, force, frep;  // 3*NV each, AoS per vertex (x[3*i+c])
   std::vector<int> vcell;                 // cell index of each vertex
+  // 1 for a vertex removed by wall contact (advanceParticles,
+  // hemoCellParticleField.cpp:566-588: the particle is deleted); its cell is
+  // incomplete from then on
+  std::vector<uint8_t> removed;
   std::vector<CellType> types;
   std::vector<int> cell_type, cell_base;  // per cell
   std::vector<Kernel> kern;               // per vertex, built by the spread
@@ -559,6 +563,10 @@ struct Oracle {
     kern.resize(NV());
     for (long i = 0; i < NV(); ++i) {
       Kernel& k = kern[i];
+      if (removed[i]) {  // no particle, no contribution
+        k.n = 0;
+        continue;
+      }
       make_kernel(&x[3 * i], k);
       double* fv = &force[3 * i];
       double fm = std::sqrt(fv[0] * fv[0] + fv[1] * fv[1] + fv[2] * fv[2]);
@@ -579,6 +587,7 @@ struct Oracle {
   void interpolate() {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < NV(); ++i) {
+      if (removed[i]) continue;
       Kernel k;
       make_kernel(&x[3 * i], k);
       double vel[3] = {0, 0, 0};
@@ -591,8 +600,11 @@ struct Oracle {
     }
   }
 
+  // A vertex whose nearest node is a wall node after its advance is removed
+  // (hemoCellParticleField.cpp:566-588; the nearest node by floor(x + 0.5)).
   void advance() {
     for (long i = 0; i < NV(); ++i) {
+      if (removed[i]) continue;
       for (int c = 0; c < 3; ++c) x[3 * i + c] += v[3 * i + c];
       int w3[3];
       bool out = false;
@@ -600,8 +612,28 @@ struct Oracle {
         w3[a] = wrap(a, (long)std::floor(x[3 * i + a] + 0.5));
         if (w3[a] < 0) out = true;
       }
-      if (!out && is_boundary(type[idx(w3[0], w3[1], w3[2])])) wall_hits++;
+      if (!out && is_boundary(type[idx(w3[0], w3[1], w3[2])])) {
+        wall_hits++;
+        removed[i] = 1;
+      }
     }
+  }
+
+  bool incomplete(int cell) const {
+    const long b = cell_base[cell];
+    for (int k = 0; k < types[cell_type[cell]].nv; ++k)
+      if (removed[b + k]) return true;
+    return false;
+  }
+
+  // deleteIncompleteCells (hemoCellParticleField.cpp:510-553): every vertex of
+  // an incomplete cell goes
+  void delete_incomplete_cells() {
+    for (int c = 0; c < (int)cell_type.size(); ++c)
+      if (incomplete(c)) {
+        const long b = cell_base[c];
+        for (int k = 0; k < types[cell_type[c]].nv; ++k) removed[b + k] = 1;
+      }
   }
 
   void mechanics_cell(int cell) {
@@ -771,9 +803,12 @@ struct Oracle {
       for (int k = 0; k < types[cell_type[c]].nv; ++k)
         for (int q = 0; q < 3; ++q) force[3 * (b + k) + q] = 0.0;
     }
+    // applyConstitutiveModel (hemoCellParticleField.cpp:633-675): the forces of
+    // every vertex of a due type are reset, the model then runs on the complete
+    // cells only
 #pragma omp parallel for schedule(dynamic)
     for (int c = 0; c < nc; ++c)
-      if (due[cell_type[c]]) mechanics_cell(c);
+      if (due[cell_type[c]] && !incomplete(c)) mechanics_cell(c);
   }
 
   // Cell-cell repulsion over nearest-node bins (hemoCellParticleField.cpp:677-743):
@@ -786,6 +821,7 @@ struct Oracle {
     for (long i = 0; i < 3 * nv; ++i) frep[i] = 0.0;
     std::vector<std::vector<long>> bin(N);
     for (long i = 0; i < nv; ++i) {
+      if (removed[i]) continue;
       int w3[3];
       bool out = false;
       for (int a = 0; a < 3; ++a) {
@@ -863,6 +899,7 @@ struct Oracle {
     fbr.assign(3 * nv, 0.0);
     std::vector<std::vector<long>> bin(N);
     for (long i = 0; i < nv; ++i) {
+      if (removed[i]) continue;
       int w3[3];
       bool out = false;
       for (int a = 0; a < 3; ++a) {
@@ -948,7 +985,7 @@ struct Oracle {
     const int nc = (int)cell_type.size();
     for (int c = 0; c < nc; ++c) {
       const int t = cell_type[c];
-      if (!(ivt(t) > 0.0)) continue;
+      if (!(ivt(t) > 0.0) || incomplete(c)) continue;
       const CellType& ct = types[t];
       const long b = cell_base[c];
       double lo[3], hi[3];
@@ -994,7 +1031,7 @@ struct Oracle {
     std::vector<std::vector<std::vector<int>>> vt(types.size());
     for (long i = 0; i < NV(); ++i) {
       const int c = vcell[i], t = cell_type[c];
-      if (!(ivt(t) > 0.0)) continue;
+      if (!(ivt(t) > 0.0) || removed[i]) continue;
       const CellType& ct = types[t];
       const long b = cell_base[c];
       const int k = (int)(i - b);
@@ -1137,7 +1174,10 @@ struct Oracle {
     advance();
     mechanics(false);
     // hemoCell.cpp:347-357: after the constitutive model
-    if (iv && iter % iv_grid_timescale == 0) iv_entire_grid();
+    if (iv && iter % iv_grid_timescale == 0) {
+      delete_incomplete_cells();  // hemoCell.cpp:347-351
+      iv_entire_grid();
+    }
     if (iv && iter % iv_timescale == 0) iv_membrane(xold);
     reset_force_field();
     iter++;
@@ -1216,6 +1256,7 @@ int ho_add_cells(void* h, int t, int ncells, const double* xyz) {
         o->frep.push_back(0.0);
       }
       o->vcell.push_back(cell);
+      o->removed.push_back(0);
     }
     if (o->brep) o->fbr.resize(3 * o->NV(), 0.0);
   }
@@ -1253,6 +1294,14 @@ int ho_num_threads() {
 }
 long ho_num_vertices(void* h) { return ((Oracle*)h)->NV(); }
 long ho_wall_hits(void* h) { return ((Oracle*)h)->wall_hits; }
+void ho_get_removed(void* h, uint8_t* out) {
+  Oracle* o = (Oracle*)h;
+  std::copy(o->removed.begin(), o->removed.end(), out);
+}
+void ho_set_removed(void* h, const uint8_t* in) {
+  Oracle* o = (Oracle*)h;
+  for (long i = 0; i < o->NV(); ++i) o->removed[i] = in[i] ? 1 : 0;
+}
 
 void ho_get_populations(void* h, double* f) { Oracle* o = (Oracle*)h; std::copy(o->f.begin(), o->f.end(), f); }
 void ho_set_populations(void* h, const double* f) { Oracle* o = (Oracle*)h; std::copy(f, f + 19 * o->N, o->f.begin()); }

@@ -35,6 +35,8 @@ _SIG = {
     "ho_num_threads": (_INT, []),
     "ho_num_vertices": (ctypes.c_long, [_P]),
     "ho_wall_hits": (ctypes.c_long, [_P]),
+    "ho_get_removed": (None, [_P, ctypes.c_void_p]),
+    "ho_set_removed": (None, [_P, ctypes.c_void_p]),
     "ho_get_populations": (None, [_P, _D]),
     "ho_set_populations": (None, [_P, _D]),
     "ho_get_force_field": (None, [_P, _D]),
@@ -205,6 +207,9 @@ class OracleSim:
         x = np.ascontiguousarray(xyz, dtype=np.float64)
         ncells = x.shape[0] if x.ndim == 3 else 1
         self.lib.ho_add_cells(self.h, type_id, ncells, _dp(x))
+        if not hasattr(self, "_cell_nv"):
+            self._cell_nv = []
+        self._cell_nv += [x.shape[-2]] * ncells
 
     def type_tables(self, t):
         s = np.zeros(4, dtype=np.int32)
@@ -267,6 +272,24 @@ class OracleSim:
 
     def wall_hits(self):
         return self.lib.ho_wall_hits(self.h)
+
+    def removed_vertices(self):
+        """One bool per vertex: removed by wall contact (advanceParticles)."""
+        r = np.zeros(self.num_vertices, dtype=np.uint8)
+        self.lib.ho_get_removed(self.h, r.ctypes.data)
+        return r.astype(bool)
+
+    def set_removed_vertices(self, removed):
+        r = np.ascontiguousarray(np.asarray(removed, dtype=bool).astype(np.uint8))
+        assert r.shape == (self.num_vertices,)
+        self.lib.ho_set_removed(self.h, r.ctypes.data)
+
+    def incomplete_cells(self):
+        """One bool per cell: a vertex of it was removed."""
+        r = self.removed_vertices()
+        nv = getattr(self, "_cell_nv", [])
+        starts = np.concatenate([[0], np.cumsum(nv)])[:-1].astype(np.int64)
+        return np.array([r[s:s + n].any() for s, n in zip(starts, nv)], dtype=bool)
 
     def ibm_kernel(self, p):
         idx = np.zeros(27, dtype=np.int64)

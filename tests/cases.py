@@ -220,13 +220,14 @@ def config_xml(dx=5e-7, dt=1e-7, nu_p=1.1e-6, rho_p=1025.0, kBT=KBT):
             '<sim> <tmax> 10 </tmax> </sim>\n</hemocell>\n' % (rho_p, nu_p, dx, dt, kBT))
 
 
-def pos_placement(mesh_v, lines, dx, boundary, deny_um, first_id):
+def pos_placement(mesh_v, lines, dx, boundary, deny_um, first_id, periodic=(False, False, False)):
     """Cells placed as io/readPositionsBloodCells.cpp:36-361 does: the mesh is
     centred on its bounding box, rotated by the negated angles (degrees) with
     the matrices of rotateTriangularMeshXYZ about its bounding-box centre and
     moved to the listed centre (um).  A cell is dropped when the nearest node
     int(v + 0.5) of one of its vertices, or a node within the deny layer
-    around it, is a boundary node.  Returns (ids, positions) of the kept cells."""
+    around it, is a boundary node (beyond a periodic face: its periodic
+    image).  Returns (ids, positions) of the kept cells."""
     lo, hi = mesh_v.min(axis=0), mesh_v.max(axis=0)
     base = mesh_v - (lo + hi) / 2.0
     ctr = (base.max(axis=0) + base.min(axis=0)) * 0.5
@@ -263,8 +264,11 @@ def pos_placement(mesh_v, lines, dx, boundary, deny_um, first_id):
             for dxx in range(-deny, deny + 1):
                 for dyy in range(-deny, deny + 1):
                     for dzz in range(-deny, deny + 1):
-                        m = (n[0] + dxx, n[1] + dyy, n[2] + dzz)
-                        if all(0 <= m[a] < dims[a] for a in range(3)) and boundary[m]:
+                        m = [n[0] + dxx, n[1] + dyy, n[2] + dzz]
+                        for a in range(3):
+                            if periodic[a] and not 0 <= m[a] < dims[a]:
+                                m[a] %= dims[a]
+                        if all(0 <= m[a] < dims[a] for a in range(3)) and boundary[tuple(m)]:
                             keep = False
             if not keep:
                 break
@@ -272,3 +276,35 @@ def pos_placement(mesh_v, lines, dx, boundary, deny_um, first_id):
             ids.append(first_id + k)
             cells.append(v)
     return ids, cells
+
+
+WALL_SHAPE = (24, 20, 24)
+
+
+def wall_contact_case(sims, u, builder, u0=-0.004, gap=0.3):
+    """Bounce-back walls y = 0 and y = 19 (periodic x, z), the fluid set moving
+    towards y = 0 at u0: an RBC lying flat with its lowest vertex `gap` spacings
+    above the y = 0 wall (the disc's thin axis along y), so the vertices below
+    y = 0.5 have a wall node as nearest node after the first advance, and a
+    platelet in the middle of the channel, clear of both walls.  Returns the two
+    cells' vertex ranges."""
+    model, v, t, p, inner = mesh("RBC", u, builder)
+    c = v - v.mean(axis=0)
+    ext = c.max(axis=0) - c.min(axis=0)
+    thin = int(np.argmin(ext))
+    perm = [0, 1, 2]
+    perm[1], perm[thin] = perm[thin], perm[1]
+    c = c[:, perm]
+    c[:, 0] += WALL_SHAPE[0] / 2.0
+    c[:, 2] += WALL_SHAPE[2] / 2.0
+    c[:, 1] += gap - c[:, 1].min()
+    pm, pv, pt, pp, pinner = mesh("PLT", u, builder)
+    xp = place(pv, (8.0, 12.0, 9.0))
+    for s in sims:
+        s.init_equilibrium(1.0, (0.0, u0, 0.0))
+        s.set_force_limit(u.f_limit)
+        tr = s.add_cell_type(model, v, t, p, inner)
+        s.add_cells(tr, c[None])
+        tp = s.add_cell_type(pm, pv, pt, pp, pinner)
+        s.add_cells(tp, xp[None])
+    return (0, len(v)), (len(v), len(v) + len(pv))

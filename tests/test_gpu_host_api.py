@@ -85,8 +85,9 @@ def test_load_particles_places_cells_like_the_reference(tmp_path):
     mr = hc.build_mesh(1, cases.RBC_MAT["radius"] / u.dx, cases.RBC_MAT["minNumTriangles"])[0]
     mp = hc.build_mesh(6, cases.PLT_MAT["radius"] / u.dx, cases.PLT_MAT["minNumTriangles"],
                        cases.PLT_MAT["aspectRatio"])[0]
-    ids_r, cells_r = cases.pos_placement(mr, rbc, u.dx, boundary, 0.0, 0)
-    ids_p, cells_p = cases.pos_placement(mp, plt, u.dx, boundary, 1.0, len(rbc))
+    per = (True, False, True)
+    ids_r, cells_r = cases.pos_placement(mr, rbc, u.dx, boundary, 0.0, 0, per)
+    ids_p, cells_p = cases.pos_placement(mp, plt, u.dx, boundary, 1.0, len(rbc), per)
     assert ids_r == [0, 1, 3] and ids_p == [4, 6, 7]
     assert list(ids) == ids_r + ids_p
     assert list(types) == [0] * len(ids_r) + [1] * len(ids_p)
