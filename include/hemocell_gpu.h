@@ -362,8 +362,8 @@ hc_status hc_get_force_terms(hc_ctx* ctx, double* area, double* volume, double* 
  * not bounce-back of |u| (what 0, u as hc_get_fluid reports it) or of |F| (what 1, the
  * vertex forces spread into a reset field, capped in place as the spread does). */
 hc_status hc_get_fluid_stats(hc_ctx* ctx, int what, double out[4]);
-/* vertices whose nearest node became a wall node during advance (the reference deletes
- * their cells, hemoCellParticleField.cpp:570-584) */
+/* vertices removed so far because their nearest node became a wall node during their
+ * advance (hemoCellParticleField.cpp:566-588); each counts once (see hc_get_removed_vertices) */
 int64_t hc_wall_hits(hc_ctx* ctx);
 /* HC_NUMERIC if any population or vertex coordinate is NaN/Inf */
 hc_status hc_check_finite(hc_ctx* ctx);
