@@ -308,8 +308,8 @@ def test_full_step_mixed_suspension_body_force_fp64(persistent, monkeypatch):
         cases.rel_l2(xg, xo), cases.rel_l2(xg - x0, xo - x0), cases.rel_l2(vg, vo), cases.rel_l2(v2, vo)))
     assert cases.rel_l2(xg, xo) < 1e-10
     assert cases.rel_l2(xg - x0, xo - x0) < 1e-10
-    cond_v = cases.rel_l2(v2, vo)
-    assert cases.rel_l2(vg, vo) < max(1e-10, 10 * cond_v)
+    # measured: v 4e-13, the same as the oracle's own response to the 1e-15 nudge
+    assert cases.rel_l2(vg, vo) < 1e-10
     _, ug = g.fluid()
     _, uo = o.fluid()
     assert cases.rel_l2(ug, uo) < 1e-10
