@@ -76,6 +76,8 @@ struct hc_ctx {
   // cells
   std::vector<std::unique_ptr<TypeBufs>> types;
   std::vector<int32_t> cell_type, cell_id;
+  std::vector<double> cell_ctr;  // per cell: vertex mean when added (3 doubles), for the launch order
+  bool cell_order = true;        // launch cells in spatial order (HC_CELL_ORDER=0: in the order added)
   std::vector<int64_t> cell_base;
   int64_t nv = 0, vcap = 0;
   double *x = nullptr, *v = nullptr, *fv = nullptr, *frep = nullptr, *bend = nullptr, *fbr = nullptr;
@@ -304,11 +306,32 @@ hc_status sync_cells(hc_ctx* ctx) {
     for (size_t c = 0; c < nc; ++c) cnv[c] = ctx->types[ctx->cell_type[c]]->host.nv;
     if (nc) HC_TRY(hipMemcpy(ctx->d_cell_nv, cnv.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice), "hipMemcpy");
   }
+  // Launch order: the cells of a type by the 8-node tile of their centre, tiles in
+  // the lattice's x, y, z order (z fastest, as the node index runs).  Workgroups
+  // resident at the same time then cover a compact region: the support nodes
+  // they gather velocities from and spread forces into share cache lines.
+  std::vector<int64_t> key(nc, 0);
+  if (ctx->cell_order && ctx->lattice_ready) {
+    const int dims[3] = {ctx->L.nx, ctx->L.ny, ctx->L.nz};
+    int64_t tiles[3];
+    for (int a = 0; a < 3; ++a) tiles[a] = (dims[a] + 7) / 8;
+    for (size_t c = 0; c < nc; ++c) {
+      int64_t q[3];
+      for (int a = 0; a < 3; ++a) {
+        const double v = ctx->cell_ctr[3 * c + a];
+        int64_t n = std::isfinite(v) ? (int64_t)std::floor(v) : 0;
+        n = ((n % dims[a]) + dims[a]) % dims[a];
+        q[a] = n / 8;
+      }
+      key[c] = (q[0] * tiles[1] + q[1]) * tiles[2] + q[2];
+    }
+  }
   for (size_t t = 0; t < ctx->types.size(); ++t) {
     TypeBufs& tb = *ctx->types[t];
     std::vector<int32_t> list;
     for (size_t c = 0; c < nc; ++c)
       if (ctx->cell_type[c] == (int)t) list.push_back((int32_t)c);
+    std::stable_sort(list.begin(), list.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
     if (tb.d_cells) hipFree(tb.d_cells);
     tb.d_cells = nullptr;
     s = dalloc(ctx, (void**)&tb.d_cells, sizeof(int32_t) * std::max<size_t>(list.size(), 1), "cell list");
@@ -330,9 +353,12 @@ hc_status sync_cells(hc_ctx* ctx) {
                      [&](int a, int b) { return ctx->types[a]->host.nv > ctx->types[b]->host.nv; });
     std::vector<int2> q;
     q.reserve(nc);
-    for (int t : order)
+    for (int t : order) {
+      const size_t first = q.size();
       for (size_t c = 0; c < nc; ++c)
         if (ctx->cell_type[c] == t) q.push_back(make_int2((int)c, t));
+      std::stable_sort(q.begin() + first, q.end(), [&](int2 a, int2 b) { return key[a.x] < key[b.x]; });
+    }
     std::vector<TypeDev> td(ctx->types.size());
     for (size_t t = 0; t < td.size(); ++t) td[t] = ctx->types[t]->dev;
     if ((s = dalloc(ctx, (void**)&ctx->d_queue, sizeof(int2) * q.size(), "cell queue")) != HC_OK) return s;
@@ -1333,6 +1359,8 @@ hc_status hc_create(int device, hc_ctx** out) {
     ctx->n_cu = ncu;
   const char* fv = std::getenv("HC_FUSE_VEL");  // 0: velocity-wall nodes in their own kernel
   if (fv && std::string(fv) == "0") ctx->fuse_vel = false;
+  const char* co = std::getenv("HC_CELL_ORDER");  // tuning: 0 launches cells in the order added
+  if (co && std::string(co) == "0") ctx->cell_order = false;
   const char* wb = std::getenv("HC_WBC_BLOCK");  // tuning: threads per WBC cell
   if (wb) ctx->wbc_block = std::atoi(wb);
   const char* gr = std::getenv("HC_GRAPH");  // 0: launch every step eagerly
@@ -1586,6 +1614,10 @@ hc_status hc_add_cells(hc_ctx* ctx, int type_id, int ncells, const double* xyz, 
            "hipMemcpy");
   }
   for (int c = 0; c < ncells; ++c) {
+    double ctr[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < nvt; ++k)
+      for (int a = 0; a < 3; ++a) ctr[a] += xyz[3 * ((int64_t)c * nvt + k) + a];
+    for (int a = 0; a < 3; ++a) ctx->cell_ctr.push_back(ctr[a] / nvt);
     ctx->cell_base.push_back(ctx->nv + (int64_t)c * nvt);
     ctx->cell_type.push_back(type_id);
     ctx->cell_id.push_back(cell_ids ? cell_ids[c] : (int32_t)ctx->cell_id.size());
