@@ -62,7 +62,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   extern __shared__ __align__(16) double sm[];
   __shared__ double red[BS];
   __shared__ double vforce_sh;
-  __shared__ int bmin[3], bmax[3], bad;
+  __shared__ int bmin[3], bmax[3], bad, farpk;
   __shared__ int pcount[kMaxPlanes + 1], pstart[kMaxPlanes + 1];
   const int tid = threadIdx.x;
   const int nv = T.nv;
@@ -102,10 +102,27 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     by = reinterpret_cast<P*>(scratch + vs + base);
     bz = reinterpret_cast<P*>(scratch + 2 * vs + base);
   }
-  // absolute position of vertex k for the spread: LDS (double) or the vertex planes
-  auto pos_x = [&](int k) -> double { return REL ? V.x[base + k] : (double)px[k]; };
-  auto pos_y = [&](int k) -> double { return REL ? V.x[vs + base + k] : (double)py[k]; };
-  auto pos_z = [&](int k) -> double { return REL ? V.x[2 * vs + base + k] : (double)pz[k]; };
+  // Absolute position of vertex k for the spread: LDS (double), or with float
+  // offsets in LDS (REL) a packed copy that phase 3 writes over the offsets once
+  // the mechanics is done: per axis the node floor(x) relative to the origin's
+  // node (8 bits) and x - floor(x) in 24-bit fixed point, which rebuilds floor(x)
+  // exactly and the fraction to 2^-24 (a float's resolution near 1) without
+  // rereading the vertex planes; the planes when a cell spans more than 127 nodes.
+  bool use_pk = false;
+  const uint32_t* const pk = reinterpret_cast<const uint32_t*>(sp_);
+  const double fo0 = floor(org0), fo1 = floor(org1), fo2 = floor(org2);
+  auto unpack = [&](double fo, uint32_t w) -> double {
+    return fo + (double)((int)(w >> 24) - 128) + (double)(w & 0xFFFFFFu) * (1.0 / 16777216.0);
+  };
+  auto pos_x = [&](int k) -> double {
+    return REL ? (use_pk ? unpack(fo0, pk[k]) : V.x[base + k]) : (double)px[k];
+  };
+  auto pos_y = [&](int k) -> double {
+    return REL ? (use_pk ? unpack(fo1, pk[nv + k]) : V.x[vs + base + k]) : (double)py[k];
+  };
+  auto pos_z = [&](int k) -> double {
+    return REL ? (use_pk ? unpack(fo2, pk[2 * nv + k]) : V.x[2 * vs + base + k]) : (double)pz[k];
+  };
   double* box = region;
   // after the box: the spread's vertex list, bucketed by x-plane
   uint16_t* vlist = reinterpret_cast<uint16_t*>(region + 3 * (int64_t)box_nodes);
@@ -267,21 +284,31 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       bmax[a] = -0x7fffffff;
     }
     bad = 0;
+    farpk = 0;
   }
-  __syncthreads();
+  __syncthreads();  // also: the mechanics no longer reads the offsets
   {
     int lmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, lmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
-    bool lbad = false;
+    bool lbad = false, lfar = false;
+    uint32_t* const pkw = reinterpret_cast<uint32_t*>(sp_);
+    const double fo[3] = {fo0, fo1, fo2};
     for (int k = tid; k < nv; k += BS) {
-      const double p[3] = {pos_x(k), pos_y(k), pos_z(k)};
+      const double p[3] = {pos_x(k), pos_y(k), pos_z(k)};  // use_pk is still false here
       for (int a = 0; a < 3; ++a) {
         if (!(fabs(p[a]) < 1.0e9)) {
           lbad = true;
           continue;
         }
-        const int cc = (int)floor(p[a]);
+        const double fl = floor(p[a]);
+        const int cc = (int)fl;
         lmin[a] = min(lmin[a], cc);
         lmax[a] = max(lmax[a], cc);
+        if (REL) {
+          const double rel = fl - fo[a];
+          if (!(rel >= -128.0 && rel <= 127.0)) lfar = true;
+          const uint32_t f24 = min((uint32_t)((p[a] - fl) * 16777216.0), 0xFFFFFFu);
+          pkw[a * nv + k] = ((uint32_t)((int)rel + 128) << 24) | f24;
+        }
       }
       const int64_t i = base + k;
       if (incomplete && (V.flag[i] & VFLAG_REMOVED)) continue;
@@ -312,8 +339,10 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       atomicMax(&bmax[a], lmax[a]);
     }
     if (lbad) atomicOr(&bad, 1);
+    if (lfar) atomicOr(&farpk, 1);
   }
   __syncthreads();
+  use_pk = REL && !bad && !farpk;
   const int o0 = bmin[0], o1 = bmin[1], o2 = bmin[2];
   const int s0 = bmax[0] - bmin[0] + 2, s1 = bmax[1] - bmin[1] + 2, s2 = bmax[2] - bmin[2] + 2;
   const int64_t plane = (int64_t)s1 * s2;
