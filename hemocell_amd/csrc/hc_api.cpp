@@ -1562,6 +1562,16 @@ hc_status hc_add_cell_type(hc_ctx* ctx, int model, int nv, int nt, const int32_t
     D.vtri4p = D.vedge4p = nullptr;
     D.vbendp = nullptr;
     D.vtri_aeq = D.vedge_l0 = D.vbend_n = nullptr;
+    D.cslot = D.cring = nullptr;
+    D.cbend = nullptr;
+    if (!T.cslot.empty() && !padded_off(ctx)) {
+      const uint32_t* pc = nullptr;
+      if ((s = upload(ctx, *tb, T.cslot, &pc)) != HC_OK) return s;
+      D.cslot = (const uint4*)pc;
+      if ((s = upload(ctx, *tb, T.cring, &pc)) != HC_OK) return s;
+      D.cring = (const uint4*)pc;
+      if ((s = upload(ctx, *tb, T.cbend, &D.cbend)) != HC_OK) return s;
+    }
     if (T.padded && !padded_off(ctx)) {
       if ((s = upload(ctx, *tb, T.vtri4p, &p4)) != HC_OK) return s;
       D.vtri4p = (const int4*)p4;

@@ -253,9 +253,14 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       const double vf = (volume - T.volume_eq) / T.volume_eq;
       vforce_sh = -T.k_volume * vf / fabs(MaxCellVolumetricChange - vf * vf);
     }
+    // FP32 RBC: the compact tables (one uint4 per triangle/edge slot)
+    constexpr bool kCompact = MODEL == HC_MODEL_RBC && std::is_same<P, float>::value;
+    const bool compact = kCompact && T.cslot != nullptr;
     if (MODEL != HC_MODEL_PLT) {
       for (int k = tid; k < nv; k += BS) {
-        const V3T<P> b = bending_force(c, T, k);
+        V3T<P> b;
+        if constexpr (kCompact) b = compact ? bending_force_c(c, T, k) : bending_force(c, T, k);
+        else b = bending_force(c, T, k);
         bx[k] = b.x;
         by[k] = b.y;
         bz[k] = b.z;
@@ -264,7 +269,11 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     __syncthreads();
     const P vforce = (P)vforce_sh;
     for (int k = tid; k < nv; k += BS) {
-      const V3T<P> F = vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
+      V3T<P> F;
+      if constexpr (kCompact)
+        F = compact ? vertex_force_rbc_c(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc)
+                    : vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
+      else F = vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
       const int64_t i = base + k;
       V.f[i] = (double)F.x;
       V.f[vs + i] = (double)F.y;

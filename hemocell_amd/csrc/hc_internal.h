@@ -44,6 +44,15 @@ struct TypeHost {
   // per padded slot: the rest area of its triangle, the rest length of its edge and
   // the ring size of its bending contributor (addresses known from the vertex alone)
   std::vector<double> vtri_aeq, vedge_l0, vbend_n;
+  // Compact copies for the FP32 RBC kernels (padded lists, nv < 65535): per vertex
+  // and slot j < kPadW one uint4 {corner0 | corner1 << 16, corner2 | edge partner
+  // << 16, rest area (float bits), rest length (float bits)} of its j-th triangle
+  // and j-th edge (0xFFFF: none); per bending slot j < kPadB the contributor i |
+  // its ring size << 16 (all ones: none); per vertex one uint4 {ring0 | ring1 <<
+  // 16, ring2 | ring3 << 16, ring4 | ring5 << 16, rest patch distance (float)}.
+  // Edges are stored from the vertex's side: the force of an edge on its second
+  // end, -F(b - a), equals F(a - b) bit for bit.
+  std::vector<uint32_t> cslot, cbend, cring;
   hc_model_params p{};
   int timescale = 1;
 };
@@ -62,6 +71,8 @@ struct TypeDev {
   const int4 *vtri4p, *vedge4p;  // fixed-width copies (null when a list does not fit)
   const int32_t* vbendp;
   const double *vtri_aeq, *vedge_l0, *vbend_n;
+  const uint4 *cslot, *cring;  // compact FP32 tables (null when not built)
+  const uint32_t* cbend;
   double volume_eq, area_mean_eq, edge_mean_eq;
   double k_link, k_bend, k_volume, k_area, eta_m, k_inner_rigid, k_cyto, core_radius, radius;
 };

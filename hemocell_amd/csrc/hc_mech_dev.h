@@ -353,6 +353,164 @@ __device__ __forceinline__ V3T<M> vertex_force(const CellViewT<M>& c, const Type
   return {F0, F1, F2};
 }
 
+// The FP32 RBC kernels' bending and vertex forces over the compact tables
+// (TypeHost::cslot / cbend / cring): the terms, their order and their operations
+// are those of bending_force and vertex_force<HC_MODEL_RBC, TERM_ALL, float>; an
+// edge is taken from the vertex's side (its force on the second end, -F(b - a),
+// is F(a - b) bit for bit), so one uint4 per slot carries a triangle and an edge.
+__device__ __forceinline__ int ring_entry(const uint4& r, int j) {
+  const uint32_t w = j < 2 ? r.x : (j < 4 ? r.y : r.z);
+  return (int)((w >> (16 * (j & 1))) & 0xFFFFu);
+}
+
+__device__ V3T<float> bending_force_c(const CellViewT<float>& c, const TypeDev& T, int i) {
+  using M = float;
+  const uint4 rq = T.cring[i];
+  int R[6];
+  int nr = 0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    R[j] = ring_entry(rq, j);
+    nr += R[j] != 0xFFFF;
+  }
+  const M patch_eq = __uint_as_float(rq.w);
+  M s0 = M(0), s1 = M(0), s2 = M(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (j >= nr) break;
+    V3T<M> p = c.pos(R[j]);
+    s0 += p.x;
+    s1 += p.y;
+    s2 += p.z;
+  }
+  const V3T<M> xi = c.pos(i);
+  const M inv_n = mrcp((M)(unsigned)nr);
+  const V3T<M> dev = {s0 * inv_n - xi.x, s1 * inv_n - xi.y, s2 * inv_n - xi.z};
+  V3T<M> pn = {M(0), M(0), M(0)};
+  V3T<M> prev = c.pos(R[0]);
+  const V3T<M> first = prev;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (j >= nr) break;
+    const V3T<M> nxt = (j + 1 < nr) ? c.pos(R[j + 1 < 6 ? j + 1 : 0]) : first;
+    V3T<M> tn = crossp(prev - xi, nxt - xi);
+    const M il = mrcp(normp(tn));
+    pn.x += tn.x * il;
+    pn.y += tn.y * il;
+    pn.z += tn.z * il;
+    prev = nxt;
+  }
+  const M il = mrcp(normp(pn));
+  pn.x *= il;
+  pn.y *= il;
+  pn.z *= il;
+  const M ndev = dotp(pn, dev);
+  const M dDev = mdiv(ndev - patch_eq, (M)T.edge_mean_eq);
+  const M s = (M)T.k_bend * (dDev + mdiv(dDev, mabs(M(MaxCellBendingAngle) - dDev * dDev)));
+  return {pn.x * s, pn.y * s, pn.z * s};
+}
+
+__device__ __forceinline__ V3T<float> vertex_force_rbc_c(const CellViewT<float>& c, const TypeDev& T, int k,
+                                                          float vforce, const float* bx, const float* by,
+                                                          const float* bz, const float* vx, const float* vy,
+                                                          const float* vz, bool visc) {
+  using M = float;
+  uint4 q[kPadW];
+  uint32_t bq[kPadB];
+#pragma unroll
+  for (int j = 0; j < kPadW; ++j) q[j] = T.cslot[kPadW * k + j];
+#pragma unroll
+  for (int j = 0; j < kPadB; ++j) bq[j] = T.cbend[kPadB * k + j];
+  M F0 = M(0), F1 = M(0), F2 = M(0);
+  const V3T<M> xk = c.pos(k);
+  {
+    M A0 = M(0), A1 = M(0), A2 = M(0), B0 = M(0), B1 = M(0), B2 = M(0);
+    const M inv_area_mean = mrcp((M)T.area_mean_eq);
+#pragma unroll
+    for (int j = 0; j < kPadW; ++j) {
+      if ((q[j].x & 0xFFFFu) == 0xFFFFu) continue;
+      const V3T<M> v0 = c.pos((int)(q[j].x & 0xFFFFu)), v1 = c.pos((int)(q[j].x >> 16)),
+                   v2 = c.pos((int)(q[j].y & 0xFFFFu));
+      const M a_eq = __uint_as_float(q[j].z);
+      V3T<M> n = crossp(v1 - v0, v2 - v0);
+      const M nn = normp(n);
+      M area = M(0);
+      if (nn != M(0)) {
+        area = M(0.5) * nn;
+        const M inn = mrcp(nn);
+        n.x *= inn;
+        n.y *= inn;
+        n.z *= inn;
+      } else {
+        n = {M(0), M(0), M(0)};
+      }
+      const M ar = mdiv(area - a_eq, a_eq);
+      const M afm = (M)T.k_area * (ar + mdiv(ar, mabs(M(MaxCellSurfaceAreaChange) - ar * ar)));
+      constexpr M third = M(1.0 / 3.0);
+      const M c0 = (v0.x + v1.x + v2.x) * third, c1 = (v0.y + v1.y + v2.y) * third,
+              c2 = (v0.z + v1.z + v2.z) * third;
+      A0 += (c0 - xk.x) * afm;
+      A1 += (c1 - xk.y) * afm;
+      A2 += (c2 - xk.z) * afm;
+      const M sc = area * inv_area_mean;
+      B0 += (vforce * n.x) * sc;
+      B1 += (vforce * n.y) * sc;
+      B2 += (vforce * n.z) * sc;
+    }
+    F0 = A0 + B0;
+    F1 = A1 + B1;
+    F2 = A2 + B2;
+  }
+#pragma unroll
+  for (int j = 0; j < kPadB; ++j) {
+    if (bq[j] == 0xFFFFFFFFu) continue;
+    const int i = (int)(bq[j] & 0xFFFFu);
+    const M b0 = bx[i], b1 = by[i], b2 = bz[i];
+    if (i == k) {
+      F0 += b0;
+      F1 += b1;
+      F2 += b2;
+    } else {
+      const M in = mrcp((M)(bq[j] >> 16));
+      F0 += -b0 * in;
+      F1 += -b1 * in;
+      F2 += -b2 * in;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kPadW; ++j) {
+    const int o = (int)(q[j].y >> 16);
+    if (o == 0xFFFF) continue;
+    const M l0 = __uint_as_float(q[j].w);
+    const V3T<M> ev = c.pos(o) - xk;
+    const M len = normp(ev);
+    const M il = mrcp(len);
+    const V3T<M> uv = {ev.x * il, ev.y * il, ev.z * il};
+    const M ef = mdiv(len - l0, l0);
+    const M fs = (M)T.k_link * (ef + mdiv(ef, mabs(M(MaxCellPersistenceLength) - ef * ef)));
+    F0 += uv.x * fs;
+    F1 += uv.y * fs;
+    F2 += uv.z * fs;
+    if (visc) {
+      const V3T<M> rv = {vx[o] - vx[k], vy[o] - vy[k], vz[o] - vz[k]};
+      const M pr = dotp(rv, uv);
+      const M eta = (M)T.eta_m;
+      V3T<M> fv = {(pr * uv.x) * eta, (pr * uv.y) * eta, (pr * uv.z) * eta};
+      const M m = normp(fv);
+      if (m > M(FORCE_LIMIT / 4.0)) {
+        const M s = mdiv(M(FORCE_LIMIT / 4.0), m);
+        fv.x *= s;
+        fv.y *= s;
+        fv.z *= s;
+      }
+      F0 += fv.x;
+      F1 += fv.y;
+      F2 += fv.z;
+    }
+  }
+  return {F0, F1, F2};
+}
+
 // Sum of the signed triple products of the cell's triangles handled by thread tid
 // (the cell volume is 1/6 of the sum over all threads).
 template <int BS, typename M>

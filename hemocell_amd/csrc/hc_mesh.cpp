@@ -7,6 +7,7 @@
 // ellipsoid, de-duplicated into an indexed mesh and inflated by 1e-3 lattice
 // units along the vertex normals.  The constant tables follow
 // mechanics/commonCellConstants.cpp:70-409 and helper/meshMetrics.h:127-184.
+#include <cstring>
 #include <array>
 #include <cmath>
 #include <map>
@@ -437,6 +438,53 @@ bool build_type_tables(TypeHost& T, std::string& err) {
         T.vbendp[(size_t)v * kPadB + j] = T.vbend[e];
         T.vbend_n[(size_t)v * kPadB + j] = (double)(unsigned)T.nring[T.vbend[e]];
       }
+    }
+  }
+  T.cslot.clear();
+  T.cbend.clear();
+  T.cring.clear();
+  if (T.padded && T.model == HC_MODEL_RBC && nv < 0xFFFF) {
+    auto fbits = [](double x) {
+      const float f = (float)x;
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      return u;
+    };
+    T.cslot.assign((size_t)nv * kPadW * 4, 0);
+    T.cbend.assign((size_t)nv * kPadB, 0xFFFFFFFFu);
+    T.cring.assign((size_t)nv * 4, 0);
+    for (int v = 0; v < nv; ++v) {
+      for (int j = 0; j < kPadW; ++j) {
+        uint32_t c0 = 0xFFFF, c1 = 0xFFFF, c2 = 0xFFFF, other = 0xFFFF;
+        double aeq = 1.0, l0 = 1.0;
+        const int et = T.vtri_off[v] + j;
+        if (et < T.vtri_off[v + 1]) {
+          const int32_t* q = &T.vtri4[4 * (size_t)et];
+          c0 = (uint32_t)q[1];
+          c1 = (uint32_t)q[2];
+          c2 = (uint32_t)q[3];
+          aeq = T.tri_area_eq[q[0]];
+        }
+        const int ee = T.vedge_off[v] + j;
+        if (ee < T.vedge_off[v + 1]) {
+          const int code = T.vedge[ee], ed = code >> 2, kind = code & 3;
+          other = (uint32_t)(kind == 0 ? T.edge[2 * ed + 1] : T.edge[2 * ed]);
+          l0 = T.edge_len_eq[ed];
+        }
+        uint32_t* r = &T.cslot[4 * ((size_t)v * kPadW + j)];
+        r[0] = c0 | (c1 << 16);
+        r[1] = c2 | (other << 16);
+        r[2] = fbits(aeq);
+        r[3] = fbits(l0);
+      }
+      for (int e = T.vbend_off[v], j = 0; e < T.vbend_off[v + 1]; ++e, ++j)
+        T.cbend[(size_t)v * kPadB + j] = (uint32_t)T.vbend[e] | ((uint32_t)T.nring[T.vbend[e]] << 16);
+      uint32_t rr[6];
+      for (int j = 0; j < 6; ++j) rr[j] = j < T.nring[v] ? (uint32_t)T.ring[6 * v + j] : 0xFFFFu;
+      T.cring[4 * (size_t)v + 0] = rr[0] | (rr[1] << 16);
+      T.cring[4 * (size_t)v + 1] = rr[2] | (rr[3] << 16);
+      T.cring[4 * (size_t)v + 2] = rr[4] | (rr[5] << 16);
+      T.cring[4 * (size_t)v + 3] = fbits(T.patch_dist_eq[v]);
     }
   }
   return true;
