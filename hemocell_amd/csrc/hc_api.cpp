@@ -378,6 +378,11 @@ bool padded_off(const hc_ctx*) {
   const char* e = std::getenv("HC_MECH_TABLES");
   return e && std::string(e) == "csr";
 }
+// tuning: HC_MECH_TABLES=compact keeps the FP32 RBC mechanics on the per-slot tables
+static bool ring_tables_off() {
+  const char* e = std::getenv("HC_MECH_TABLES");
+  return e && std::string(e) == "compact";
+}
 
 template <typename T>
 hc_status upload(hc_ctx* ctx, TypeBufs& tb, const std::vector<T>& h, const T** out) {
@@ -1564,6 +1569,14 @@ hc_status hc_add_cell_type(hc_ctx* ctx, int model, int nv, int nt, const int32_t
     D.vtri_aeq = D.vedge_l0 = D.vbend_n = nullptr;
     D.cslot = D.cring = nullptr;
     D.cbend = nullptr;
+    D.rpar = nullptr;
+    D.rnb = nullptr;
+    if (!T.rpar.empty() && !padded_off(ctx) && !ring_tables_off()) {
+      const float* pf = nullptr;
+      if ((s = upload(ctx, *tb, T.rpar, &pf)) != HC_OK) return s;
+      D.rpar = (const float4*)pf;
+      if ((s = upload(ctx, *tb, T.rnb, &D.rnb)) != HC_OK) return s;
+    }
     if (!T.cslot.empty() && !padded_off(ctx)) {
       const uint32_t* pc = nullptr;
       if ((s = upload(ctx, *tb, T.cslot, &pc)) != HC_OK) return s;

@@ -253,7 +253,8 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       const double vf = (volume - T.volume_eq) / T.volume_eq;
       vforce_sh = -T.k_volume * vf / fabs(MaxCellVolumetricChange - vf * vf);
     }
-    // FP32 RBC: the compact tables (one uint4 per triangle/edge slot)
+    // FP32 RBC: the ring-ordered tables, else the compact per-slot ones (both
+    // built beside the padded tables)
     constexpr bool kCompact = MODEL == HC_MODEL_RBC && std::is_same<P, float>::value;
     const bool compact = kCompact && T.cslot != nullptr;
     if (MODEL != HC_MODEL_PLT) {
@@ -271,8 +272,9 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     for (int k = tid; k < nv; k += BS) {
       V3T<P> F;
       if constexpr (kCompact)
-        F = compact ? vertex_force_rbc_c(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc)
-                    : vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
+        F = T.rpar ? vertex_force_rbc_r(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc)
+            : compact ? vertex_force_rbc_c(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc)
+                      : vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
       else F = vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
       const int64_t i = base + k;
       V.f[i] = (double)F.x;

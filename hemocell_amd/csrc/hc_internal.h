@@ -53,6 +53,13 @@ struct TypeHost {
   // Edges are stored from the vertex's side: the force of an edge on its second
   // end, -F(b - a), equals F(a - b) bit for bit.
   std::vector<uint32_t> cslot, cbend, cring;
+  // Ring-ordered tables for the FP32 RBC staged kernel, built when every vertex's
+  // triangles are (v, ring[j], ring[j+1]) (cyclic, one orientation per vertex), its
+  // edges go to ring[j] and its bending contributors are v and its ring: per vertex
+  // 12 floats (rest areas of triangles j = 0..5, then rest lengths of edges j) and
+  // one word: ring size | reversed orientation << 3 | ring size of ring[j] << (4 + 3j).
+  std::vector<float> rpar;
+  std::vector<uint32_t> rnb;
   hc_model_params p{};
   int timescale = 1;
 };
@@ -73,6 +80,8 @@ struct TypeDev {
   const double *vtri_aeq, *vedge_l0, *vbend_n;
   const uint4 *cslot, *cring;  // compact FP32 tables (null when not built)
   const uint32_t* cbend;
+  const float4* rpar;  // ring-ordered FP32 tables (null when not built)
+  const uint32_t* rnb;
   double volume_eq, area_mean_eq, edge_mean_eq;
   double k_link, k_bend, k_volume, k_area, eta_m, k_inner_rigid, k_cyto, core_radius, radius;
 };

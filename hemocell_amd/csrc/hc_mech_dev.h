@@ -511,6 +511,105 @@ __device__ __forceinline__ V3T<float> vertex_force_rbc_c(const CellViewT<float>&
   return {F0, F1, F2};
 }
 
+// The FP32 RBC kernels' vertex force over the ring-ordered tables (TypeHost::rpar
+// / rnb): the terms of vertex_force<HC_MODEL_RBC>, summed around the 1-ring.
+// Triangle j is (k, ring[j], ring[j+1]) and edge j is (k, ring[j]), so each
+// neighbour's position is read once and enters as its offset d_j from vertex k:
+// the triangle's centroid offset is (d_j + d_j+1) / 3 and its unit normal times
+// its area is half the cross product.  The sums run in ring order (own bending
+// force first) rather than in the reference's triangle, contributor and edge
+// order: the same force up to FP32 rounding.
+__device__ __forceinline__ V3T<float> vertex_force_rbc_r(const CellViewT<float>& c, const TypeDev& T, int k,
+                                                          float vforce, const float* bx, const float* by,
+                                                          const float* bz, const float* vx, const float* vy,
+                                                          const float* vz, bool visc) {
+  using M = float;
+  const uint4 rq = T.cring[k];
+  const uint32_t nbw = T.rnb[k];
+  const float4 q0 = T.rpar[3 * k], q1 = T.rpar[3 * k + 1], q2 = T.rpar[3 * k + 2];
+  const M aeq[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+  const M l0[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+  const int nr = (int)(nbw & 7u);  // 3..6
+  const bool rev = (nbw & 8u) != 0;
+  const V3T<M> xk = c.pos(k);
+  int R[6];
+  V3T<M> d[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    R[j] = ring_entry(rq, j);
+    if (j < 3 || j < nr) d[j] = c.pos(R[j]) - xk;
+    else d[j] = {M(0), M(0), M(0)};
+  }
+  M F0, F1, F2;
+  {
+    M A0 = M(0), A1 = M(0), A2 = M(0), B0 = M(0), B1 = M(0), B2 = M(0);
+    const M half_vf = M(0.5) * vforce * mrcp((M)T.area_mean_eq);
+    constexpr M third = M(1.0 / 3.0);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      if (j >= 3 && j >= nr) break;
+      const V3T<M> a = d[j];
+      V3T<M> b = d[j < 5 ? j + 1 : 0];
+      if (j + 1 >= nr) b = d[0];
+      const V3T<M> cr = rev ? crossp(b, a) : crossp(a, b);
+      const M area = M(0.5) * normp(cr);
+      const M ar = mdiv(area - aeq[j], aeq[j]);
+      const M afm = (M)T.k_area * (ar + mdiv(ar, mabs(M(MaxCellSurfaceAreaChange) - ar * ar)));
+      A0 += ((a.x + b.x) * third) * afm;
+      A1 += ((a.y + b.y) * third) * afm;
+      A2 += ((a.z + b.z) * third) * afm;
+      B0 += cr.x * half_vf;
+      B1 += cr.y * half_vf;
+      B2 += cr.z * half_vf;
+    }
+    F0 = A0 + B0;
+    F1 = A1 + B1;
+    F2 = A2 + B2;
+  }
+  F0 += bx[k];
+  F1 += by[k];
+  F2 += bz[k];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (j >= 3 && j >= nr) break;
+    const M in = mrcp((M)((nbw >> (4 + 3 * j)) & 7u));
+    F0 += -bx[R[j]] * in;
+    F1 += -by[R[j]] * in;
+    F2 += -bz[R[j]] * in;
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (j >= 3 && j >= nr) break;
+    const V3T<M> ev = d[j];
+    const M len = normp(ev);
+    const M il = mrcp(len);
+    const V3T<M> uv = {ev.x * il, ev.y * il, ev.z * il};
+    const M ef = mdiv(len - l0[j], l0[j]);
+    const M fs = (M)T.k_link * (ef + mdiv(ef, mabs(M(MaxCellPersistenceLength) - ef * ef)));
+    F0 += uv.x * fs;
+    F1 += uv.y * fs;
+    F2 += uv.z * fs;
+    if (visc) {
+      const int o = R[j];
+      const V3T<M> rv = {vx[o] - vx[k], vy[o] - vy[k], vz[o] - vz[k]};
+      const M pr = dotp(rv, uv);
+      const M eta = (M)T.eta_m;
+      V3T<M> fv = {(pr * uv.x) * eta, (pr * uv.y) * eta, (pr * uv.z) * eta};
+      const M m = normp(fv);
+      if (m > M(FORCE_LIMIT / 4.0)) {
+        const M s = mdiv(M(FORCE_LIMIT / 4.0), m);
+        fv.x *= s;
+        fv.y *= s;
+        fv.z *= s;
+      }
+      F0 += fv.x;
+      F1 += fv.y;
+      F2 += fv.z;
+    }
+  }
+  return {F0, F1, F2};
+}
+
 // Sum of the signed triple products of the cell's triangles handled by thread tid
 // (the cell volume is 1/6 of the sum over all threads).
 template <int BS, typename M>

@@ -487,6 +487,86 @@ bool build_type_tables(TypeHost& T, std::string& err) {
       T.cring[4 * (size_t)v + 3] = fbits(T.patch_dist_eq[v]);
     }
   }
+  T.rpar.clear();
+  T.rnb.clear();
+  if (!T.cring.empty()) {
+    std::vector<float> par((size_t)nv * 12, 1.0f);
+    std::vector<uint32_t> nb((size_t)nv, 0);
+    bool ok = true;
+    for (int v = 0; v < nv && ok; ++v) {
+      const int nr = T.nring[v];
+      const int32_t* R = &T.ring[6 * (size_t)v];
+      ok = nr >= 3 && nr <= 6;
+      auto slot = [&](int32_t a) {
+        for (int j = 0; j < nr; ++j)
+          if (R[j] == a) return j;
+        return -1;
+      };
+      int orient = 0;
+      unsigned seen_t = 0, seen_e = 0;
+      for (int e = T.vtri_off[v]; ok && e < T.vtri_off[v + 1]; ++e) {
+        const int32_t* q = &T.vtri4[4 * (size_t)e];
+        int r = 0;
+        while (r < 3 && q[1 + r] != v) ++r;
+        if (r == 3) {
+          ok = false;
+          break;
+        }
+        const int32_t a = q[1 + (r + 1) % 3], b = q[1 + (r + 2) % 3];
+        int j = slot(a), o = 1;
+        if (j < 0 || R[(j + 1) % nr] != b) {
+          j = slot(b);
+          o = -1;
+          if (j < 0 || R[(j + 1) % nr] != a) {
+            ok = false;
+            break;
+          }
+        }
+        if ((orient != 0 && orient != o) || (seen_t >> j & 1u)) ok = false;
+        orient = o;
+        seen_t |= 1u << j;
+        par[12 * (size_t)v + j] = (float)T.tri_area_eq[q[0]];
+      }
+      for (int e = T.vedge_off[v]; ok && e < T.vedge_off[v + 1]; ++e) {
+        const int code = T.vedge[e], ed = code >> 2, kind = code & 3;
+        const int32_t other = kind == 0 ? T.edge[2 * ed + 1] : (kind == 1 ? T.edge[2 * ed] : -1);
+        const int j = slot(other);
+        if (j < 0 || (seen_e >> j & 1u)) {
+          ok = false;
+          break;
+        }
+        seen_e |= 1u << j;
+        par[12 * (size_t)v + 6 + j] = (float)T.edge_len_eq[ed];
+      }
+      const unsigned all = (1u << nr) - 1u;
+      ok = ok && seen_t == all && seen_e == all && T.vbend_off[v + 1] - T.vbend_off[v] == nr + 1;
+      unsigned seen_b = 0;
+      bool self = false;
+      for (int e = T.vbend_off[v]; ok && e < T.vbend_off[v + 1]; ++e) {
+        const int32_t i = T.vbend[e];
+        if (i == v) {
+          self = true;
+          continue;
+        }
+        const int j = slot(i);
+        if (j < 0) ok = false;
+        else seen_b |= 1u << j;
+      }
+      ok = ok && self && seen_b == all;
+      if (!ok) break;
+      uint32_t w = (uint32_t)nr | (orient < 0 ? 8u : 0u);
+      for (int j = 0; j < nr; ++j) {
+        const int n = T.nring[R[j]];
+        if (n < 1 || n > 7) ok = false;
+        w |= (uint32_t)(n & 7) << (4 + 3 * j);
+      }
+      nb[v] = w;
+    }
+    if (ok) {
+      T.rpar.swap(par);
+      T.rnb.swap(nb);
+    }
+  }
   return true;
 }
 
