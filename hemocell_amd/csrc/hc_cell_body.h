@@ -34,6 +34,9 @@ namespace {
 enum { CELL_INTERP = 1, CELL_MECH = 2, CELL_SPREAD = 4, CELL_REPULSION = 8 };
 // x-planes of a cell's support box up to which the spread buckets its vertices
 constexpr int kMaxPlanes = 64;
+// 64-triangle chunks of a mesh whose volume is summed in an order independent of
+// the block size (meshes up to 8192 triangles)
+constexpr int kVolChunks = 128;
 
 // Scalar type of the positions in LDS and of the membrane mechanics: the FP32-
 // shifted form's staged kernels keep float offsets from the cell's first vertex
@@ -60,7 +63,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   // force field are FP32), FP64 otherwise
   using SW = typename std::conditional<std::is_same<R, float>::value, float, double>::type;
   extern __shared__ __align__(16) double sm[];
-  __shared__ double red[BS];
+  __shared__ double red[kVolChunks];
   __shared__ double vforce_sh;
   __shared__ int bmin[3], bmax[3], bad, farpk;
   __shared__ int pcount[kMaxPlanes + 1], pstart[kMaxPlanes + 1];
@@ -69,14 +72,16 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   const int64_t vs = V.vs;
   const bool visc = (MODEL != HC_MODEL_RBC) || (T.eta_m != 0.0);
   // origin of the offsets: the first vertex before this step's advance, read by
-  // every thread before any thread moves it
+  // every thread before any thread moves it (its advanced position is stored
+  // only after the barrier that ends phase 1)
   double org0 = 0.0, org1 = 0.0, org2 = 0.0;
   if (REL) {
     org0 = V.x[base];
     org1 = V.x[vs + base];
     org2 = V.x[2 * vs + base];
-    __syncthreads();
   }
+  __shared__ double held_sh[3];  // REL: vertex 0's advanced position until then
+  bool held = false;
   // LDS: positions, then (STAGED) velocities, then a region shared by the bending
   // forces (phase 2) and the spread box (phase 3)
   P* const sp_ = reinterpret_cast<P*>(sm);
@@ -197,9 +202,16 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
         V.v[2 * vs + i] = v2;
       }
       const double x0 = xa[u][0] + v0, x1 = xa[u][1] + v1, x2 = xa[u][2] + v2;
-      V.x[i] = x0;
-      V.x[vs + i] = x1;
-      V.x[2 * vs + i] = x2;
+      if (REL && k == 0) {
+        held = true;
+        held_sh[0] = x0;
+        held_sh[1] = x1;
+        held_sh[2] = x2;
+      } else {
+        V.x[i] = x0;
+        V.x[vs + i] = x1;
+        V.x[2 * vs + i] = x2;
+      }
       if (!GPOS) {
         px[k] = (P)(x0 - org0);
         py[k] = (P)(x1 - org1);
@@ -230,7 +242,35 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     }
   }
   const bool incomplete = __syncthreads_or(lrem) != 0;
+  if (held) {
+    V.x[base] = held_sh[0];
+    V.x[vs + base] = held_sh[1];
+    V.x[2 * vs + base] = held_sh[2];
+  }
   const CellViewT<P> c{px, py, pz};
+
+  // The force a vertex spreads: its force plus any extra force, capped to f_limit
+  // (hemoCellParticleField.cpp:846-852).  Applied where the mechanics stores the
+  // force when it ran on this cell, else by the spread.
+  auto cap_force = [&](int64_t i, double& f0, double& f1, double& f2) -> bool {
+    bool changed = false;
+    if (extra) {
+      f0 += extra[i];
+      f1 += extra[vs + i];
+      f2 += extra[2 * vs + i];
+      changed = true;
+    }
+    const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+    if (fm > f_limit) {
+      const double sc = f_limit / fm;
+      f0 *= sc;
+      f1 *= sc;
+      f2 *= sc;
+      changed = true;
+    }
+    return changed;
+  };
+  const bool capped = (flags & CELL_MECH) && !incomplete;
 
   // ---- 2. mechanics ------------------------------------------------------------
   if ((flags & CELL_MECH) && incomplete) {
@@ -242,14 +282,34 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     }
     __syncthreads();
   } else if (flags & CELL_MECH) {
-    red[tid] = volume_partial<BS>(c, T, tid);
-    __syncthreads();
-    for (int h = BS / 2; h > 0; h >>= 1) {
-      if (tid < h) red[tid] += red[tid + h];
-      __syncthreads();
+    // the volume: triangle t is lane t % 64 of chunk t / 64; a wavefront sums a
+    // chunk by a butterfly over its lanes and thread 0 adds the chunks in order,
+    // so the sum is the same at every block size (batched, mixed and alone
+    // launches agree bit for bit); larger meshes sum per thread, then per
+    // wavefront
+    const int nchunk = (T.nt + 63) >> 6;
+    int nred = nchunk;
+    if (nchunk <= kVolChunks) {
+      const V3T<P> rm = c.pos(0);
+      const D3 r = {(double)rm.x, (double)rm.y, (double)rm.z};
+      for (int ch = tid >> 6; ch < nchunk; ch += BS / 64) {
+        double part = volume_term(c, T, (ch << 6) | (tid & 63), r);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+        if ((tid & 63) == 0) red[ch] = part;
+      }
+    } else {
+      double part = volume_partial<BS>(c, T, tid);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+      if ((tid & 63) == 0) red[tid >> 6] = part;
+      nred = BS / 64;
     }
+    __syncthreads();
     if (tid == 0) {
-      const double volume = red[0] * (1.0 / 6.0);
+      double vsum = red[0];
+      for (int w = 1; w < nred; ++w) vsum += red[w];
+      const double volume = vsum * (1.0 / 6.0);
       const double vf = (volume - T.volume_eq) / T.volume_eq;
       vforce_sh = -T.k_volume * vf / fabs(MaxCellVolumetricChange - vf * vf);
     }
@@ -277,9 +337,11 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
                       : vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
       else F = vertex_force<MODEL, TERM_ALL, P>(c, T, k, vforce, bx, by, bz, vx, vy, vz, visc);
       const int64_t i = base + k;
-      V.f[i] = (double)F.x;
-      V.f[vs + i] = (double)F.y;
-      V.f[2 * vs + i] = (double)F.z;
+      double f0 = (double)F.x, f1 = (double)F.y, f2 = (double)F.z;
+      if (flags & CELL_SPREAD) cap_force(i, f0, f1, f2);
+      V.f[i] = f0;
+      V.f[vs + i] = f1;
+      V.f[2 * vs + i] = f2;
     }
     __syncthreads();  // forces visible to the whole workgroup; the region is free again
   }
@@ -288,7 +350,8 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   // ---- 3. spread of the next step ---------------------------------------------
   // the force each vertex spreads: its force plus any extra force, capped to
   // f_limit and written back in place when that changed it
-  // (hemoCellParticleField.cpp:846-852), plus the repulsion force when enabled
+  // (hemoCellParticleField.cpp:846-852; done by the mechanics above when it ran
+  // on this cell), plus the repulsion force when enabled
   if (tid == 0) {
     for (int a = 0; a < 3; ++a) {
       bmin[a] = 0x7fffffff;
@@ -322,24 +385,9 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
         }
       }
       const int64_t i = base + k;
-      if (incomplete && (V.flag[i] & VFLAG_REMOVED)) continue;
+      if (capped || (incomplete && (V.flag[i] & VFLAG_REMOVED))) continue;
       double f0 = V.f[i], f1 = V.f[vs + i], f2 = V.f[2 * vs + i];
-      bool store = false;
-      if (extra) {
-        f0 += extra[i];
-        f1 += extra[vs + i];
-        f2 += extra[2 * vs + i];
-        store = true;
-      }
-      const double fm = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
-      if (fm > f_limit) {
-        const double sc = f_limit / fm;
-        f0 *= sc;
-        f1 *= sc;
-        f2 *= sc;
-        store = true;
-      }
-      if (store) {
+      if (cap_force(i, f0, f1, f2)) {
         V.f[i] = f0;
         V.f[vs + i] = f1;
         V.f[2 * vs + i] = f2;

@@ -610,6 +610,23 @@ __device__ __forceinline__ V3T<float> vertex_force_rbc_r(const CellViewT<float>&
   return {F0, F1, F2};
 }
 
+// Signed triple product of triangle t's corners relative to r (0 for t >= nt);
+// the cell volume is 1/6 of the sum over the triangles.
+template <typename M>
+__device__ __forceinline__ double volume_term(const CellViewT<M>& c, const TypeDev& T, int t, const D3& r) {
+  if (t >= T.nt) return 0.0;
+  const V3T<M> a = c.pos(T.tri[3 * t]), b = c.pos(T.tri[3 * t + 1]), e = c.pos(T.tri[3 * t + 2]);
+  const D3 v0 = D3{(double)a.x, (double)a.y, (double)a.z} - r, v1 = D3{(double)b.x, (double)b.y, (double)b.z} - r,
+           v2 = D3{(double)e.x, (double)e.y, (double)e.z} - r;
+  const double v210 = v2.x * v1.y * v0.z;
+  const double v120 = v1.x * v2.y * v0.z;
+  const double v201 = v2.x * v0.y * v1.z;
+  const double v021 = v0.x * v2.y * v1.z;
+  const double v102 = v1.x * v0.y * v2.z;
+  const double v012 = v0.x * v1.y * v2.z;
+  return -v210 + v120 + v201 - v021 - v102 + v012;
+}
+
 // Sum of the signed triple products of the cell's triangles handled by thread tid
 // (the cell volume is 1/6 of the sum over all threads).
 template <int BS, typename M>

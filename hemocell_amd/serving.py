@@ -76,8 +76,9 @@ def _cell_types(units):
     return _TYPES[key]
 
 
-def build(case, units, precision=hc.HC_FP32_SHIFTED, device=0):
-    """The context of one case, set up and ready to step."""
+def build(case, units, precision=hc.HC_FP32_SHIFTED, device=0, exact_spread=False):
+    """The context of one case, set up and ready to step (exact_spread: the
+    order-independent spread of the FP64 form, for bit-reproducible runs)."""
     E = case.edge
     types = _cell_types(units)
     t = types[case.cell]
@@ -102,6 +103,8 @@ def build(case, units, precision=hc.HC_FP32_SHIFTED, device=0):
         sim.lattice_init(E, E, E, units.tau, periodic=(0, 0, 0), node_type=mask.reshape(-1), precision=precision)
     sim.init_equilibrium(1.0, (0.0, 0.0, 0.0))
     sim.set_force_limit(units.f_limit)
+    if exact_spread:
+        sim.set_exact_spread(True)
     tid = sim.add_cell_type(t.model, t.verts, t.tris, t.params, t.inner)
     x = t.verts - 0.5 * (t.verts.min(axis=0) + t.verts.max(axis=0)) + (E - 1) / 2.0
     sim.add_cells(tid, x[None])
@@ -123,8 +126,8 @@ def summarise(sim):
                 bbox=[float(b) for b in st["bbox"][0]], centroid=[float(c) for c in st["centroid"][0]])
 
 
-def run_alone(case, units, precision=hc.HC_FP32_SHIFTED):
-    sim = build(case, units, precision)
+def run_alone(case, units, precision=hc.HC_FP32_SHIFTED, exact_spread=False):
+    sim = build(case, units, precision, exact_spread=exact_spread)
     sim.step(case.steps)
     sim.synchronize()
     r = summarise(sim)
@@ -142,7 +145,7 @@ def footprint(case, precision=hc.HC_FP32_SHIFTED):
 
 
 def serve(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SHIFTED, keep_vertices=False,
-          progress=None, hbm_budget=64 << 30, batched=True):
+          progress=None, hbm_budget=64 << 30, batched=True, exact_spread=False):
     """Run the stream in real time; returns the cases with latencies and results.
     Arrivals are admitted while fewer than max_active cases run and their
     footprints fit in hbm_budget bytes."""
@@ -160,7 +163,7 @@ def serve(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SHIFTED, 
             k += 1
             c.started = time.perf_counter() - t0
             used += footprint(c, precision)
-            active.append([c, build(c, units, precision), c.steps])
+            active.append([c, build(c, units, precision, exact_spread=exact_spread), c.steps])
         idle = True
         stepping = [a for a in active if a[2] > 0]
         if batched and stepping:

@@ -165,7 +165,11 @@ def test_stream_at_full_ranges_matches_alone():
     """The bench's stream generator at its full ranges (edge 32-96, 2000-20000
     steps, shear 10-1000 1/s or stretch 0-200 pN, RBC/PLT/WBC): every case of an
     8-case stream served by the batched loop ends as close to the case run alone
-    as two lone runs are to each other."""
+    as two lone runs are to each other, or within 1e-8.  The FP32 form's spread
+    box adds doubles in the order the wavefronts' LDS atomics land, so two lone
+    runs of the stretched RBC (case 4, 6925 steps) differ by 0 to 3e-9 from run
+    to run (measured); a batching mistake moves a case by far more.  The exact
+    spread removes that freedom: see the FP64 stream below."""
     print("[serving] full ranges", flush=True)
     u = hc.lbm_base_parameters(5e-7, 1e-7)
     stream = serving.make_cases(8, rate=100.0, seed=23)
@@ -179,7 +183,27 @@ def test_stream_at_full_ranges_matches_alone():
         mix = cases.rel_l2(c.result["x"], x)
         print("[serving] case %d (%s %s, edge %d, %d steps): mix %.2e, lone-vs-lone %.2e" % (
             c.ident, c.kind, c.cell, c.edge, c.steps, mix, noise))
-        assert mix <= 100 * max(noise, 1e-12), (c, mix, noise)
+        assert mix <= max(100 * noise, 1e-8), (c, mix, noise)
+
+
+def test_exact_spread_stream_is_bit_identical_to_alone():
+    """The same stream in the FP64 form with the exact spread, steps capped at
+    1200: every case served by the batched loop ends bit for bit where it ends
+    when run alone (the exact spread's sums do not depend on the order of the
+    adds, and the batch runs the same kernels per case)."""
+    import dataclasses
+    print("[serving] exact spread stream", flush=True)
+    u = hc.lbm_base_parameters(5e-7, 1e-7)
+    stream = [dataclasses.replace(c, steps=min(c.steps, 1200)) for c in serving.make_cases(8, rate=100.0, seed=23)]
+    done, _ = serving.serve(stream, u, max_active=8, chunk=250, keep_vertices=True, precision=hc.HC_FP64,
+                            exact_spread=True)
+    assert sorted(c.ident for c in done) == list(range(8))
+    for c in done:
+        _, x = serving.run_alone(c, u, precision=hc.HC_FP64, exact_spread=True)
+        mix = cases.rel_l2(c.result["x"], x)
+        print("[serving] exact case %d (%s %s, edge %d, %d steps): mix %.2e" % (
+            c.ident, c.kind, c.cell, c.edge, c.steps, mix))
+        assert np.array_equal(c.result["x"], x), (c.ident, mix)
 
 
 def _oracle_case(case, units):

@@ -5,8 +5,7 @@ with the kernel's static LDS, the spread box and the vertex list after it stay
 inside the dynamic allocation, the staged layout holds the positions,
 velocities and bending forces, and a layout that cannot fit is refused (the
 launcher then tries a smaller block) instead of launched.  Among the refused:
-the 2562-vertex WBC at 1024 threads on a 64 KB device, whose 8 KB reduction
-array once overflowed the workgroup's LDS."""
+the 2562-vertex WBC with its positions in LDS on a 64 KB device."""
 import ctypes
 import os
 import subprocess
@@ -37,11 +36,11 @@ def lib(tmp_path_factory):
 
 
 def static_lds(block):
-    """Static LDS of a k_cell_step instantiation: the block's double reduction
-    array, the volume force, the box bounds and the spread's plane counters,
-    rounded up to the 512-byte allocation granule (3072 B at 256 threads and 1536 B
-    at 64, as the profiler reports)."""
-    raw = 8 * block + 8 + 7 * 4 + 2 * 65 * 4
+    """Static LDS of a k_cell_step instantiation: 128 doubles of volume chunk
+    sums, the volume force, vertex 0's held position, the box bounds and the
+    spread's plane counters, rounded up to the 512-byte allocation granule
+    (2048 B at every block size)."""
+    raw = 8 * 128 + 8 + 24 + 8 * 4 + 2 * 65 * 4
     return (raw + 511) // 512 * 512
 
 
@@ -108,7 +107,7 @@ def test_a_launch_always_finds_a_block(lib, psize):
 
 def test_wbc_1024_threads_on_a_64k_device_is_refused(lib):
     p = plan(lib, WBC, 2562, 1, 8, 12288, 65536, 1024, 0)
-    assert not p["ok"]  # 2562 x 24 B positions + 8.7 KB static + list leave no room
+    assert not p["ok"]  # 2562 x 24 B positions + 1 KB static + list leave no room
     assert p["fixed"] + p["lists"] + p["st"] + MARGIN > 65536
     q = plan(lib, WBC, 2562, 1, 8, 12288, 163840, 1024, 0)
     assert q["ok"] and not q["staged"] and q["box_nodes"] == 512
