@@ -71,6 +71,15 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   const int nv = T.nv;
   const int64_t vs = V.vs;
   const bool visc = (MODEL != HC_MODEL_RBC) || (T.eta_m != 0.0);
+  // the spread's support box bounds, first used after the barrier ending phase 1
+  if (tid == 0) {
+    for (int a = 0; a < 3; ++a) {
+      bmin[a] = 0x7fffffff;
+      bmax[a] = -0x7fffffff;
+    }
+    bad = 0;
+    farpk = 0;
+  }
   // origin of the offsets: the first vertex before this step's advance, read by
   // every thread before any thread moves it (its advanced position is stored
   // only after the barrier that ends phase 1)
@@ -351,16 +360,8 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   // the force each vertex spreads: its force plus any extra force, capped to
   // f_limit and written back in place when that changed it
   // (hemoCellParticleField.cpp:846-852; done by the mechanics above when it ran
-  // on this cell), plus the repulsion force when enabled
-  if (tid == 0) {
-    for (int a = 0; a < 3; ++a) {
-      bmin[a] = 0x7fffffff;
-      bmax[a] = -0x7fffffff;
-    }
-    bad = 0;
-    farpk = 0;
-  }
-  __syncthreads();  // also: the mechanics no longer reads the offsets
+  // on this cell), plus the repulsion force when enabled.  The packed positions
+  // below overwrite the offsets: the mechanics ended with a barrier.
   {
     int lmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, lmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
     bool lbad = false, lfar = false;
@@ -399,6 +400,10 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     }
     if (lbad) atomicOr(&bad, 1);
     if (lfar) atomicOr(&farpk, 1);
+    // the spread box starts at zero (each slab's flush clears what it used), and
+    // so do the plane counts of the bucketing
+    for (int e = tid; e < 3 * box_nodes; e += BS) box[e] = 0.0;
+    for (int b = tid; b <= kMaxPlanes; b += BS) pcount[b] = 0;
   }
   __syncthreads();
   use_pk = REL && !bad && !farpk;
@@ -448,17 +453,25 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
   // visits only the vertices that reach it.
   const bool bucketed = sx < s0 && s0 <= kMaxPlanes && nv <= 65535;
   if (bucketed) {
-    for (int b = tid; b <= s0; b += BS) pcount[b] = 0;
-    __syncthreads();
     for (int k = tid; k < nv; k += BS) atomicAdd(&pcount[(int)floor(pos_x(k)) - o0], 1);
     __syncthreads();
-    if (tid == 0) {
-      int acc = 0;
-      for (int b = 0; b <= s0; ++b) {
-        const int c = pcount[b];
-        pstart[b] = acc;
-        pcount[b] = acc;  // becomes the insertion cursor
-        acc += c;
+    if (tid < 64) {
+      // exclusive prefix sum of the s0 + 1 <= 65 plane counts over wavefront 0;
+      // the counts become the insertion cursors
+      const int c = tid <= s0 ? pcount[tid] : 0;
+      int inc = c;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (tid >= o) inc += y;
+      }
+      if (tid <= s0) {
+        pstart[tid] = inc - c;
+        pcount[tid] = inc - c;
+      }
+      if (tid == 63 && s0 == 64) {
+        pstart[64] = inc;
+        pcount[64] = inc;
       }
     }
     __syncthreads();
@@ -470,11 +483,9 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     const int nxp = std::min(sx, s0 - xs);
     // the slab fits the box (plane <= box_nodes), so 32-bit indices suffice
     const int total = nxp * (int)plane;
-    for (int e = tid; e < 3 * total; e += BS) box[e] = 0.0;
     // the vertices to visit: a range of the bucketed list, or all of them
     const int first = bucketed ? pstart[max(xs - 1, 0)] : 0;
     const int count = bucketed ? pstart[min(xs + nxp, s0)] - first : nv;
-    __syncthreads();
     const int per = (count + 63) / 64;
     for (int k = tid; k < 64 * per; k += BS) {
       // lanes of one wavefront take vertices ~count/64 apart (fewer LDS same-address adds)
@@ -501,6 +512,9 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
     for (int e = tid; e < total; e += BS) {
       const double a = box[e], b = box[total + e], cz = box[2 * total + e];
       if (a == 0.0 && b == 0.0 && cz == 0.0) continue;
+      box[e] = 0.0;  // zero for the next slab, whose box is no larger
+      box[total + e] = 0.0;
+      box[2 * total + e] = 0.0;
       const unsigned r = (unsigned)e / us2;
       const int lz = (int)((unsigned)e - r * us2);
       const unsigned r2 = r / us1;
@@ -513,7 +527,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
       if (b != 0.0) atomic_add(&Fnext[S + n], (R)b);
       if (cz != 0.0) atomic_add(&Fnext[2 * S + n], (R)cz);
     }
-    __syncthreads();  // the box is zeroed again by the next slab
+    __syncthreads();  // the box is zero again for the next slab
   }
 }
 
