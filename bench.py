@@ -243,12 +243,12 @@ def support_nodes(x, shape):
 
 def whole_step(r):
     """Algorithmic HBM bytes of one step over the step time: collide-stream (164/328 B per
-    node), velocity field (22 reads + 4 writes per node) and the vertex state of the cell
+    node), velocity field (22 reads + 3 writes per node) and the vertex state of the cell
     kernel (x read, x, v and f written: 96 B per vertex).  `with_support` adds the IBM
     support traffic counted once per distinct support node touched in the step: the
     interpolation's 3 velocity reads and the spread's 3 force read-modify-writes."""
     rs = 4 if r["label"] == "fp32" else 8
-    b = (BYTES_PER_LUP[r["label"]] + 26 * rs) * r["nodes"] + 96 * r["vertices"]
+    b = (BYTES_PER_LUP[r["label"]] + 25 * rs) * r["nodes"] + 96 * r["vertices"]
     gbs = b / (r["ms_per_step"] * 1e-3) / 1e9
     bs = b + 9 * rs * r["support_nodes"]
     gbs_s = bs / (r["ms_per_step"] * 1e-3) / 1e9

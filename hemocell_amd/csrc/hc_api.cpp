@@ -1470,7 +1470,7 @@ hc_status hc_lattice_init(hc_ctx* ctx, int nx, int ny, int nz, const uint8_t* no
   if ((s = dalloc(ctx, &ctx->f, rs * 19 * ctx->L.stride, "populations")) != HC_OK) return s;
   for (int b = 0; b < 2; ++b)
     if ((s = dalloc(ctx, &ctx->Fibm[b], rs * 3 * ctx->L.stride, "force field")) != HC_OK) return s;
-  if ((s = dalloc(ctx, &ctx->ufield, rs * 4 * ctx->L.stride, "velocity field")) != HC_OK) return s;
+  if ((s = dalloc(ctx, &ctx->ufield, rs * 3 * ctx->L.stride, "velocity field")) != HC_OK) return s;
   HC_TRY(hipMemsetAsync(ctx->f, 0, rs * 19 * ctx->L.stride, ctx->stream), "hipMemsetAsync");
   for (int b = 0; b < 2; ++b) HC_TRY(hipMemsetAsync(ctx->Fibm[b], 0, rs * 3 * ctx->L.stride, ctx->stream), "hipMemsetAsync");
   if (has_walls) {

@@ -185,7 +185,7 @@ __device__ __forceinline__ void cell_step_body(const VertexDev& V, const TypeDev
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           if (sp.w[q] == SW(0)) continue;
-          const R* un = uf + 4 * (int64_t)sp.idx[q];
+          const R* un = uf + 3 * (int64_t)sp.idx[q];
           v0 += (SW)un[0] * sp.w[q];
           v1 += (SW)un[1] * sp.w[q];
           v2 += (SW)un[2] * sp.w[q];

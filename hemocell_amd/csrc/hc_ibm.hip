@@ -107,11 +107,10 @@ __device__ __forceinline__ void velocity_node_field(const R* __restrict__ f, con
   R rhoBar, j0, j1, j2;
   rhobar_j(fl, rhoBar, j0, j1, j2);
   const R invRho = R(1) / (rhoBar + R(1));
-  R* q = uf + 4 * (int64_t)n;  // interleaved: one vector load per support node
+  R* q = uf + 3 * (int64_t)n;  // interleaved: one 3-component load per support node
   q[0] = j0 * invRho + (R(g0) + Fcur[n]) / R(2);
   q[1] = j1 * invRho + (R(g1) + Fcur[S + n]) / R(2);
   q[2] = j2 * invRho + (R(g2) + Fcur[2 * S + n]) / R(2);
-  q[3] = R(0);
 }
 
 template <typename R, bool PHASE1, bool CHECK>
@@ -155,7 +154,7 @@ __global__ __launch_bounds__(BLOCK) void k_interp_advance(VertexDev V, const R* 
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       if (sp.w[k] == 0.0) continue;
-      const R* q = uf + 4 * (int64_t)sp.idx[k];
+      const R* q = uf + 3 * (int64_t)sp.idx[k];
       v0 += (double)q[0] * sp.w[k];
       v1 += (double)q[1] * sp.w[k];
       v2 += (double)q[2] * sp.w[k];

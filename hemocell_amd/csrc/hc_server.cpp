@@ -62,7 +62,7 @@ hc_status serr(hc_server* s, hc_status st, const std::string& m) {
 
 double footprint(const hc_case_spec& c) {
   const double e = (double)c.edge, r = c.precision == HC_FP32_SHIFTED ? 4.0 : 8.0;
-  return e * e * e * ((19 + 6 + 4) * r + 1) + (double)c.nv * 7 * 3 * 8;
+  return e * e * e * ((19 + 6 + 3) * r + 1) + (double)c.nv * 7 * 3 * 8;
 }
 
 // the context of one case, set up and ready to step

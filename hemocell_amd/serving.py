@@ -141,7 +141,7 @@ def footprint(case, precision=hc.HC_FP32_SHIFTED):
     field and the node types per node, plus the cell's vertex arrays."""
     s = 4 if precision == hc.HC_FP32_SHIFTED else 8
     nv = {"RBC": 642, "PLT": 66, "WBC": 642}[case.cell]
-    return case.edge ** 3 * ((19 + 6 + 4) * s + 1) + nv * 7 * 3 * 8
+    return case.edge ** 3 * ((19 + 6 + 3) * s + 1) + nv * 7 * 3 * 8
 
 
 def serve(cases, units, max_active=32, chunk=250, precision=hc.HC_FP32_SHIFTED, keep_vertices=False,
