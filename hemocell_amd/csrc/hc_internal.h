@@ -51,13 +51,14 @@ struct TypeHost {
   // its ring size << 16 (all ones: none); per vertex one uint4 {ring0 | ring1 <<
   // 16, ring2 | ring3 << 16, ring4 | ring5 << 16, rest patch distance (float)}.
   // Edges are stored from the vertex's side: the force of an edge on its second
-  // end, -F(b - a), equals F(a - b) bit for bit.
+  // end, -F(b - a), equals F(a - b) up to the rounding of fused multiply-adds.
   std::vector<uint32_t> cslot, cbend, cring;
   // Ring-ordered tables for the FP32 RBC staged kernel, built when every vertex's
-  // triangles are (v, ring[j], ring[j+1]) (cyclic, one orientation per vertex), its
-  // edges go to ring[j] and its bending contributors are v and its ring: per vertex
-  // 12 floats (rest areas of triangles j = 0..5, then rest lengths of edges j) and
-  // one word: ring size | reversed orientation << 3 | ring size of ring[j] << (4 + 3j).
+  // triangles are (v, ring[j], ring[j+1]) up to rotation (the ring is walked along
+  // the triangles' winding, so this holds for a closed, consistently wound mesh),
+  // its edges go to ring[j] and its bending contributors are v and its ring: per
+  // vertex 12 floats (rest areas of triangles j = 0..5, then rest lengths of edges
+  // j) and one word: ring size | ring size of ring[j] << (4 + 3j).
   std::vector<float> rpar;
   std::vector<uint32_t> rnb;
   hc_model_params p{};

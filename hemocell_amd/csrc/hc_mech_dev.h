@@ -357,7 +357,8 @@ __device__ __forceinline__ V3T<M> vertex_force(const CellViewT<M>& c, const Type
 // (TypeHost::cslot / cbend / cring): the terms, their order and their operations
 // are those of bending_force and vertex_force<HC_MODEL_RBC, TERM_ALL, float>; an
 // edge is taken from the vertex's side (its force on the second end, -F(b - a),
-// is F(a - b) bit for bit), so one uint4 per slot carries a triangle and an edge.
+// is F(a - b) in exact arithmetic; the multiply-adds the compiler fuses differ),
+// so one uint4 per slot carries a triangle and an edge.
 __device__ __forceinline__ int ring_entry(const uint4& r, int j) {
   const uint32_t w = j < 2 ? r.x : (j < 4 ? r.y : r.z);
   return (int)((w >> (16 * (j & 1))) & 0xFFFFu);
@@ -530,7 +531,6 @@ __device__ __forceinline__ V3T<float> vertex_force_rbc_r(const CellViewT<float>&
   const M aeq[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
   const M l0[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
   const int nr = (int)(nbw & 7u);  // 3..6
-  const bool rev = (nbw & 8u) != 0;
   const V3T<M> xk = c.pos(k);
   int R[6];
   V3T<M> d[6];
@@ -551,7 +551,7 @@ __device__ __forceinline__ V3T<float> vertex_force_rbc_r(const CellViewT<float>&
       const V3T<M> a = d[j];
       V3T<M> b = d[j < 5 ? j + 1 : 0];
       if (j + 1 >= nr) b = d[0];
-      const V3T<M> cr = rev ? crossp(b, a) : crossp(a, b);
+      const V3T<M> cr = crossp(a, b);
       const M area = M(0.5) * normp(cr);
       const M ar = mdiv(area - aeq[j], aeq[j]);
       const M afm = (M)T.k_area * (ar + mdiv(ar, mabs(M(MaxCellSurfaceAreaChange) - ar * ar)));

@@ -502,7 +502,6 @@ bool build_type_tables(TypeHost& T, std::string& err) {
           if (R[j] == a) return j;
         return -1;
       };
-      int orient = 0;
       unsigned seen_t = 0, seen_e = 0;
       for (int e = T.vtri_off[v]; ok && e < T.vtri_off[v + 1]; ++e) {
         const int32_t* q = &T.vtri4[4 * (size_t)e];
@@ -513,17 +512,11 @@ bool build_type_tables(TypeHost& T, std::string& err) {
           break;
         }
         const int32_t a = q[1 + (r + 1) % 3], b = q[1 + (r + 2) % 3];
-        int j = slot(a), o = 1;
-        if (j < 0 || R[(j + 1) % nr] != b) {
-          j = slot(b);
-          o = -1;
-          if (j < 0 || R[(j + 1) % nr] != a) {
-            ok = false;
-            break;
-          }
+        const int j = slot(a);
+        if (j < 0 || R[(j + 1) % nr] != b || (seen_t >> j & 1u)) {
+          ok = false;
+          break;
         }
-        if ((orient != 0 && orient != o) || (seen_t >> j & 1u)) ok = false;
-        orient = o;
         seen_t |= 1u << j;
         par[12 * (size_t)v + j] = (float)T.tri_area_eq[q[0]];
       }
@@ -554,7 +547,7 @@ bool build_type_tables(TypeHost& T, std::string& err) {
       }
       ok = ok && self && seen_b == all;
       if (!ok) break;
-      uint32_t w = (uint32_t)nr | (orient < 0 ? 8u : 0u);
+      uint32_t w = (uint32_t)nr;
       for (int j = 0; j < nr; ++j) {
         const int n = T.nring[R[j]];
         if (n < 1 || n > 7) ok = false;
